@@ -1,0 +1,159 @@
+/*
+ * psf.h -- C ABI of the MI355X reduced-space fluid step (libpsf_hip.so).
+ *
+ * The library replaces the per-frame hot path of the reference solver
+ * (Solver/SolverImpl/psfsolver.cpp:21-234 on the CPU and
+ *  Solver/SolverImpl/psfsolvergpu.cpp:216-414 + Resources/Simulation/simulation.cl
+ *  on OpenCL):
+ *   1. the reduced advection step  y_k = w^T C_k w, w += dt*y, energy
+ *      renormalisation, viscous decay, forcing       (psfsolver.cpp:34-59)
+ *   2. the face-flux reconstruction u = float(Phi w)  (psfsolver.cpp:60)
+ *   3. particle tracing by trilinear face-flux interpolation
+ *                                                     (psfsolver.cpp:70-232)
+ * Preprocessing (mesh, eigenbasis, structure tensor C) stays with the caller,
+ * as in the reference, where AbstractSolver's protected members are the data
+ * contract between buildLaplace/buildAdvection and integrate()
+ * (Solver/abstractsolver.h:94-126).
+ *
+ * Conventions
+ *  - every function returns 0 (PSF_OK) or a negative psf_status; the message
+ *    of the last failure on a context is psf_last_error(ctx).  Nothing exits.
+ *  - host arrays passed in are copied; device memory is owned by the library.
+ *  - work is queued on the context's stream; functions that return data to
+ *    the host synchronise that stream.  A context is used by one thread at a
+ *    time.
+ *  - array layouts are the reference's Eigen layouts (column-major).
+ */
+#ifndef PSF_PSF_H
+#define PSF_PSF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSF_ABI_VERSION 1
+
+#if defined(__GNUC__)
+#define PSF_API __attribute__((visibility("default")))
+#else
+#define PSF_API
+#endif
+
+typedef enum psf_status {
+    PSF_OK = 0,
+    PSF_ERR_ARG = -1,          /* invalid argument or shape */
+    PSF_ERR_HIP = -2,          /* HIP runtime failure */
+    PSF_ERR_NOMEM = -3,        /* device or host allocation failed */
+    PSF_ERR_STATE = -4,        /* call out of order (e.g. step before upload) */
+    PSF_ERR_UNSUPPORTED = -5,  /* combination not built (e.g. path/precision) */
+    PSF_ERR_NONFINITE = -6     /* the FP64 energy check saw 0, NaN or Inf */
+} psf_status;
+
+/* storage/arithmetic of the structure tensor C in the reduced step */
+typedef enum psf_c_precision {
+    PSF_C_FP64 = 0,    /* FP64 C, FP64 arithmetic (parity path) */
+    PSF_C_FP32 = 1,    /* FP32 C, FP32 products, FP64 reduction */
+    PSF_C_BF16 = 2,    /* BF16 C on MFMA, FP32 accumulation */
+    PSF_C_FP16 = 3,    /* FP16 C (scaled per mode) on MFMA, FP32 accumulation */
+    PSF_C_BF16X2 = 4   /* C = hi + lo, both BF16, on MFMA, FP32 accumulation */
+} psf_c_precision;
+
+/* storage of the basis Phi; the sum u_f = sum_k Phi[f][k] w_k is FP64 either way */
+typedef enum psf_phi_precision { PSF_PHI_FP64 = 0, PSF_PHI_FP32 = 1 } psf_phi_precision;
+
+/* how the reduced step is launched */
+typedef enum psf_step_path {
+    PSF_PATH_PLAIN = 0,       /* two launches per step: contraction, FP64 update */
+    PSF_PATH_PERSISTENT = 1   /* one launch for n steps, C kept on chip, one
+                                 device-wide exchange of y per step (MFMA modes) */
+} psf_step_path;
+
+typedef struct psf_ctx psf_ctx;
+typedef struct psf_scene psf_scene;
+
+/* Grid and basis description.  Counts follow DECMesh3D (DEC/decmesh3d.cpp:64-70):
+   F = (nz+1)nx ny + (ny+1)nx nz + (nx+1)ny nz faces, V = nx ny nz voxels. */
+typedef struct psf_scene_desc {
+    uint32_t nx, ny, nz;       /* voxel counts (DECMesh3D::getDimensions) */
+    double resolution;         /* AbstractSolver::resolution, the voxel size h */
+    float aabb_min[3];         /* Model::getAABB().min; the particle cell lookup
+                                  (psfsolver.cpp:80-89) assumes max == -min */
+    float aabb_max[3];
+    uint32_t K;                /* AbstractSolver::nEigenFunctions */
+    int32_t c_precision;       /* psf_c_precision */
+    int32_t phi_precision;     /* psf_phi_precision */
+    uint32_t max_particles;    /* AbstractSolver::maxParticles */
+} psf_scene_desc;
+
+/* ---- context --------------------------------------------------------------- */
+PSF_API int psf_ctx_create(int device, psf_ctx** out);
+PSF_API int psf_ctx_destroy(psf_ctx* ctx);
+PSF_API const char* psf_last_error(const psf_ctx* ctx);
+PSF_API int psf_ctx_sync(psf_ctx* ctx);
+/* the context's hipStream_t, for callers that order their own work after ours */
+PSF_API void* psf_ctx_stream(psf_ctx* ctx);
+PSF_API int psf_abi_version(void);
+
+/* ---- scene lifetime and uploads -------------------------------------------
+ * Replaces PSFSolverGPU::buildAdvection's upload tail
+ * (Solver/SolverImpl/psfsolvergpu.cpp:718-874). */
+PSF_API int psf_scene_create(psf_ctx* ctx, const psf_scene_desc* desc, psf_scene** out);
+PSF_API int psf_scene_destroy(psf_scene* scene);
+/* bytes of device memory the scene holds (admission by footprint) */
+PSF_API int psf_scene_footprint(const psf_scene_desc* desc, uint64_t* bytes);
+
+/* Phi: F x K column-major (AbstractSolver::velBasisField); lambda: K
+   (eigenValues); gravity: K (gravity = Phi^T f, psfsolver.cpp:344-358). */
+PSF_API int psf_scene_upload_basis(psf_scene* scene, const double* Phi, const double* lambda,
+                           const double* gravity);
+/* C: K matrices K x K, each column-major, matrix i at C + i*K*K
+   (std::vector<Eigen::MatrixXd> advection, abstractsolver.h:118). */
+PSF_API int psf_scene_upload_advection(psf_scene* scene, const double* C);
+/* Optional: the caller's DECMesh3D::getSignBitString() (decmesh3d.cpp:82-106,
+   1390-1399).  The tracer evaluates the sign bits in closed form; this call
+   checks the caller's table against that closed form and fails on mismatch. */
+PSF_API int psf_scene_check_signs(psf_scene* scene, const uint32_t* words, size_t nwords);
+
+/* setTimestep / setViscosity / setGravityActive / setLifeTime
+   (Solver/abstractsolver.cpp:226-250).  lifetime is the value stored in
+   AbstractSolver::lifeTime (frames). */
+PSF_API int psf_set_params(psf_scene* scene, double dt, double viscosity, int gravity_active, float lifetime);
+PSF_API int psf_set_step_path(psf_scene* scene, int path);
+
+/* basisCoeff (AbstractSolver::basisCoeff) */
+PSF_API int psf_set_omega(psf_scene* scene, const double* omega);
+PSF_API int psf_get_omega(psf_scene* scene, double* omega);
+
+/* particles: n records (x, y, z, w) as glm::vec4 (Graphics/Particle/Particle.h:14) */
+PSF_API int psf_set_particles(psf_scene* scene, const float* xyzw, uint32_t n);
+PSF_API int psf_get_particles(psf_scene* scene, float* xyzw, uint32_t n);
+
+/* ---- the hot path ----------------------------------------------------------- */
+/* nsteps reduced steps (psfsolver.cpp:34-59) on the scene's path */
+PSF_API int psf_step(psf_scene* scene, int nsteps);
+/* u = float(Phi w) on the device (psfsolver.cpp:60) */
+PSF_API int psf_reconstruct(psf_scene* scene);
+/* velocityField (AbstractSolver::velocityField), F floats */
+PSF_API int psf_get_velocity(psf_scene* scene, float* u);
+/* one pass of the particle loop with the current u (psfsolver.cpp:70-232);
+   cells (optional, host, 12 ints per particle) receives the clamped
+   xOfs,yOfs,zOfs,xOfs-,xOfs+,yOfs-,yOfs+,zOfs-,zOfs+ of advected particles
+   (-1 for respawned ones) */
+PSF_API int psf_trace(psf_scene* scene, int32_t* cells);
+/* nframes x AbstractSolver::integrate(): step, reconstruct, trace */
+PSF_API int psf_integrate(psf_scene* scene, int nframes);
+
+/* ---- measurement -----------------------------------------------------------
+ * Runs `count` launches of one phase back to back on the context stream
+ * between two HIP events and returns the elapsed milliseconds.
+ * phase: 0 reduced step (one step per launch), 1 reconstruction, 2 trace,
+ *        3 full integrate frame, 4 one launch of `count` reduced steps. */
+PSF_API int psf_time_phase(psf_scene* scene, int phase, int count, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSF_PSF_H */

@@ -1,0 +1,93 @@
+"""ctypes binding of libpsf_hip.so (include/psf/psf.h).
+
+The library is built in-tree by parallelspectralfluids3d_amd.build; loading
+fails loudly when it is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libpsf_hip.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "psf", "psf.h")
+
+PSF_OK, PSF_ERR_ARG, PSF_ERR_HIP, PSF_ERR_NOMEM, PSF_ERR_STATE, PSF_ERR_UNSUPPORTED, PSF_ERR_NONFINITE = \
+    0, -1, -2, -3, -4, -5, -6
+C_FP64, C_FP32, C_BF16, C_FP16, C_BF16X2 = 0, 1, 2, 3, 4
+PHI_FP64, PHI_FP32 = 0, 1
+PATH_PLAIN, PATH_PERSISTENT = 0, 1
+PHASE_STEP, PHASE_RECON, PHASE_TRACE, PHASE_FRAME, PHASE_NSTEP = 0, 1, 2, 3, 4
+
+_STATUS = {0: "PSF_OK", -1: "PSF_ERR_ARG", -2: "PSF_ERR_HIP", -3: "PSF_ERR_NOMEM", -4: "PSF_ERR_STATE",
+           -5: "PSF_ERR_UNSUPPORTED", -6: "PSF_ERR_NONFINITE"}
+
+
+class PsfError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class SceneDesc(ct.Structure):
+    _fields_ = [("nx", ct.c_uint32), ("ny", ct.c_uint32), ("nz", ct.c_uint32),
+                ("resolution", ct.c_double),
+                ("aabb_min", ct.c_float * 3), ("aabb_max", ct.c_float * 3),
+                ("K", ct.c_uint32), ("c_precision", ct.c_int32), ("phi_precision", ct.c_int32),
+                ("max_particles", ct.c_uint32)]
+
+
+_lib = None
+
+
+def load():
+    """Load libpsf_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m parallelspectralfluids3d_amd.build` "
+                           "(the HIP path has no fallback)")
+    L = ct.CDLL(LIB_PATH)
+    vp, i32, u32, dbl, flt = ct.c_void_p, ct.c_int, ct.c_uint32, ct.c_double, ct.c_float
+    pp = ct.POINTER(ct.c_void_p)
+    sig = {
+        "psf_abi_version": (i32, []),
+        "psf_ctx_create": (i32, [i32, pp]),
+        "psf_ctx_destroy": (i32, [vp]),
+        "psf_last_error": (ct.c_char_p, [vp]),
+        "psf_ctx_sync": (i32, [vp]),
+        "psf_ctx_stream": (vp, [vp]),
+        "psf_scene_create": (i32, [vp, ct.POINTER(SceneDesc), pp]),
+        "psf_scene_destroy": (i32, [vp]),
+        "psf_scene_footprint": (i32, [ct.POINTER(SceneDesc), ct.POINTER(ct.c_uint64)]),
+        "psf_scene_upload_basis": (i32, [vp, vp, vp, vp]),
+        "psf_scene_upload_advection": (i32, [vp, vp]),
+        "psf_scene_check_signs": (i32, [vp, vp, ct.c_size_t]),
+        "psf_set_params": (i32, [vp, dbl, dbl, i32, flt]),
+        "psf_set_step_path": (i32, [vp, i32]),
+        "psf_set_omega": (i32, [vp, vp]),
+        "psf_get_omega": (i32, [vp, vp]),
+        "psf_set_particles": (i32, [vp, vp, u32]),
+        "psf_get_particles": (i32, [vp, vp, u32]),
+        "psf_step": (i32, [vp, i32]),
+        "psf_reconstruct": (i32, [vp]),
+        "psf_get_velocity": (i32, [vp, vp]),
+        "psf_trace": (i32, [vp, vp]),
+        "psf_integrate": (i32, [vp, i32]),
+        "psf_time_phase": (i32, [vp, i32, i32, ct.POINTER(ct.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def declared_symbols():
+    """Names of every function include/psf/psf.h declares."""
+    import re
+    with open(HEADER) as fh:
+        txt = fh.read()
+    return sorted(set(re.findall(r"^PSF_API\s+[\w\s\*]*?\b(psf_\w+)\s*\(", txt, flags=re.M)))

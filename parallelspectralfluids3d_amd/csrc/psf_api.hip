@@ -1,0 +1,547 @@
+// C ABI of the psf HIP library (declared in include/psf/psf.h).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "psf_kernels.hpp"
+#include "psf_persistent.hpp"
+
+using namespace psf;
+
+namespace {
+
+int set_err(psf_ctx* ctx, int code, const char* fmt, ...)
+{
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return set_err((ctx), PSF_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+}
+
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+bool is_plain_precision(int p) { return p == PSF_C_FP64 || p == PSF_C_FP32; }
+
+size_t c_bytes(const psf_scene_desc& d)
+{
+    const size_t K = d.K;
+    switch (d.c_precision) {
+    case PSF_C_FP64: return K * K * K * 8;
+    case PSF_C_FP32: return K * K * K * 4;
+    default: return persistent_c_bytes(d.c_precision, static_cast<int>(d.K));
+    }
+}
+
+void counts(const psf_scene_desc& d, uint32_t& V, uint32_t& F, uint32_t& nZF, uint32_t& nYF, uint32_t& nXF)
+{
+    nXF = (d.nx + 1) * (d.ny * d.nz);
+    nYF = (d.ny + 1) * (d.nx * d.nz);
+    nZF = (d.nz + 1) * (d.nx * d.ny);
+    F = nXF + nYF + nZF;
+    V = d.nx * d.ny * d.nz;
+}
+
+int validate_desc(psf_ctx* ctx, const psf_scene_desc* d)
+{
+    if (!d) return set_err(ctx, PSF_ERR_ARG, "desc is null");
+    if (d->nx == 0 || d->ny == 0 || d->nz == 0 || d->nx > 2048 || d->ny > 2048 || d->nz > 2048)
+        return set_err(ctx, PSF_ERR_ARG, "grid dims out of range");
+    const double nf = (double)(d->nx + 1) * d->ny * d->nz + (double)(d->ny + 1) * d->nx * d->nz +
+                      (double)(d->nz + 1) * d->nx * d->ny;
+    if (nf > 2.0e9) return set_err(ctx, PSF_ERR_ARG, "too many faces");
+    if (d->K == 0 || d->K > 4096) return set_err(ctx, PSF_ERR_ARG, "K must be in [1, 4096]");
+    if (!(d->resolution > 0.0) || !std::isfinite(d->resolution)) return set_err(ctx, PSF_ERR_ARG, "resolution must be > 0");
+    if (d->c_precision < PSF_C_FP64 || d->c_precision > PSF_C_BF16X2)
+        return set_err(ctx, PSF_ERR_ARG, "unknown c_precision %d", d->c_precision);
+    if (d->phi_precision != PSF_PHI_FP64 && d->phi_precision != PSF_PHI_FP32)
+        return set_err(ctx, PSF_ERR_ARG, "unknown phi_precision %d", d->phi_precision);
+    for (int c = 0; c < 3; ++c)
+        if (!(d->aabb_max[c] > d->aabb_min[c])) return set_err(ctx, PSF_ERR_ARG, "aabb_max must exceed aabb_min");
+    return PSF_OK;
+}
+
+int check_status(psf_scene* s)
+{
+    int st = 0;
+    HIPCHK(s->ctx, hipMemcpyAsync(&st, s->dev.status, sizeof(int), hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    if (st) return set_err(s->ctx, PSF_ERR_NONFINITE, "energy renormalisation saw a zero or non-finite energy");
+    return PSF_OK;
+}
+
+TraceParams trace_params(const psf_scene* s)
+{
+    TraceParams p{};
+    p.nx = s->d.nx; p.ny = s->d.ny; p.nz = s->d.nz;
+    p.nZF = s->nZF; p.nYF = s->nYF;
+    for (int c = 0; c < 3; ++c) {
+        p.aabb_min[c] = s->d.aabb_min[c];
+        p.aabb_max[c] = s->d.aabb_max[c];
+        p.ext[c] = s->d.aabb_max[c] - s->d.aabb_min[c];
+    }
+    p.res = s->d.resolution;
+    p.hf = static_cast<float>(s->d.resolution);
+    p.inv_h = static_cast<float>(1.0 / s->d.resolution);
+    p.dtf = static_cast<float>(s->dt);
+    p.lifetime = s->lifetime;
+    p.center_z = s->d.aabb_min[2] + 0.5f * (s->d.aabb_max[2] - s->d.aabb_min[2]);
+    p.seed = s->seed;
+    p.frame = s->frame;
+    return p;
+}
+
+int ready_for_step(psf_scene* s)
+{
+    if (!s->has_C) return set_err(s->ctx, PSF_ERR_STATE, "advection tensor not uploaded");
+    if (!s->has_omega) return set_err(s->ctx, PSF_ERR_STATE, "omega not set");
+    if (!s->has_basis) return set_err(s->ctx, PSF_ERR_STATE, "basis not uploaded");
+    return PSF_OK;
+}
+
+int enqueue_steps(psf_scene* s, int n)
+{
+    const int K = static_cast<int>(s->d.K);
+    hipStream_t st = s->ctx->stream;
+    if (s->path == PSF_PATH_PERSISTENT) {
+        hipError_t e = launch_persistent_steps(s, n, st);
+        if (e != hipSuccess) return set_err(s->ctx, PSF_ERR_HIP, "persistent step launch: %s", hipGetErrorString(e));
+        return PSF_OK;
+    }
+    if (!is_plain_precision(s->d.c_precision))
+        return set_err(s->ctx, PSF_ERR_UNSUPPORTED, "c_precision %d runs on the persistent path only", s->d.c_precision);
+    for (int it = 0; it < n; ++it) {
+        HIPCHK(s->ctx, launch_contract_plain(s->d.c_precision, s->dev.C, s->dev.omega, s->dev.y, K, st));
+        HIPCHK(s->ctx, launch_update(s->dev.omega, s->dev.y, s->dev.visc, s->dev.gravity, K, s->dt, s->gravity_active,
+                                     s->dev.status, st));
+    }
+    return PSF_OK;
+}
+
+int enqueue_reconstruct(psf_scene* s)
+{
+    HIPCHK(s->ctx, launch_reconstruct(s->d.phi_precision, s->dev.Phi, s->dev.ldF, s->F, static_cast<int>(s->d.K),
+                                      s->dev.omega, s->dev.u, s->ctx->stream));
+    return PSF_OK;
+}
+
+int enqueue_trace(psf_scene* s, int32_t* dcells)
+{
+    TraceParams p = trace_params(s);
+    HIPCHK(s->ctx, launch_trace(p, s->dev.u, s->dev.parts, s->n_particles, dcells, s->ctx->stream));
+    s->frame++;
+    return PSF_OK;
+}
+
+int refresh_visc(psf_scene* s, const std::vector<double>& lam)
+{
+    // omega_k *= exp(viscosity * eigenValues(k) * timeStep)  (psfsolver.cpp:51-55)
+    std::vector<double> v(s->d.K);
+    for (uint32_t k = 0; k < s->d.K; ++k) v[k] = std::exp(s->nu * lam[k] * s->dt);
+    HIPCHK(s->ctx, hipMemcpyAsync(s->dev.visc, v.data(), sizeof(double) * s->d.K, hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    return PSF_OK;
+}
+
+}  // namespace
+
+// host copy of lambda per scene, kept beside the scene for viscosity refreshes
+struct psf_scene_host {
+    std::vector<double> lambda;
+};
+static psf_scene_host* host_of(psf_scene* s) { return reinterpret_cast<psf_scene_host*>(s->host); }
+
+extern "C" {
+
+int psf_abi_version(void) { return PSF_ABI_VERSION; }
+
+int psf_ctx_create(int device, psf_ctx** out)
+{
+    if (!out) return PSF_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return PSF_ERR_HIP;
+    if (device < 0 || device >= n) return PSF_ERR_ARG;
+    psf_ctx* c = new (std::nothrow) psf_ctx();
+    if (!c) return PSF_ERR_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return PSF_ERR_HIP;
+    }
+    *out = c;
+    return PSF_OK;
+}
+
+int psf_ctx_destroy(psf_ctx* ctx)
+{
+    if (!ctx) return PSF_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PSF_OK;
+}
+
+const char* psf_last_error(const psf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int psf_ctx_sync(psf_ctx* ctx)
+{
+    if (!ctx) return PSF_ERR_ARG;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return PSF_OK;
+}
+
+void* psf_ctx_stream(psf_ctx* ctx) { return ctx ? reinterpret_cast<void*>(ctx->stream) : nullptr; }
+
+int psf_scene_footprint(const psf_scene_desc* d, uint64_t* bytes)
+{
+    if (!bytes) return PSF_ERR_ARG;
+    int rc = validate_desc(nullptr, d);
+    if (rc) return rc;
+    uint32_t V, F, a, b, c;
+    counts(*d, V, F, a, b, c);
+    const size_t ldF = round_up(F, 256);
+    const size_t phi = ldF * d->K * (d->phi_precision == PSF_PHI_FP32 ? 4 : 8);
+    *bytes = c_bytes(*d) + phi + size_t(F) * 4 + size_t(d->max_particles) * (16 + 48) + size_t(d->K) * 8 * 6 + 256;
+    return PSF_OK;
+}
+
+int psf_scene_create(psf_ctx* ctx, const psf_scene_desc* desc, psf_scene** out)
+{
+    if (!ctx || !out) return set_err(ctx, PSF_ERR_ARG, "null argument");
+    *out = nullptr;
+    int rc = validate_desc(ctx, desc);
+    if (rc) return rc;
+    if (!is_plain_precision(desc->c_precision) && !persistent_supported(desc->c_precision, static_cast<int>(desc->K)))
+        return set_err(ctx, PSF_ERR_UNSUPPORTED, "c_precision %d with K=%u is not built", desc->c_precision, desc->K);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    psf_scene* s = new (std::nothrow) psf_scene();
+    psf_scene_host* h = new (std::nothrow) psf_scene_host();
+    if (!s || !h) {
+        delete s;
+        delete h;
+        return set_err(ctx, PSF_ERR_NOMEM, "host allocation failed");
+    }
+    s->ctx = ctx;
+    s->d = *desc;
+    s->host = h;
+    counts(*desc, s->V, s->F, s->nZF, s->nYF, s->nXF);
+    s->path = is_plain_precision(desc->c_precision) ? PSF_PATH_PLAIN : PSF_PATH_PERSISTENT;
+    const size_t K = desc->K;
+    s->dev.ldF = round_up(s->F, 256);
+    s->dev.C_bytes = c_bytes(*desc);
+    const size_t phi_bytes = s->dev.ldF * K * (desc->phi_precision == PSF_PHI_FP32 ? 4 : 8);
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    chk(dalloc(&s->dev.omega, K));
+    chk(dalloc(&s->dev.lambda, K));
+    chk(dalloc(&s->dev.gravity, K));
+    chk(dalloc(&s->dev.visc, K));
+    chk(dalloc(&s->dev.y, K));
+    chk(dalloc(&s->dev.status, 64));
+    chk(dalloc(reinterpret_cast<unsigned char**>(&s->dev.C), s->dev.C_bytes));
+    chk(dalloc(reinterpret_cast<unsigned char**>(&s->dev.Phi), phi_bytes));
+    chk(dalloc(&s->dev.u, s->F));
+    chk(dalloc(&s->dev.parts, desc->max_particles));
+    chk(dalloc(&s->dev.cells, size_t(desc->max_particles) * 12));
+    if (e == hipSuccess) e = persistent_scene_init(s);
+    if (e != hipSuccess) {
+        psf_scene_destroy(s);
+        return set_err(ctx, e == hipErrorOutOfMemory ? PSF_ERR_NOMEM : PSF_ERR_HIP, "scene allocation: %s",
+                       hipGetErrorString(e));
+    }
+    // zero padding of Phi (rows F..ldF-1 of each column are read by the
+    // vectorised reconstruction) and the status word
+    chk(hipMemsetAsync(s->dev.Phi, 0, phi_bytes, ctx->stream));
+    chk(launch_zero_int(s->dev.status, 64, ctx->stream));
+    std::vector<double> ones(K, 1.0), zeros(K, 0.0);
+    chk(hipMemcpyAsync(s->dev.visc, ones.data(), K * 8, hipMemcpyHostToDevice, ctx->stream));
+    chk(hipMemcpyAsync(s->dev.gravity, zeros.data(), K * 8, hipMemcpyHostToDevice, ctx->stream));
+    chk(hipMemcpyAsync(s->dev.lambda, zeros.data(), K * 8, hipMemcpyHostToDevice, ctx->stream));
+    chk(hipStreamSynchronize(ctx->stream));
+    if (e != hipSuccess) {
+        psf_scene_destroy(s);
+        return set_err(ctx, PSF_ERR_HIP, "scene init: %s", hipGetErrorString(e));
+    }
+    h->lambda.assign(K, 0.0);
+    *out = s;
+    return PSF_OK;
+}
+
+int psf_scene_destroy(psf_scene* s)
+{
+    if (!s) return PSF_ERR_ARG;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    persistent_scene_free(s);
+    dfree(s->dev.omega); dfree(s->dev.lambda); dfree(s->dev.gravity); dfree(s->dev.visc); dfree(s->dev.y);
+    dfree(s->dev.status); dfree(s->dev.C); dfree(s->dev.Phi); dfree(s->dev.u); dfree(s->dev.parts); dfree(s->dev.cells);
+    delete host_of(s);
+    delete s;
+    return PSF_OK;
+}
+
+int psf_scene_upload_basis(psf_scene* s, const double* Phi, const double* lambda, const double* gravity)
+{
+    if (!s || !Phi || !lambda) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    psf_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t K = s->d.K, F = s->F, ld = s->dev.ldF;
+    for (size_t k = 0; k < K; ++k)
+        if (!std::isfinite(lambda[k])) return set_err(ctx, PSF_ERR_ARG, "lambda[%zu] is not finite", k);
+    // column by column through a pinned staging buffer, converted to the storage type
+    const size_t esz = s->d.phi_precision == PSF_PHI_FP32 ? 4 : 8;
+    void* stage = nullptr;
+    HIPCHK(ctx, hipHostMalloc(&stage, F * esz, hipHostMallocDefault));
+    int rc = PSF_OK;
+    for (size_t k = 0; k < K && rc == PSF_OK; ++k) {
+        const double* col = Phi + k * F;
+        if (esz == 4) {
+            float* st = static_cast<float*>(stage);
+            for (size_t f = 0; f < F; ++f) st[f] = static_cast<float>(col[f]);
+        } else {
+            std::memcpy(stage, col, F * 8);
+        }
+        hipError_t e = hipMemcpyAsync(static_cast<unsigned char*>(s->dev.Phi) + k * ld * esz, stage, F * esz,
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "Phi upload: %s", hipGetErrorString(e));
+    }
+    (void)hipHostFree(stage);
+    if (rc) return rc;
+    host_of(s)->lambda.assign(lambda, lambda + K);
+    HIPCHK(ctx, hipMemcpyAsync(s->dev.lambda, lambda, K * 8, hipMemcpyHostToDevice, ctx->stream));
+    if (gravity) {
+        HIPCHK(ctx, hipMemcpyAsync(s->dev.gravity, gravity, K * 8, hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    s->has_basis = true;
+    return refresh_visc(s, host_of(s)->lambda);
+}
+
+int psf_scene_upload_advection(psf_scene* s, const double* C)
+{
+    if (!s || !C) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    psf_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t K = s->d.K, n = K * K * K;
+    if (s->d.c_precision == PSF_C_FP64) {
+        HIPCHK(ctx, hipMemcpy(s->dev.C, C, n * 8, hipMemcpyHostToDevice));
+    } else if (s->d.c_precision == PSF_C_FP32) {
+        std::vector<float> t(n);
+        for (size_t i = 0; i < n; ++i) t[i] = static_cast<float>(C[i]);
+        HIPCHK(ctx, hipMemcpy(s->dev.C, t.data(), n * 4, hipMemcpyHostToDevice));
+    } else {
+        int rc = persistent_upload_c(s, C);
+        if (rc) return rc;
+    }
+    s->has_C = true;
+    return PSF_OK;
+}
+
+int psf_scene_check_signs(psf_scene* s, const uint32_t* words, size_t nwords)
+{
+    if (!s || !words) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    const size_t need = (s->V + 4) / 5;
+    if (nwords != need) return set_err(s->ctx, PSF_ERR_ARG, "sign table has %zu words, expected %zu", nwords, need);
+    std::vector<uint32_t> ref(need, 0u);
+    uint32_t v = 0;
+    for (uint32_t z = 0; z < s->d.nz; ++z)
+        for (uint32_t y = 0; y < s->d.ny; ++y)
+            for (uint32_t x = 0; x < s->d.nx; ++x, ++v) {
+                const uint32_t bits = (z == 0 ? 1u : 0u) | 2u | (y == 0 ? 4u : 0u) | 8u | (x == 0 ? 16u : 0u) | 32u;
+                ref[v / 5] |= bits << (6 * (v % 5));
+            }
+    for (size_t i = 0; i < need; ++i)
+        if (ref[i] != words[i])
+            return set_err(s->ctx, PSF_ERR_ARG, "sign word %zu is 0x%08x, the grid's closed form gives 0x%08x", i,
+                           words[i], ref[i]);
+    return PSF_OK;
+}
+
+int psf_set_params(psf_scene* s, double dt, double viscosity, int gravity_active, float lifetime)
+{
+    if (!s) return PSF_ERR_ARG;
+    if (!std::isfinite(dt) || !std::isfinite(viscosity)) return set_err(s->ctx, PSF_ERR_ARG, "non-finite parameter");
+    s->dt = dt;
+    s->nu = viscosity;
+    s->gravity_active = gravity_active ? 1 : 0;
+    s->lifetime = lifetime;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    return refresh_visc(s, host_of(s)->lambda);
+}
+
+int psf_set_step_path(psf_scene* s, int path)
+{
+    if (!s) return PSF_ERR_ARG;
+    if (path != PSF_PATH_PLAIN && path != PSF_PATH_PERSISTENT) return set_err(s->ctx, PSF_ERR_ARG, "unknown path %d", path);
+    if (path == PSF_PATH_PLAIN && !is_plain_precision(s->d.c_precision))
+        return set_err(s->ctx, PSF_ERR_UNSUPPORTED, "c_precision %d has no plain path", s->d.c_precision);
+    if (path == PSF_PATH_PERSISTENT && !persistent_supported(s->d.c_precision, static_cast<int>(s->d.K)))
+        return set_err(s->ctx, PSF_ERR_UNSUPPORTED, "c_precision %d has no persistent path", s->d.c_precision);
+    s->path = path;
+    return PSF_OK;
+}
+
+int psf_set_omega(psf_scene* s, const double* omega)
+{
+    if (!s || !omega) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    HIPCHK(s->ctx, hipMemcpyAsync(s->dev.omega, omega, s->d.K * 8, hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(s->ctx, launch_zero_int(s->dev.status, 64, s->ctx->stream));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    s->has_omega = true;
+    return PSF_OK;
+}
+
+int psf_get_omega(psf_scene* s, double* omega)
+{
+    if (!s || !omega) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    HIPCHK(s->ctx, hipMemcpyAsync(omega, s->dev.omega, s->d.K * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+    return check_status(s);
+}
+
+int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
+{
+    if (!s || (!xyzw && n)) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    if (n > s->d.max_particles) return set_err(s->ctx, PSF_ERR_ARG, "%u particles exceed max_particles %u", n, s->d.max_particles);
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if (n) HIPCHK(s->ctx, hipMemcpy(s->dev.parts, xyzw, size_t(n) * 16, hipMemcpyHostToDevice));
+    s->n_particles = n;
+    return PSF_OK;
+}
+
+int psf_get_particles(psf_scene* s, float* xyzw, uint32_t n)
+{
+    if (!s || (!xyzw && n)) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    if (n > s->n_particles) return set_err(s->ctx, PSF_ERR_ARG, "asked for %u particles, scene holds %u", n, s->n_particles);
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    HIPCHK(s->ctx, hipMemcpyAsync(xyzw, s->dev.parts, size_t(n) * 16, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    return PSF_OK;
+}
+
+int psf_step(psf_scene* s, int nsteps)
+{
+    if (!s || nsteps < 0) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    int rc = ready_for_step(s);
+    if (rc) return rc;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    return enqueue_steps(s, nsteps);
+}
+
+int psf_reconstruct(psf_scene* s)
+{
+    if (!s) return PSF_ERR_ARG;
+    if (!s->has_basis || !s->has_omega) return set_err(s->ctx, PSF_ERR_STATE, "basis or omega missing");
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    return enqueue_reconstruct(s);
+}
+
+int psf_get_velocity(psf_scene* s, float* u)
+{
+    if (!s || !u) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    HIPCHK(s->ctx, hipMemcpyAsync(u, s->dev.u, size_t(s->F) * 4, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    return PSF_OK;
+}
+
+int psf_trace(psf_scene* s, int32_t* cells)
+{
+    if (!s) return PSF_ERR_ARG;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    int rc = enqueue_trace(s, cells ? s->dev.cells : nullptr);
+    if (rc || !cells) return rc;
+    HIPCHK(s->ctx, hipMemcpyAsync(cells, s->dev.cells, size_t(s->n_particles) * 48, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    return PSF_OK;
+}
+
+int psf_integrate(psf_scene* s, int nframes)
+{
+    if (!s || nframes < 0) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    int rc = ready_for_step(s);
+    if (rc) return rc;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    for (int f = 0; f < nframes; ++f) {
+        if ((rc = enqueue_steps(s, 1))) return rc;
+        if ((rc = enqueue_reconstruct(s))) return rc;
+        if ((rc = enqueue_trace(s, nullptr))) return rc;
+    }
+    return PSF_OK;
+}
+
+int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
+{
+    if (!s || !ms || count <= 0 || phase < 0 || phase > 4) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    if (phase != 2) {
+        int rc = ready_for_step(s);
+        if (rc) return rc;
+    }
+    psf_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipEvent_t a, b;
+    HIPCHK(ctx, hipEventCreate(&a));
+    HIPCHK(ctx, hipEventCreate(&b));
+    int rc = PSF_OK;
+    hipError_t e = hipEventRecord(a, ctx->stream);
+    if (e != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "event record: %s", hipGetErrorString(e));
+    if (rc == PSF_OK) {
+        switch (phase) {
+        case 0: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_steps(s, 1); break;
+        case 1: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_reconstruct(s); break;
+        case 2: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_trace(s, nullptr); break;
+        case 3:
+            for (int i = 0; i < count && rc == PSF_OK; ++i) {
+                rc = enqueue_steps(s, 1);
+                if (rc == PSF_OK) rc = enqueue_reconstruct(s);
+                if (rc == PSF_OK) rc = enqueue_trace(s, nullptr);
+            }
+            break;
+        default: rc = enqueue_steps(s, count); break;
+        }
+    }
+    if (rc == PSF_OK) {
+        e = hipEventRecord(b, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+        if (e != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "event timing: %s", hipGetErrorString(e));
+        *ms = t;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc == PSF_OK && phase != 1 && phase != 2) rc = check_status(s);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+// Shared host/device definitions of the psf HIP library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "psf/psf.h"
+
+namespace psf {
+
+constexpr int kWave = 64;
+
+// Device-side state of one scene.  All arrays live in HBM and stay resident
+// between calls; nothing is re-uploaded per frame.
+struct DeviceScene {
+    // reduced state, FP64
+    double* omega = nullptr;     // K
+    double* lambda = nullptr;    // K
+    double* gravity = nullptr;   // K
+    double* visc = nullptr;      // K, exp(nu * lambda_k * dt)
+    double* y = nullptr;         // K, contraction result of the last step
+    int* status = nullptr;       // device status word (1 = non-finite energy)
+    // structure tensor
+    void* C = nullptr;           // layout and type depend on c_precision
+    size_t C_bytes = 0;
+    // basis, column-major with leading dimension ldF (zero padded)
+    void* Phi = nullptr;
+    size_t ldF = 0;
+    float* u = nullptr;          // F (velocityField)
+    // particles
+    float4* parts = nullptr;     // max_particles
+    int32_t* cells = nullptr;    // 12 * max_particles (debug output of cell lookups)
+};
+
+}  // namespace psf
+
+struct psf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+struct psf_scene {
+    psf_ctx* ctx = nullptr;
+    psf_scene_desc d{};
+    uint32_t F = 0, V = 0;
+    uint32_t nZF = 0, nYF = 0, nXF = 0;
+    psf::DeviceScene dev;
+    bool has_basis = false, has_C = false, has_omega = false;
+    double dt = 1.0 / 60.0, nu = 0.0;
+    int gravity_active = 0;
+    float lifetime = 60.0f;
+    int path = PSF_PATH_PLAIN;
+    uint32_t n_particles = 0;
+    uint32_t frame = 0;          // counter of trace passes (respawn draws)
+    uint32_t seed = 0;
+    void* host = nullptr;        // host-side copies (psf_scene_host)
+    void* persistent = nullptr;  // persistent-path state (PersistentState)
+};

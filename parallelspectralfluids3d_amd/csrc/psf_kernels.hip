@@ -1,0 +1,369 @@
+// HIP kernels of the reduced-space fluid step for gfx950 (MI355X).
+//
+//   k_contract_plain   y_i = w^T C_i w      psfsolver.cpp:39-44
+//   k_update           FP64 energy renormalisation, viscosity, forcing
+//                                           psfsolver.cpp:34-38, 45-59
+//   k_reconstruct      u = float(Phi w)     psfsolver.cpp:60
+//   k_trace            particle pass        psfsolver.cpp:70-232
+// (paths relative to the reference repository root)
+#include "psf_kernels.hpp"
+
+namespace psf {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Sum over a 256-thread block in a fixed order (wave butterflies, then waves
+// 0..3 in order); every thread receives the result.
+__device__ __forceinline__ double block_sum256(double v, double* red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    return ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// ---------------------------------------------------------------------------
+// Plain contraction.  C holds K matrices K x K, each column-major (element
+// (a, b) of C_i at (i*K + b)*K + a), in FP64 or FP32.  One block per mode i;
+// wave w reads the columns b = w, w+4, ... with coalesced VEC-wide loads along
+// a, and accumulates w_b * sum_a w_a C_i(a, b).
+template <typename T, int VEC>
+__global__ __launch_bounds__(256) void k_contract_plain(const T* __restrict__ C, const double* __restrict__ omega,
+                                                        double* __restrict__ y, int K)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* sw = reinterpret_cast<T*>(smem_raw);
+    __shared__ double red[4];
+    for (int k = threadIdx.x; k < K; k += 256) sw[k] = static_cast<T>(omega[k]);
+    __syncthreads();
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+    const T* Ci = C + static_cast<size_t>(i) * K * K;
+    T acc = 0;
+#pragma unroll 4
+    for (int b = wv; b < K; b += 4) {
+        const T* col = Ci + static_cast<size_t>(b) * K;
+        T part = 0;
+        for (int a0 = lane * VEC; a0 < K; a0 += kWave * VEC) {
+            if constexpr (VEC == 4) {
+                const float4 c = *reinterpret_cast<const float4*>(col + a0);
+                part += sw[a0] * c.x + sw[a0 + 1] * c.y + sw[a0 + 2] * c.z + sw[a0 + 3] * c.w;
+            } else if constexpr (VEC == 2) {
+                const double2 c = *reinterpret_cast<const double2*>(col + a0);
+                part += sw[a0] * c.x + sw[a0 + 1] * c.y;
+            } else {
+                part += sw[a0] * col[a0];
+            }
+        }
+        acc += sw[b] * part;
+    }
+    const double s = block_sum256(static_cast<double>(acc), red);
+    if (threadIdx.x == 0) y[i] = s;
+}
+
+// FP64 update, one block (psfsolver.cpp:34-59):
+//   e1 = w.w; w += dt*y; e2 = w.w; w *= sqrt(e1/e2); w_k *= exp(nu lambda_k dt);
+//   w += dt*g if forcing.
+// A zero or non-finite energy sets *status and leaves w unchanged.
+__global__ __launch_bounds__(256) void k_update(double* __restrict__ omega, const double* __restrict__ y,
+                                                const double* __restrict__ visc, const double* __restrict__ g,
+                                                int K, double dt, int gravity_active, int* __restrict__ status)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double swn[];
+    __shared__ double red[4];
+    double s1 = 0.0;
+    for (int k = threadIdx.x; k < K; k += 256) s1 += omega[k] * omega[k];
+    const double e1 = block_sum256(s1, red);
+    double s2 = 0.0;
+    for (int k = threadIdx.x; k < K; k += 256) {
+        const double wn = omega[k] + dt * y[k];
+        swn[k] = wn;
+        s2 += wn * wn;
+    }
+    const double e2 = block_sum256(s2, red);
+    if (!(e2 > 0.0) || !__builtin_isfinite(e2) || !__builtin_isfinite(e1)) {
+        if (threadIdx.x == 0) *status = 1;
+        return;
+    }
+    const double fac = sqrt(e1 / e2);
+    for (int k = threadIdx.x; k < K; k += 256) {
+        double w = swn[k] * fac;
+        w = w * visc[k];
+        if (gravity_active) w = w + dt * g[k];
+        omega[k] = w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction u_f = float(sum_k Phi[k][f] w_k).  Phi is column-major with a
+// zero-padded leading dimension ldF (a multiple of 256), so every lane issues
+// aligned 16-byte loads of VEC consecutive faces.  The four waves of a block
+// share 64*VEC faces and split the modes (k = wave, wave+4, ...); partial sums
+// are FP64 and combined in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
+                                                     const double* __restrict__ omega, float* __restrict__ u)
+{
+    constexpr int VEC = 16 / sizeof(T);
+    extern __shared__ double sw[];
+    __shared__ double part[3][kWave * VEC];
+    for (int k = threadIdx.x; k < K; k += 256) sw[k] = omega[k];
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t f0 = static_cast<size_t>(blockIdx.x) * (kWave * VEC) + static_cast<size_t>(lane) * VEC;
+    double acc[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) acc[j] = 0.0;
+    const T* base = Phi + f0;
+#pragma unroll 8
+    for (int k = wv; k < K; k += 4) {
+        const double w = sw[k];
+        if constexpr (VEC == 4) {
+            const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(base + static_cast<size_t>(k) * ldF));
+            acc[0] += static_cast<double>(v.x) * w;
+            acc[1] += static_cast<double>(v.y) * w;
+            acc[2] += static_cast<double>(v.z) * w;
+            acc[3] += static_cast<double>(v.w) * w;
+        } else {
+            const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(base + static_cast<size_t>(k) * ldF));
+            acc[0] += v.x * w;
+            acc[1] += v.y * w;
+        }
+    }
+    if (wv > 0) {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) part[wv - 1][lane * VEC + j] = acc[j];
+    }
+    __syncthreads();
+    if (wv == 0) {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+            const double s = ((acc[j] + part[0][lane * VEC + j]) + part[1][lane * VEC + j]) + part[2][lane * VEC + j];
+            const size_t f = f0 + j;
+            if (f < F) u[f] = static_cast<float>(s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Particle pass, one thread per particle.  The arithmetic restates
+// psfsolver.cpp:78-222 operation by operation with contraction off, so the
+// cell lookups and the float results match the CPU path bit for bit.
+
+// face sign of voxel (x,y,z), face j: the closed form of the sign bit string
+// (decmesh3d.cpp:1383-1399): a face is positive iff the voxel created it,
+// i.e. {z==0, 1, y==0, 1, x==0, 1}.
+__device__ __forceinline__ float face_sign(int j, int x, int y, int z)
+{
+    bool pos = (j & 1) ? true : (j == 0 ? z == 0 : (j == 2 ? y == 0 : x == 0));
+    return pos ? 1.0f : -1.0f;
+}
+
+__device__ __forceinline__ float mixf(float x, float y, float a)
+{
+#pragma clang fp contract(off)
+    return x * (1.0f - a) + y * a;   // glm::mix
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ uint32_t respawn_draw(uint32_t seed, uint32_t step, uint32_t particle, uint32_t k)
+{
+    uint64_t z = (static_cast<uint64_t>(seed) << 32) ^ (static_cast<uint64_t>(step) * 0x9E3779B97F4A7C15ull) ^
+                 (static_cast<uint64_t>(particle) * 0xD1B54A32D192ED03ull) ^
+                 (static_cast<uint64_t>(k) * 0x94D049BB133111EBull);
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return static_cast<uint32_t>(z & 1023u);
+}
+
+__global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __restrict__ u, float4* __restrict__ parts,
+                                               uint32_t n, int32_t* __restrict__ cells)
+{
+#pragma clang fp contract(off)
+    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= n) return;
+    float4 p = parts[id];
+    const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny), nz = static_cast<int>(P.nz);
+    if (p.w > 0.0f) {
+        const float sx = (P.aabb_min[0] + P.ext[0]) + p.x;
+        const float sy = (P.aabb_min[1] + P.ext[1]) + p.y;
+        const float sz = (P.aabb_min[2] + P.ext[2]) + p.z;
+        const double res = P.res, hres = P.res / 2;
+        int yO = static_cast<int>(static_cast<double>(sy) / res);
+        int xO = static_cast<int>(static_cast<double>(sx) / res);
+        int zO = static_cast<int>(static_cast<double>(sz) / res);
+        int zM = static_cast<int>((static_cast<double>(sz) - hres) / res);
+        int zP = static_cast<int>((static_cast<double>(sz) + hres) / res);
+        int yM = static_cast<int>((static_cast<double>(sy) - hres) / res);
+        int yP = static_cast<int>((static_cast<double>(sy) + hres) / res);
+        int xM = static_cast<int>((static_cast<double>(sx) - hres) / res);
+        int xP = static_cast<int>((static_cast<double>(sx) + hres) / res);
+        xO = clampi(xO, 0, nx - 1); yO = clampi(yO, 0, ny - 1); zO = clampi(zO, 0, nz - 1);
+        xM = clampi(xM, 0, nx - 1); xP = clampi(xP, 0, nx - 1);
+        yM = clampi(yM, 0, ny - 1); yP = clampi(yP, 0, ny - 1);
+        zM = clampi(zM, 0, nz - 1); zP = clampi(zP, 0, nz - 1);
+        if (cells) {
+            int32_t* c = cells + 12ull * id;
+            c[0] = xO; c[1] = yO; c[2] = zO; c[3] = xM; c[4] = xP; c[5] = yM; c[6] = yP; c[7] = zM; c[8] = zP;
+            c[9] = 0; c[10] = 0; c[11] = 0;
+        }
+        // face indices (decmesh3d.cpp:17-30): Z faces, then Y, then X
+        const uint32_t nxy = P.nx * P.ny;
+        auto zf = [&](int x, int y, int z) { return static_cast<uint32_t>(z) * nxy + static_cast<uint32_t>(y) * P.nx + x; };
+        auto yf = [&](int x, int y, int z) {
+            return static_cast<uint32_t>(z) * ((P.ny + 1) * P.nx) + static_cast<uint32_t>(y) * P.nx + x + P.nZF;
+        };
+        auto xf = [&](int x, int y, int z) {
+            return static_cast<uint32_t>(z) * (P.ny * (P.nx + 1)) + static_cast<uint32_t>(y) * (P.nx + 1) + x + P.nZF + P.nYF;
+        };
+        // x component: voxels (xO, yM|yP, zM|zP), faces f5 (left) / f6 (right)
+        const float s4 = face_sign(4, xO, 0, 0), s5 = face_sign(5, xO, 0, 0);
+        const float vel1x = -s4 * u[xf(xO, yM, zM)], vel3x = s5 * u[xf(xO + 1, yM, zM)];
+        const float vel2x = -s4 * u[xf(xO, yP, zM)], vel4x = s5 * u[xf(xO + 1, yP, zM)];
+        const float vel5x = -s4 * u[xf(xO, yM, zP)], vel7x = s5 * u[xf(xO + 1, yM, zP)];
+        const float vel6x = -s4 * u[xf(xO, yP, zP)], vel8x = s5 * u[xf(xO + 1, yP, zP)];
+        // y component: voxels (xM|xP, yO, zM|zP), faces f3 (bottom) / f4 (top)
+        const float s2 = face_sign(2, 0, yO, 0), s3 = face_sign(3, 0, yO, 0);
+        const float vel1y = -s2 * u[yf(xM, yO, zM)], vel2y = s3 * u[yf(xM, yO + 1, zM)];
+        const float vel3y = -s2 * u[yf(xP, yO, zM)], vel4y = s3 * u[yf(xP, yO + 1, zM)];
+        const float vel5y = -s2 * u[yf(xM, yO, zP)], vel6y = s3 * u[yf(xM, yO + 1, zP)];
+        const float vel7y = -s2 * u[yf(xP, yO, zP)], vel8y = s3 * u[yf(xP, yO + 1, zP)];
+        // z component: voxels (xM|xP, yM|yP, zO), faces f1 (front) / f2 (back)
+        const float s0 = face_sign(0, 0, 0, zO), s1 = face_sign(1, 0, 0, zO);
+        const float vel1z = s0 * u[zf(xM, yM, zO)], vel2z = s0 * u[zf(xM, yP, zO)];
+        const float vel3z = s0 * u[zf(xP, yM, zO)], vel4z = s0 * u[zf(xP, yP, zO)];
+        const float vel5z = -s1 * u[zf(xM, yM, zO + 1)], vel6z = -s1 * u[zf(xM, yP, zO + 1)];
+        const float vel7z = -s1 * u[zf(xP, yM, zO + 1)], vel8z = -s1 * u[zf(xP, yP, zO + 1)];
+
+        const float hf = P.hf;
+        float cx0 = (P.aabb_min[0] + hf * static_cast<float>(xO)) + 0.5f * 0.0f;
+        float cx1 = (P.aabb_min[1] + hf * static_cast<float>(yM)) + 0.5f * hf;
+        float cx2 = (P.aabb_min[2] + hf * static_cast<float>(zM)) + 0.5f * hf;
+        float cy0 = (P.aabb_min[0] + hf * static_cast<float>(xM)) + 0.5f * hf;
+        float cy1 = (P.aabb_min[1] + hf * static_cast<float>(yO)) + 0.5f * 0.0f;
+        float cy2 = (P.aabb_min[2] + hf * static_cast<float>(zM)) + 0.5f * hf;
+        float cz0 = (P.aabb_min[0] + hf * static_cast<float>(xM)) + 0.5f * hf;
+        float cz1 = (P.aabb_min[1] + hf * static_cast<float>(yM)) + 0.5f * hf;
+        float cz2 = (P.aabb_min[2] + hf * static_cast<float>(zO)) + 0.5f * 0.0f;
+        const float inv = P.inv_h;
+        const float nx0 = inv * (p.x - cx0), nx1 = inv * (p.y - cx1), nx2 = inv * (p.z - cx2);
+        const float ny0 = inv * (p.x - cy0), ny1 = inv * (p.y - cy1), ny2 = inv * (p.z - cy2);
+        const float nz0 = inv * (p.x - cz0), nz1 = inv * (p.y - cz1), nz2 = inv * (p.z - cz2);
+        float vx, vy, vz;
+        {
+            const float a0 = mixf(vel1x, vel2x, nx1), a1 = mixf(vel3x, vel4x, nx1);
+            const float a2 = mixf(vel5x, vel6x, nx1), a3 = mixf(vel7x, vel8x, nx1);
+            vx = mixf(mixf(a0, a1, nx0), mixf(a2, a3, nx0), nx2);
+        }
+        {
+            const float a0 = mixf(vel1y, vel2y, ny1), a1 = mixf(vel3y, vel4y, ny1);
+            const float a2 = mixf(vel5y, vel6y, ny1), a3 = mixf(vel7y, vel8y, ny1);
+            vy = mixf(mixf(a0, a1, ny0), mixf(a2, a3, ny0), ny2);
+        }
+        {
+            const float a0 = mixf(vel1z, vel2z, nz1), a1 = mixf(vel3z, vel4z, nz1);
+            const float a2 = mixf(vel5z, vel6z, nz1), a3 = mixf(vel7z, vel8z, nz1);
+            vz = mixf(mixf(a0, a1, nz0), mixf(a2, a3, nz0), nz2);
+        }
+        // glm::clamp(v, lo, hi) = min(max(v, lo), hi)
+        auto clampf = [](float v, float lo, float hi) {
+            const float t = (v < lo) ? lo : v;
+            return (hi < t) ? hi : t;
+        };
+        p.x = clampf(p.x + P.dtf * vx, P.aabb_min[0], P.aabb_max[0]);
+        p.y = clampf(p.y + P.dtf * vy, P.aabb_min[1], P.aabb_max[1]);
+        p.z = clampf(p.z + P.dtf * vz, P.aabb_min[2], P.aabb_max[2]);
+        p.w = p.w - 1.0f;
+    } else {
+        if (cells) {
+            int32_t* c = cells + 12ull * id;
+#pragma unroll
+            for (int k = 0; k < 12; ++k) c[k] = -1;
+        }
+        const uint32_t r0 = respawn_draw(P.seed, P.frame, id, 0);
+        const uint32_t r1 = respawn_draw(P.seed, P.frame, id, 1);
+        const uint32_t r2 = respawn_draw(P.seed, P.frame, id, 2);
+        p.x = static_cast<float>((r0 / 1024.0 - 0.5) * 0.25);
+        p.y = P.aabb_min[1] + 0.2f;
+        p.z = static_cast<float>(-static_cast<double>(P.center_z) + (r1 / 1024.0 - 0.5) * 0.25);
+        p.w = static_cast<float>(P.lifetime * (r2 / 1024.0));
+    }
+    parts[id] = p;
+}
+
+__global__ void k_zero_int(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_contract_plain(int prec, const void* C, const double* omega, double* y, int K, hipStream_t s)
+{
+    if (prec == PSF_C_FP64) {
+        const size_t sh = sizeof(double) * K;
+        if (K % 2 == 0)
+            k_contract_plain<double, 2><<<K, 256, sh, s>>>(static_cast<const double*>(C), omega, y, K);
+        else
+            k_contract_plain<double, 1><<<K, 256, sh, s>>>(static_cast<const double*>(C), omega, y, K);
+    } else {
+        const size_t sh = sizeof(float) * K;
+        if (K % 4 == 0)
+            k_contract_plain<float, 4><<<K, 256, sh, s>>>(static_cast<const float*>(C), omega, y, K);
+        else
+            k_contract_plain<float, 1><<<K, 256, sh, s>>>(static_cast<const float*>(C), omega, y, K);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_update(double* omega, const double* y, const double* visc, const double* g, int K, double dt,
+                         int gravity_active, int* status, hipStream_t s)
+{
+    k_update<<<1, 256, sizeof(double) * K, s>>>(omega, y, visc, g, K, dt, gravity_active, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F, int K, const double* omega,
+                              float* u, hipStream_t s)
+{
+    const size_t sh = sizeof(double) * K;
+    if (prec == PSF_PHI_FP32) {
+        const unsigned grid = static_cast<unsigned>(ldF / (kWave * 4));
+        k_reconstruct<float><<<grid, 256, sh, s>>>(static_cast<const float*>(Phi), ldF, F, K, omega, u);
+    } else {
+        const unsigned grid = static_cast<unsigned>(ldF / (kWave * 2));
+        k_reconstruct<double><<<grid, 256, sh, s>>>(static_cast<const double*>(Phi), ldF, F, K, omega, u);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, uint32_t n, int32_t* cells,
+                        hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_trace<<<(n + 255) / 256, 256, 0, s>>>(p, u, parts, n, cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_int(int* p, int n, hipStream_t s)
+{
+    k_zero_int<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+}  // namespace psf

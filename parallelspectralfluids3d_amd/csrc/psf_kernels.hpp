@@ -1,0 +1,35 @@
+// Launchers of the psf HIP kernels (implemented in psf_kernels.hip).
+#pragma once
+#include "psf_common.hpp"
+
+namespace psf {
+
+struct TraceParams {
+    uint32_t nx, ny, nz;
+    uint32_t nZF, nYF;          // face-block offsets (Z faces first, then Y, then X)
+    float aabb_min[3], aabb_max[3];
+    float ext[3];               // aabb_max - aabb_min (AABB::getExtent, float)
+    double res;                 // AbstractSolver::resolution
+    float hf;                   // (float)resolution
+    float inv_h;                // float(1.0 / resolution)
+    float dtf;                  // float(timeStep)
+    float lifetime;
+    float center_z;             // AABB::getCenter().z (float)
+    uint32_t seed, frame;
+};
+
+// reduced step, plain path: y_i = w^T C_i w (one block per mode i), then the
+// FP64 update of w (one block)
+hipError_t launch_contract_plain(int c_precision, const void* C, const double* omega, double* y, int K,
+                                 hipStream_t s);
+hipError_t launch_update(double* omega, const double* y, const double* visc, const double* gravity, int K,
+                         double dt, int gravity_active, int* status, hipStream_t s);
+// u = float(Phi w)
+hipError_t launch_reconstruct(int phi_precision, const void* Phi, size_t ldF, uint32_t F, int K,
+                              const double* omega, float* u, hipStream_t s);
+// one particle pass
+hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, uint32_t n, int32_t* cells,
+                        hipStream_t s);
+hipError_t launch_zero_int(int* p, int n, hipStream_t s);
+
+}  // namespace psf

@@ -1,0 +1,171 @@
+"""Python handles over the C ABI: Context (one GPU, one stream) and Scene.
+
+Every call goes through libpsf_hip.so; numpy arrays are passed as host
+pointers and copied by the library.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib as L
+from .grid import Grid
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = ct.c_void_p()
+        rc = self.lib.psf_ctx_create(device, ct.byref(h))
+        if rc != 0:
+            raise L.PsfError(rc, f"psf_ctx_create(device={device}) failed")
+        self.h = h
+
+    def check(self, rc, what=""):
+        if rc != 0:
+            msg = self.lib.psf_last_error(self.h)
+            raise L.PsfError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def sync(self):
+        self.check(self.lib.psf_ctx_sync(self.h), "sync")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.psf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_desc(grid: Grid, resolution, K, c_precision=L.C_FP64, phi_precision=L.PHI_FP64, max_particles=0,
+              aabb_min=(-0.5, -0.5, -0.5), aabb_max=(0.5, 0.5, 0.5)):
+    d = L.SceneDesc()
+    d.nx, d.ny, d.nz = grid.nx, grid.ny, grid.nz
+    d.resolution = float(resolution)
+    d.aabb_min = (ct.c_float * 3)(*aabb_min)
+    d.aabb_max = (ct.c_float * 3)(*aabb_max)
+    d.K = int(K)
+    d.c_precision = int(c_precision)
+    d.phi_precision = int(phi_precision)
+    d.max_particles = int(max_particles)
+    return d
+
+
+def footprint(desc) -> int:
+    lib = L.load()
+    out = ct.c_uint64()
+    rc = lib.psf_scene_footprint(ct.byref(desc), ct.byref(out))
+    if rc != 0:
+        raise L.PsfError(rc, "psf_scene_footprint")
+    return int(out.value)
+
+
+class Scene:
+    """One scene resident on the device (C, Phi, omega, u, particles)."""
+
+    def __init__(self, ctx: Context, desc):
+        self.ctx = ctx
+        self.desc = desc
+        self.K = int(desc.K)
+        self.grid = Grid(desc.nx, desc.ny, desc.nz)
+        h = ct.c_void_p()
+        ctx.check(ctx.lib.psf_scene_create(ctx.h, ct.byref(desc), ct.byref(h)), "psf_scene_create")
+        self.h = h
+        self.n_particles = 0
+
+    def _c(self, rc, what):
+        self.ctx.check(rc, what)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.psf_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # uploads
+    def upload_basis(self, Phi_cm, lam, gravity=None):
+        Phi = np.ascontiguousarray(Phi_cm, dtype=np.float64)
+        assert Phi.shape == (self.K, self.grid.nF)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        g = None if gravity is None else np.ascontiguousarray(gravity, dtype=np.float64)
+        self._c(self.ctx.lib.psf_scene_upload_basis(self.h, _ptr(Phi), _ptr(lam), None if g is None else _ptr(g)),
+                "upload_basis")
+
+    def upload_advection(self, C):
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        assert C.shape == (self.K, self.K, self.K)
+        self._c(self.ctx.lib.psf_scene_upload_advection(self.h, _ptr(C)), "upload_advection")
+
+    def check_signs(self, words):
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        self._c(self.ctx.lib.psf_scene_check_signs(self.h, _ptr(w), w.size), "check_signs")
+
+    def set_params(self, dt, viscosity=0.0, gravity_active=False, lifetime=60.0):
+        self._c(self.ctx.lib.psf_set_params(self.h, float(dt), float(viscosity), int(bool(gravity_active)),
+                                            float(lifetime)), "set_params")
+
+    def set_path(self, path):
+        self._c(self.ctx.lib.psf_set_step_path(self.h, int(path)), "set_step_path")
+
+    def set_omega(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.shape == (self.K,)
+        self._c(self.ctx.lib.psf_set_omega(self.h, _ptr(w)), "set_omega")
+
+    def get_omega(self):
+        w = np.zeros(self.K)
+        self._c(self.ctx.lib.psf_get_omega(self.h, _ptr(w)), "get_omega")
+        return w
+
+    def set_particles(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.ndim == 2 and p.shape[1] == 4
+        self._c(self.ctx.lib.psf_set_particles(self.h, _ptr(p), p.shape[0]), "set_particles")
+        self.n_particles = p.shape[0]
+
+    def get_particles(self):
+        p = np.zeros((self.n_particles, 4), np.float32)
+        self._c(self.ctx.lib.psf_get_particles(self.h, _ptr(p), self.n_particles), "get_particles")
+        return p
+
+    # hot path
+    def step(self, n=1):
+        self._c(self.ctx.lib.psf_step(self.h, int(n)), "step")
+
+    def reconstruct(self):
+        self._c(self.ctx.lib.psf_reconstruct(self.h), "reconstruct")
+
+    def get_velocity(self):
+        u = np.zeros(self.grid.nF, np.float32)
+        self._c(self.ctx.lib.psf_get_velocity(self.h, _ptr(u)), "get_velocity")
+        return u
+
+    def trace(self, cells=False):
+        if cells:
+            c = np.zeros((self.n_particles, 12), np.int32)
+            self._c(self.ctx.lib.psf_trace(self.h, _ptr(c)), "trace")
+            return c
+        self._c(self.ctx.lib.psf_trace(self.h, None), "trace")
+        return None
+
+    def integrate(self, nframes=1):
+        self._c(self.ctx.lib.psf_integrate(self.h, int(nframes)), "integrate")
+
+    def time_phase(self, phase, count):
+        ms = ct.c_double()
+        self._c(self.ctx.lib.psf_time_phase(self.h, int(phase), int(count), ct.byref(ms)), "time_phase")
+        return float(ms.value)

@@ -1,0 +1,60 @@
+"""Structure tensor C: the product's sparse construction against the oracle's
+literal buildAdvection (psfsolver.cpp:431-551), plus the wedge identities
+lambda_i C_i(k, j) = -lambda_j C_j(k, i) and C_i(k, i) = 0."""
+import numpy as np
+import pytest
+
+import oracle as O
+from parallelspectralfluids3d_amd import advection, basis
+from parallelspectralfluids3d_amd.grid import Grid, box_mask, sphere_obstacle_mask
+
+
+def _inputs(dims, K, obstacle=False, eigen=False):
+    g = Grid(*dims)
+    h = 1.0 / max(dims)
+    amin = (-0.5, -0.5, -0.5)
+    ins = sphere_obstacle_mask(g, h, amin, radius=0.3) if obstacle else box_mask(g)
+    if eigen:
+        Phi, lam = basis.laplacian_eigenbasis(g, h, ins, K)
+    else:
+        Phi, lam, _ = basis.analytic_box_modes(g, K)
+    V = (basis.curl_operator(g, h) @ Phi.T).T
+    return g, h, amin, ins, Phi, lam, V
+
+
+@pytest.mark.parametrize("dims,K,obstacle,eigen", [((2, 2, 2), 4, False, False), ((4, 4, 4), 12, False, False),
+                                                   ((8, 8, 8), 20, True, True), ((4, 4, 8), 10, False, True)])
+def test_matches_literal_build(dims, K, obstacle, eigen):
+    g, h, amin, ins, Phi, lam, V = _inputs(dims, K, obstacle, eigen)
+    m = O.Mesh(dims, h, amin, inside=ins)
+    _, curl, _ = m.laplacian()
+    Vref = (curl @ Phi.T).T
+    assert np.abs(V - Vref).max() <= 1e-13 * np.abs(Vref).max()
+    Cref = O.build_advection(m, Phi, lam, Vref)
+    C = advection.build_advection(g, h, ins, Phi, lam, V)
+    assert np.abs(Cref).max() > 1e-6
+    assert np.abs(C - Cref).max() <= 1e-12 * np.abs(Cref).max()
+
+
+def test_wedge_identities():
+    g, h, amin, ins, Phi, lam, V = _inputs((6, 6, 6), 16)
+    C = advection.build_advection(g, h, ins, Phi, lam, V)
+    K = C.shape[0]
+    # C[i, j, k] = C_i(k, j)
+    lhs = lam[:, None, None] * C                       # lam_i C_i(k, j)  at [i, j, k]
+    rhs = -lam[None, :, None] * np.transpose(C, (1, 0, 2))   # -lam_j C_j(k, i) at [i, j, k]
+    assert np.abs(lhs - rhs).max() <= 1e-13 * np.abs(lhs).max()
+    diag = C[np.arange(K), np.arange(K), :]            # C_i(k, i)
+    assert np.abs(diag).max() <= 1e-15
+    assert np.abs(C).max() > 1e-4
+
+
+def test_slot_table_is_the_voxel_edge_set():
+    """Each voxel's 12 slot edges are exactly its 12 edges (psfsolver.cpp:456-467)."""
+    g = Grid(3, 3, 3)
+    e, _, _, _ = advection.wedge_terms(g, box_mask(g))
+    per_voxel = e.reshape(12, g.nV).T
+    for v in range(g.nV):
+        faces = g.voxel_faces[v]
+        edges = set(g.face_edges[faces].reshape(-1))
+        assert set(per_voxel[v]) == edges and len(set(per_voxel[v])) == 12

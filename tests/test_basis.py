@@ -1,0 +1,58 @@
+"""Bases: analytic box modes (exact eigenvectors of the free-slip DEC
+Laplacian) and the reference-M eigensolve (psfsolver.cpp:268-341 acceptance)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as O
+from parallelspectralfluids3d_amd import basis
+from parallelspectralfluids3d_amd.grid import Grid, box_mask
+
+
+@pytest.mark.parametrize("dims,K", [((8, 8, 8), 30), ((4, 8, 2), 12)])
+def test_analytic_modes(dims, K):
+    g = Grid(*dims)
+    h = 1.0 / max(dims)
+    Phi, lam, _ = basis.analytic_box_modes(g, K)
+    assert np.abs(Phi @ Phi.T - np.eye(K)).max() < 1e-13
+    assert np.all(np.diff(lam) >= -1e-12)
+    m = O.Mesh(dims, h, (-0.5, -0.5, -0.5), inside=box_mask(g))
+    assert np.abs(m.derivative2() @ Phi.T).max() < 1e-13          # divergence-free
+    wall = g.face_inside_count(box_mask(g)) != 2
+    assert np.abs(Phi[:, wall]).max() == 0.0                       # no flux through walls
+    # free-slip M: the reference operator with zero Hodge weight on wall edges
+    d1 = m.derivative1()
+    he = m.hodge2_dual()
+    _, cnt = g.edge_faces
+    he_fs = np.where(cnt == 4, he, 0.0)
+    M = d1.T @ sp.diags(he_fs) @ d1 @ sp.diags(m.hodge2_primal())
+    M = sp.diags((~wall).astype(float)) @ M
+    res = np.linalg.norm(M @ Phi.T - Phi.T * lam, axis=0)
+    assert res.max() < 1e-12
+
+
+def test_laplacian_eigenbasis_box():
+    g = Grid(8, 8, 8)
+    h = 1.0 / 8
+    ins = box_mask(g)
+    Phi, lam = basis.laplacian_eigenbasis(g, h, ins, 24)
+    m = O.Mesh((8, 8, 8), h, (-0.5, -0.5, -0.5), inside=ins)
+    M, _, _ = m.laplacian()
+    assert np.linalg.norm(M @ Phi.T - Phi.T * lam, axis=0).max() < 1e-11
+    assert np.abs(Phi @ Phi.T - np.eye(24)).max() < 1e-12
+    assert np.all(lam > 1e-10) and np.all(np.diff(lam) >= -1e-12)
+    # nothing smaller was skipped: a dense eigensolve of the same operator agrees
+    act = np.flatnonzero(g.face_inside_count(ins) == 2)
+    ev = np.linalg.eigvalsh(M[act][:, act].toarray())
+    pos = ev[ev > 1e-10]
+    assert np.allclose(lam, pos[:24], rtol=1e-10)
+
+
+def test_omega_generators():
+    w = basis.unit_energy_omega(50, 0)
+    assert abs(np.linalg.norm(w) - 1) < 1e-15
+    assert np.array_equal(w, basis.unit_energy_omega(50, 0))
+    p = basis.power_law_omega(64, 1)
+    assert abs(np.linalg.norm(p) - 1) < 1e-15
+    r = np.abs(p[1:]) / np.abs(p[:-1])
+    assert np.allclose(r, (np.arange(2, 65) / np.arange(1, 64)) ** (-5 / 6))

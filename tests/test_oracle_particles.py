@@ -1,0 +1,134 @@
+"""Pins the oracle's particle pass (psfsolver.cpp:70-232) with closed-form
+trajectories: zero field, uniform flow, and exact trilinear reproduction of a
+linear velocity field (a wrong face index, sign or weight breaks it)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from parallelspectralfluids3d_amd.grid import Grid
+
+N_GRID = 8
+H = 1.0 / N_GRID
+AMIN, AMAX = (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)
+
+
+def _mesh():
+    return O.Mesh((N_GRID,) * 3, H, AMIN, inside=np.ones(N_GRID ** 3, bool))
+
+
+def _face_field(vx, vy, vz):
+    """face values u such that the reference's signed lookups (psfsolver.cpp:160-188)
+    read velocity component fields vx/vy/vz sampled at face centres."""
+    g = Grid(N_GRID, N_GRID, N_GRID)
+    c = g.face_coords.astype(np.float64)
+    u = np.zeros(g.nF)
+    n = N_GRID
+    z0, y0, x0 = g.nZF, g.nZF + g.nYF, g.nF
+    # Z faces: centre (x+.5, y+.5, z); read as +u at z node 0, -u elsewhere
+    cz = c[:z0]
+    pz = -0.5 + H * np.stack([cz[:, 0] + .5, cz[:, 1] + .5, cz[:, 2]], 1)
+    u[:z0] = np.where(cz[:, 2] == 0, 1.0, -1.0) * vz(pz)
+    # Y faces: centre (x+.5, y, z+.5); -u at y node 0, +u elsewhere
+    cy = c[z0:y0]
+    py = -0.5 + H * np.stack([cy[:, 0] + .5, cy[:, 1], cy[:, 2] + .5], 1)
+    u[z0:y0] = np.where(cy[:, 1] == 0, -1.0, 1.0) * vy(py)
+    # X faces: centre (x, y+.5, z+.5); -u at x node 0, +u elsewhere
+    cx = c[y0:x0]
+    px = -0.5 + H * np.stack([cx[:, 0], cx[:, 1] + .5, cx[:, 2] + .5], 1)
+    u[y0:x0] = np.where(cx[:, 0] == 0, -1.0, 1.0) * vx(px)
+    assert n == N_GRID
+    return u.astype(np.float32)
+
+
+def _run(u, p, dt=0.05, **kw):
+    return O.particles(_mesh(), u, p, resolution=H, timestep=dt, aabb_min=AMIN, aabb_max=AMAX, **kw)
+
+
+def _particles(n, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros((n, 4), np.float32)
+    p[:, :3] = lo + (hi - lo) * rng.random((n, 3))
+    p[:, 3] = np.inf
+    return p
+
+
+def test_zero_field_keeps_particles():
+    g = Grid(N_GRID, N_GRID, N_GRID)
+    p = _particles(500, -0.5, 0.5, 0)
+    p[:, 3] = 10.0
+    out = _run(np.zeros(g.nF, np.float32), p)
+    assert np.array_equal(out[:, :3], p[:, :3])
+    assert np.array_equal(out[:, 3], p[:, 3] - 1.0)
+
+
+def test_uniform_flow_closed_form():
+    """constant velocity (U, V, W): p(n) = clamp(p0 + n dt (U, V, W)) to float rounding,
+    including the reference's z sign convention (velocity -flux on interior Z faces)."""
+    U, V, W = 0.3, -0.2, 0.25
+    u = _face_field(lambda q: np.full(len(q), U), lambda q: np.full(len(q), V), lambda q: np.full(len(q), W))
+    p0 = _particles(1000, -0.5, 0.5, 1)
+    p = p0
+    for _ in range(20):
+        p = _run(u, p)
+    expect = np.clip(p0[:, :3].astype(np.float64) + 20 * 0.05 * np.array([U, V, W]), -0.5, 0.5)
+    assert np.abs(p[:, :3] - expect).max() < 2e-6
+
+
+def test_linear_field_reproduced_exactly():
+    """trilinear interpolation of face samples of a linear field is the field
+    itself (psfsolver.cpp:195-212), for particles at least h/2 from the walls."""
+    A = np.array([[0.1, 0.4, -0.3, 0.2], [-0.2, 0.1, 0.5, -0.4], [0.3, -0.1, 0.2, 0.6]])
+    f = [lambda q, a=a: a[0] + q @ a[1:] for a in A]
+    u = _face_field(*f)
+    lo, hi = -0.5 + H / 2 + 1e-3, 0.5 - H / 2 - 1e-3
+    p0 = _particles(2000, lo, hi, 2)
+    dt = 0.05
+    p1 = _run(u, p0, dt=dt)
+    vel = (p1[:, :3].astype(np.float64) - p0[:, :3]) / dt
+    q = p0[:, :3].astype(np.float64)
+    expect = np.stack([fi(q) for fi in f], 1)
+    assert np.abs(vel - expect).max() < 5e-5
+
+
+def test_cell_lookup_matches_floor():
+    """xOfs = int((min + extent + p)/h) and the -h/2, +h/2 variants, clamped
+    (psfsolver.cpp:80-100), equal floor((p - min)/h (+-1/2)) away from ties."""
+    g = Grid(N_GRID, N_GRID, N_GRID)
+    p = _particles(3000, -0.5, 0.5, 3)
+    _, cells = _run(np.zeros(g.nF, np.float32), p, cells=True)
+    q = (p[:, :3].astype(np.float64) + 0.5) / H
+    cl = lambda a: np.clip(a, 0, N_GRID - 1)
+    o = cl(np.floor(q)).astype(int)
+    m = cl(np.floor(q - 0.5)).astype(int)
+    pl = cl(np.floor(q + 0.5)).astype(int)
+    assert np.array_equal(cells[:, [0, 1, 2]], o)
+    assert np.array_equal(cells[:, [3, 5, 7]], m)
+    assert np.array_equal(cells[:, [4, 6, 8]], pl)
+
+
+def test_respawn_formula():
+    """w <= 0: respawn at x = (r/1024 - 1/2)/4, y = min.y + 0.2, z likewise,
+    w = lifetime * r/1024 (psfsolver.cpp:219-222), with counter-based draws."""
+    g = Grid(N_GRID, N_GRID, N_GRID)
+    p = _particles(64, -0.4, 0.4, 4)
+    p[:, 3] = 0.0
+    out = _run(np.zeros(g.nF, np.float32), p, lifetime=60.0, seed=7, step=3)
+    for i in range(64):
+        r = [O.respawn_draw(7, 3, i, k) for k in range(3)]
+        assert out[i, 0] == np.float32((r[0] / 1024.0 - 0.5) * 0.25)
+        assert out[i, 1] == np.float32(-0.5) + np.float32(0.2)
+        assert out[i, 2] == np.float32(-0.0 + (r[1] / 1024.0 - 0.5) * 0.25)
+        assert out[i, 3] == np.float32(60.0 * (r[2] / 1024.0))
+    draws = np.array([O.respawn_draw(1, 2, i, 0) for i in range(4000)])
+    assert draws.min() >= 0 and draws.max() < 1024 and 400 < draws.mean() < 624
+
+
+@pytest.mark.parametrize("nthreads", [1, 4])
+def test_deterministic_across_threads(nthreads):
+    g = Grid(N_GRID, N_GRID, N_GRID)
+    rng = np.random.default_rng(9)
+    u = rng.standard_normal(g.nF).astype(np.float32)
+    p = _particles(5000, -0.5, 0.5, 5)
+    a = _run(u, p, nthreads=1)
+    b = _run(u, p, nthreads=nthreads)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
