@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark of the reduced-space fluid step on one MI355X.
+
+Workload: the 64^3 closed box of the precision configuration -- K = 256
+modes (analytic discrete sin/cos eigenvectors of the box's DEC 1-form
+Laplacian), omega ~ N(0,1) from seed 0 at unit energy, dt = 0.01, no
+viscosity or forcing, 2^20 particles uniform in the box.  A "step" is one
+AbstractSolver::integrate() frame (psfsolver.cpp:21-234): reduced step
+(BF16 hi+lo MFMA contraction, FP64 update), u = Phi w (FP32 Phi, FP64 sums)
+and one particle pass.
+
+Printed on stdout: one JSON line.  value = particle-steps/s of the frame
+loop; beside it the reduced steps/s of one persistent launch of 1000 steps
+and of the plain FP32 path, omega error against the FP64 path every 100
+steps, frame latency p50/p99, the roofline of the dominant kernel
+(reconstruction) and the CPU oracle timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "reduced steps/s + particle-steps/s at 64³,K=256; p99 latency; err vs FP64"
+HBM_PEAK = 8.0e12
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_scene(n, K, N, seed=0):
+    from parallelspectralfluids3d_amd import advection, basis
+    from parallelspectralfluids3d_amd.grid import Grid, box_mask
+    g = Grid(n, n, n)
+    h = 1.0 / n
+    t = time.time()
+    Phi, lam, tab = basis.analytic_box_modes(g, K)
+    V = (basis.curl_operator(g, h) @ Phi.T).T
+    log(f"basis {g.nF} faces x {K} modes in {time.time() - t:.1f}s")
+    t = time.time()
+    C = advection.build_advection_box_analytic(g, h, Phi, lam, tab, V)
+    log(f"structure tensor C ({K}^3) in {time.time() - t:.1f}s")
+    del V
+    gvec = basis.project(Phi, basis.gravity_field(g, box_mask(g)))
+    omega = basis.unit_energy_omega(K, seed)
+    rng = np.random.default_rng(seed + 1)
+    parts = np.zeros((N, 4), np.float32)
+    parts[:, :3] = rng.random((N, 3)) - 0.5
+    parts[:, 3] = np.inf
+    return dict(grid=g, h=h, Phi=Phi, lam=lam, C=C, g=gvec, omega=omega, parts=parts)
+
+
+def make_hip_scene(ctx, sc, c_prec, phi_prec, with_phi, N):
+    from parallelspectralfluids3d_amd import hip
+    d = hip.make_desc(sc["grid"], sc["h"], len(sc["lam"]), c_prec, phi_prec, N)
+    s = hip.Scene(ctx, d)
+    s.upload_basis(sc["Phi"] if with_phi else None, sc["lam"], sc["g"])
+    s.upload_advection(sc["C"])
+    s.set_omega(sc["omega"])
+    s.set_params(0.01, 0.0, False, lifetime=60.0)
+    if N:
+        s.set_particles(sc["parts"][:N])
+    return s
+
+
+def cpu_baseline(sc, budget_s=12.0):
+    """The oracle (a C restatement of psfsolver.cpp:21-234, OpenMP as in the
+    reference) on this host: whole integrate() frames until budget_s elapse."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    nthreads = min(16, os.cpu_count() or 1)
+    g = sc["grid"]
+    m = O.Mesh((g.nx, g.ny, g.nz), sc["h"], (-0.5, -0.5, -0.5), inside=np.ones(g.nV, bool))
+    Phi = np.ascontiguousarray(sc["Phi"])
+    w = sc["omega"].copy()
+    p = sc["parts"].copy()
+    frames, t_red, t_rec, t_par = 0, 0.0, 0.0, 0.0
+    t_start = time.time()
+    while frames < 2 or (time.time() - t_start < budget_s and frames < 200):
+        t0 = time.perf_counter()
+        w = O.reduced_step(w, sc["C"], sc["lam"], sc["g"], 0.01, 0.0, False, nthreads=nthreads)
+        t1 = time.perf_counter()
+        u = O.reconstruct(Phi, w, nthreads=nthreads)
+        t2 = time.perf_counter()
+        p = O.particles(m, u, p, resolution=sc["h"], timestep=0.01, aabb_min=(-.5, -.5, -.5),
+                        aabb_max=(.5, .5, .5), lifetime=60.0, seed=0, step=frames, nthreads=nthreads)
+        t3 = time.perf_counter()
+        if frames > 0:                      # first frame is warm-up
+            t_red += t1 - t0; t_rec += t2 - t1; t_par += t3 - t2
+        frames += 1
+    timed = frames - 1
+    tot = t_red + t_rec + t_par
+    N = p.shape[0]
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": N * timed / tot, "unit": "particle-steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"{timed} integrate() frames of the same {g.nx}^3/K={len(sc['lam'])}/{N}-particle workload "
+                      f"after 1 warm-up frame (FP64 C and Phi as in psfsolver.cpp)",
+            "ms_per_frame": 1e3 * tot / timed, "ms_reduced": 1e3 * t_red / timed,
+            "ms_reconstruct": 1e3 * t_rec / timed, "ms_particles": 1e3 * t_par / timed,
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--modes", type=int, default=256)
+    ap.add_argument("--particles", type=int, default=1 << 20)
+    ap.add_argument("--reduced-steps", type=int, default=1000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from parallelspectralfluids3d_amd import _lib as L
+    from parallelspectralfluids3d_amd import hip
+
+    n, K, N = args.grid, args.modes, args.particles
+    sc = build_scene(n, K, N)
+    F = sc["grid"].nF
+    ctx = hip.Context(0)
+
+    # ---- frame loop (the value) ---------------------------------------------------
+    s = make_hip_scene(ctx, sc, L.C_BF16X2, L.PHI_FP32, True, N)
+    log("frame scene uploaded")
+    s.time_phase(L.PHASE_FRAME, args.warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ms = s.time_phase(L.PHASE_FRAME, args.steps)
+    wall = time.perf_counter() - t0
+    ms_frame = ms / args.steps
+    log(f"{args.steps} frames: {ms:.2f} ms (events), {1e3 * wall:.2f} ms (wall)")
+    # per-phase averages: reconstruction is the dominant kernel
+    reps = 100
+    ms_rec = s.time_phase(L.PHASE_RECON, reps) / reps
+    ms_tr = s.time_phase(L.PHASE_TRACE, reps) / reps
+    ms_st = s.time_phase(L.PHASE_STEP, reps) / reps
+    # frame latency distribution
+    lat = np.array([s.time_phase(L.PHASE_FRAME, 1) for _ in range(200)])
+    s.close()
+
+    rec_bytes = F * K * 4 + F * 4 + K * 8
+    achieved = rec_bytes / (ms_rec * 1e-3)
+
+    # ---- reduced steps/s: one persistent launch vs the plain FP32 path -------------------
+    nred = args.reduced_steps
+    res = {}
+    for name, prec in (("bf16x2", L.C_BF16X2), ("bf16", L.C_BF16), ("fp16", L.C_FP16)):
+        r = make_hip_scene(ctx, sc, prec, L.PHI_FP32, False, 0)
+        r.time_phase(L.PHASE_NSTEP, 50)
+        r.set_omega(sc["omega"])
+        res[name] = nred / (1e-3 * r.time_phase(L.PHASE_NSTEP, nred))
+        r.close()
+    for name, prec in (("plain_fp32", L.C_FP32), ("plain_fp64", L.C_FP64)):
+        r = make_hip_scene(ctx, sc, prec, L.PHI_FP32, False, 0)
+        r.time_phase(L.PHASE_STEP, 20)
+        res[name] = nred / (1e-3 * r.time_phase(L.PHASE_STEP, nred))
+        r.close()
+    log("reduced steps/s " + ", ".join(f"{k}={v:.0f}" for k, v in res.items()))
+
+    # ---- omega error against the FP64 path (which matches the oracle to 1e-12) -----------
+    errors = {}
+    if not args.no_precision:
+        ref = make_hip_scene(ctx, sc, L.C_FP64, L.PHI_FP32, False, 0)
+        traj = []
+        for _ in range(10):
+            ref.step(100)
+            traj.append(ref.get_omega())
+        ref.close()
+        for name, prec in (("bf16x2", L.C_BF16X2), ("bf16", L.C_BF16), ("fp16", L.C_FP16)):
+            r = make_hip_scene(ctx, sc, prec, L.PHI_FP32, False, 0)
+            e = []
+            for b in range(10):
+                r.step(100)
+                w = r.get_omega()
+                e.append(float(np.linalg.norm(w - traj[b]) / np.linalg.norm(traj[b])))
+            errors[name] = e
+            r.close()
+        log("omega rel err at 1000 steps: " + ", ".join(f"{k}={v[-1]:.2e}" for k, v in errors.items()))
+    ctx.close()
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
+    if cpu:
+        log(f"cpu baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": N / (ms_frame * 1e-3),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_frame,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "dtype_detail": "C in BF16 hi+lo on MFMA with FP32 accumulation; FP64 omega update; FP32 Phi with FP64 "
+                        "sums; FP32 particles",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 closed box, K={K} analytic DEC eigenbasis, {N} particles, dt=0.01, "
+                               f"one integrate() frame per step", "grid": n, "K": K, "particles": N},
+        "reduced_steps_per_s": res.get("bf16x2"),
+        "reduced_steps_per_s_by_path": res,
+        "reduced_speedup_vs_plain_fp32": res["bf16x2"] / res["plain_fp32"],
+        "omega_rel_err_vs_fp64_every_100_steps": errors,
+        "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                             "samples": int(lat.size)},
+        "phase_ms": {"reduced_step": ms_st, "reconstruct": ms_rec, "trace": ms_tr},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_reconstruct<float>", "algorithmic_bytes_per_launch": rec_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -90,6 +90,7 @@ typedef struct psf_scene_desc {
 
 /* ---- context --------------------------------------------------------------- */
 PSF_API int psf_ctx_create(int device, psf_ctx** out);
+/* also destroys every scene still alive on the context; their handles become invalid */
 PSF_API int psf_ctx_destroy(psf_ctx* ctx);
 PSF_API const char* psf_last_error(const psf_ctx* ctx);
 PSF_API int psf_ctx_sync(psf_ctx* ctx);
@@ -105,8 +106,9 @@ PSF_API int psf_scene_destroy(psf_scene* scene);
 /* bytes of device memory the scene holds (admission by footprint) */
 PSF_API int psf_scene_footprint(const psf_scene_desc* desc, uint64_t* bytes);
 
-/* Phi: F x K column-major (AbstractSolver::velBasisField); lambda: K
-   (eigenValues); gravity: K (gravity = Phi^T f, psfsolver.cpp:344-358). */
+/* Phi: F x K column-major (AbstractSolver::velBasisField), or NULL for a
+   scene that only runs reduced steps; lambda: K (eigenValues); gravity: K
+   (gravity = Phi^T f, psfsolver.cpp:344-358), or NULL for zero. */
 PSF_API int psf_scene_upload_basis(psf_scene* scene, const double* Phi, const double* lambda,
                            const double* gravity);
 /* C: K matrices K x K, each column-major, matrix i at C + i*K*K

@@ -75,3 +75,96 @@ def build_advection(g: Grid, resolution: float, inside, Phi_cm, lam, V_cm=None, 
         if progress:
             progress(i)
     return C
+
+
+# ---------------------------------------------------------------------------
+# Closed boxes with the analytic sin/cos basis.
+#
+# Every voxel is inside and every mode vanishes exactly on wall faces, so a
+# slot term survives only when both of its faces are interior; interior faces
+# are created BACK/TOP/RIGHT, which fixes each slot's normals (hence c_s) and
+# its edge for all surviving terms.  Each mode is a product of 1-D profiles
+# (node sine / cell cosine) along x, y and z, so the voxel sum of a slot,
+#     T_s[i, j, k] = sum_v V[e_s(v), k] Phi[a_s(v), i] Phi[b_s(v), j],
+# factorises into three 1-D contractions over the distinct frequency pairs of
+# each axis, with V = curl Phi kept as computed (its wall edges included).
+
+# per face slot f1..f6 of voxel (x,y,z): (face axis, offset along that axis)
+_FACE_OF_SLOT = [("z", 0), ("z", 1), ("y", 0), ("y", 1), ("x", 0), ("x", 1)]
+# edge of each slot for interior creators (BACK/TOP/RIGHT edge lists):
+# (edge axis, offset dx, dy, dz) relative to the voxel
+_SLOT_EDGE = [("x", 0, 1, 0), ("y", 0, 0, 0), ("x", 0, 0, 0), ("y", 1, 0, 0),     # f1 (BACK at z)
+              ("x", 0, 1, 1), ("y", 0, 0, 1), ("x", 0, 0, 1), ("y", 1, 0, 1),     # f2 (BACK at z+1)
+              ("z", 0, 1, 0), ("z", 0, 0, 0),                                     # f5 (RIGHT at x) e1, e3
+              ("z", 1, 1, 0), ("z", 1, 0, 0)]                                     # f6 (RIGHT at x+1) e1, e3
+_INTERIOR_NORMAL = {"z": np.array([0.0, 0.0, 1.0]), "y": np.array([0.0, -1.0, 0.0]), "x": np.array([-1.0, 0.0, 0.0])}
+
+
+def _profile_table(n, freqs, kind, off):
+    """values over voxel coordinate v = 0..n-1: node sine at v+off or cell cosine at v."""
+    v = np.arange(n)
+    out = np.empty((len(freqs), n))
+    for r, f in enumerate(freqs):
+        if kind == "node":
+            s = np.sin(f * np.pi * (v + off) / n)
+            s[(v + off) == 0] = 0.0
+            s[(v + off) == n] = 0.0
+            out[r] = s
+        else:
+            out[r] = np.cos(f * np.pi * (v + 0.5) / n)
+    return out
+
+
+def build_advection_box_analytic(g: Grid, resolution: float, Phi_cm, lam, table, V_cm=None, kchunk=64):
+    """C (K, K, K) for a closed box whose basis is analytic_box_modes(g, K)
+    (table = its mode table, amplitudes normalised)."""
+    K = Phi_cm.shape[0]
+    if V_cm is None:
+        V_cm = (curl_operator(g, resolution) @ Phi_cm.T).T        # (K, E)
+    dims = {"x": g.nx, "y": g.ny, "z": g.nz}
+    freq = {"x": table[:, 1].astype(int), "y": table[:, 2].astype(int), "z": table[:, 3].astype(int)}
+    amp = {"x": table[:, 4], "y": table[:, 5], "z": table[:, 6]}
+    uniq = {ax: np.unique(freq[ax]) for ax in "xyz"}
+    pos = {ax: np.searchsorted(uniq[ax], freq[ax]) for ax in "xyz"}
+    x, y, z = g.voxel_coords()
+    one = np.ones(3)
+    sh = scale_sh(resolution)
+    C = np.zeros((K, K, K))
+    for s, (fs, es, a, b) in enumerate(SLOTS):
+        fa, fb = _FACE_OF_SLOT[a], _FACE_OF_SLOT[b]
+        c_s = float(np.cross(_INTERIOR_NORMAL[fa[0]], _INTERIOR_NORMAL[fb[0]]) @ one)
+        if c_s == 0.0:
+            continue
+        ax_e, dx, dy, dz = _SLOT_EDGE[s]
+        e_idx = g.edge_index(ax_e, x + dx, y + dy, z + dz)
+        # per axis: profile tables of face a and face b over voxel coordinates
+        G = {}
+        for ax in "xyz":
+            n = dims[ax]
+            ka = ("node", fa[1]) if fa[0] == ax else ("cell", 0)
+            kb = ("node", fb[1]) if fb[0] == ax else ("cell", 0)
+            Pa = _profile_table(n, uniq[ax], *ka)
+            Pb = _profile_table(n, uniq[ax], *kb)
+            G[ax] = (Pa[:, None, :] * Pb[None, :, :]).reshape(-1, n)     # (D*D, n)
+        Dz, Dy, Dx = G["z"].shape[0], G["y"].shape[0], G["x"].shape[0]
+        ia = amp[fa[0]]
+        ib = amp[fb[0]]
+        # pair index of (i, j) per axis
+        du = {ax: len(uniq[ax]) for ax in "xyz"}
+        pz = pos["z"][:, None] * du["z"] + pos["z"][None, :]
+        py = pos["y"][:, None] * du["y"] + pos["y"][None, :]
+        px = pos["x"][:, None] * du["x"] + pos["x"][None, :]
+        for k0 in range(0, K, kchunk):
+            k1 = min(K, k0 + kchunk)
+            Vs = V_cm[k0:k1, e_idx].reshape(k1 - k0, g.nz, g.ny, g.nx)         # [k, z, y, x]
+            T1 = np.tensordot(Vs, G["z"], axes=([1], [1]))                     # [k, y, x, Dz]
+            T2 = np.tensordot(T1, G["y"], axes=([1], [1]))                     # [k, x, Dz, Dy]
+            T3 = np.tensordot(T2, G["x"], axes=([1], [1]))                     # [k, Dz, Dy, Dx]
+            T3 = T3.reshape(k1 - k0, -1)
+            flat = (pz * Dy + py) * Dx + px                                    # [i, j]
+            Tij = T3[:, flat]                                                  # [k, i, j]
+            Tij = Tij * (ia[None, :, None] * ib[None, None, :])
+            anti = Tij - np.transpose(Tij, (0, 2, 1))                          # T[i,j] - T[j,i]
+            C[:, :, k0:k1] += np.transpose(anti, (1, 2, 0)) * (sh * c_s)
+    C /= np.asarray(lam)[:, None, None]
+    return C

@@ -93,7 +93,8 @@ int check_status(psf_scene* s)
     int st = 0;
     HIPCHK(s->ctx, hipMemcpyAsync(&st, s->dev.status, sizeof(int), hipMemcpyDeviceToHost, s->ctx->stream));
     HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
-    if (st) return set_err(s->ctx, PSF_ERR_NONFINITE, "energy renormalisation saw a zero or non-finite energy");
+    if (st == 1) return set_err(s->ctx, PSF_ERR_NONFINITE, "energy renormalisation saw a zero or non-finite energy");
+    if (st) return set_err(s->ctx, PSF_ERR_HIP, "persistent step: the exchange of y between workgroups timed out");
     return PSF_OK;
 }
 
@@ -204,6 +205,7 @@ int psf_ctx_destroy(psf_ctx* ctx)
 {
     if (!ctx) return PSF_ERR_ARG;
     (void)hipSetDevice(ctx->device);
+    while (!ctx->scenes.empty()) psf_scene_destroy(*ctx->scenes.begin());
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PSF_OK;
@@ -291,6 +293,7 @@ int psf_scene_create(psf_ctx* ctx, const psf_scene_desc* desc, psf_scene** out)
         return set_err(ctx, PSF_ERR_HIP, "scene init: %s", hipGetErrorString(e));
     }
     h->lambda.assign(K, 0.0);
+    ctx->scenes.insert(s);
     *out = s;
     return PSF_OK;
 }
@@ -303,6 +306,7 @@ int psf_scene_destroy(psf_scene* s)
     persistent_scene_free(s);
     dfree(s->dev.omega); dfree(s->dev.lambda); dfree(s->dev.gravity); dfree(s->dev.visc); dfree(s->dev.y);
     dfree(s->dev.status); dfree(s->dev.C); dfree(s->dev.Phi); dfree(s->dev.u); dfree(s->dev.parts); dfree(s->dev.cells);
+    s->ctx->scenes.erase(s);
     delete host_of(s);
     delete s;
     return PSF_OK;
@@ -310,31 +314,34 @@ int psf_scene_destroy(psf_scene* s)
 
 int psf_scene_upload_basis(psf_scene* s, const double* Phi, const double* lambda, const double* gravity)
 {
-    if (!s || !Phi || !lambda) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    if (!s || !lambda) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
     psf_ctx* ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const size_t K = s->d.K, F = s->F, ld = s->dev.ldF;
     for (size_t k = 0; k < K; ++k)
         if (!std::isfinite(lambda[k])) return set_err(ctx, PSF_ERR_ARG, "lambda[%zu] is not finite", k);
-    // column by column through a pinned staging buffer, converted to the storage type
-    const size_t esz = s->d.phi_precision == PSF_PHI_FP32 ? 4 : 8;
-    void* stage = nullptr;
-    HIPCHK(ctx, hipHostMalloc(&stage, F * esz, hipHostMallocDefault));
     int rc = PSF_OK;
-    for (size_t k = 0; k < K && rc == PSF_OK; ++k) {
-        const double* col = Phi + k * F;
-        if (esz == 4) {
-            float* st = static_cast<float*>(stage);
-            for (size_t f = 0; f < F; ++f) st[f] = static_cast<float>(col[f]);
-        } else {
-            std::memcpy(stage, col, F * 8);
+    if (Phi) {
+        // column by column through a pinned staging buffer, converted to the storage type
+        const size_t esz = s->d.phi_precision == PSF_PHI_FP32 ? 4 : 8;
+        void* stage = nullptr;
+        HIPCHK(ctx, hipHostMalloc(&stage, F * esz, hipHostMallocDefault));
+        for (size_t k = 0; k < K && rc == PSF_OK; ++k) {
+            const double* col = Phi + k * F;
+            if (esz == 4) {
+                float* sg = static_cast<float*>(stage);
+                for (size_t f = 0; f < F; ++f) sg[f] = static_cast<float>(col[f]);
+            } else {
+                std::memcpy(stage, col, F * 8);
+            }
+            hipError_t e = hipMemcpyAsync(static_cast<unsigned char*>(s->dev.Phi) + k * ld * esz, stage, F * esz,
+                                          hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "Phi upload: %s", hipGetErrorString(e));
         }
-        hipError_t e = hipMemcpyAsync(static_cast<unsigned char*>(s->dev.Phi) + k * ld * esz, stage, F * esz,
-                                      hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "Phi upload: %s", hipGetErrorString(e));
+        (void)hipHostFree(stage);
+        if (rc == PSF_OK) s->has_phi = true;
     }
-    (void)hipHostFree(stage);
     if (rc) return rc;
     host_of(s)->lambda.assign(lambda, lambda + K);
     HIPCHK(ctx, hipMemcpyAsync(s->dev.lambda, lambda, K * 8, hipMemcpyHostToDevice, ctx->stream));
@@ -461,7 +468,7 @@ int psf_step(psf_scene* s, int nsteps)
 int psf_reconstruct(psf_scene* s)
 {
     if (!s) return PSF_ERR_ARG;
-    if (!s->has_basis || !s->has_omega) return set_err(s->ctx, PSF_ERR_STATE, "basis or omega missing");
+    if (!s->has_phi || !s->has_omega) return set_err(s->ctx, PSF_ERR_STATE, "Phi or omega missing");
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
     return enqueue_reconstruct(s);
 }
@@ -493,6 +500,7 @@ int psf_integrate(psf_scene* s, int nframes)
     if (rc) return rc;
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
     for (int f = 0; f < nframes; ++f) {
+        if (!s->has_phi) return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
         if ((rc = enqueue_steps(s, 1))) return rc;
         if ((rc = enqueue_reconstruct(s))) return rc;
         if ((rc = enqueue_trace(s, nullptr))) return rc;
@@ -507,6 +515,7 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
         int rc = ready_for_step(s);
         if (rc) return rc;
     }
+    if ((phase == 1 || phase == 3) && !s->has_phi) return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
     psf_ctx* ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     hipEvent_t a, b;

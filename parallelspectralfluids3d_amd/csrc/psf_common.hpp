@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <set>
 #include <string>
 
 #include "psf/psf.h"
@@ -39,6 +40,7 @@ struct psf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    std::set<psf_scene*> scenes;   // live scenes; destroyed with the context
 };
 
 struct psf_scene {
@@ -47,7 +49,7 @@ struct psf_scene {
     uint32_t F = 0, V = 0;
     uint32_t nZF = 0, nYF = 0, nXF = 0;
     psf::DeviceScene dev;
-    bool has_basis = false, has_C = false, has_omega = false;
+    bool has_basis = false, has_phi = false, has_C = false, has_omega = false;
     double dt = 1.0 / 60.0, nu = 0.0;
     int gravity_active = 0;
     float lifetime = 60.0f;

@@ -1,13 +1,509 @@
-// Persistent reduced-step path (MFMA precisions).
+// Persistent reduced-step path on MFMA (BF16, FP16, BF16 hi+lo).
+//
+// One cooperative launch runs n steps of psfsolver.cpp:34-59.
+//
+// Storage.  Only the symmetric part of C_i enters y_i = w^T C_i w, so C_i is
+// replaced by S_i = (C_i + C_i^T)/2 and stored as its 16x16 tiles T_pq with
+// p <= q (T_pp = S_pp, T_pq = 2 S_pq): y_i = sum_{p<=q} w_p^T T_pq w_q.
+// An MFMA block is the A operand of one v_mfma_f32_16x16x32: 16 rows (the
+// row tile p) by 32 reduction entries, either two column tiles (q1, q2) of a
+// single-precision tensor or the hi and lo parts of one tile.  Blocks are
+// stored pre-swizzled in the per-lane register layout (16 bytes per lane, a
+// coalesced 1 KiB load per wave) and are loaded into VGPRs once per launch.
+//
+// w enters as two B columns: column 0 holds w_hi, column 1 w_lo (w - w_hi,
+// scaled for FP16), so one MFMA gives T (w_hi + w_lo) with FP32 accumulation.
+// The row contraction with w_p and all reductions over rows, blocks, waves
+// and workgroups are FP64.
+//
+// Exchange.  Workgroup g owns modes i = g, g+G, ...  After the contraction it
+// publishes y_i as two 64-bit words (step tag << 32 | high or low half of the
+// FP64 bits) with relaxed agent-scope atomic stores, and every workgroup
+// polls all K pairs until both tags match.  Each word carries its own
+// validity, so no release/acquire fences (L2 write-back/invalidate) are
+// needed, and the y values arrive exactly.  Buffers alternate by tag parity:
+// a workgroup can only overwrite a buffer two steps later, after every
+// workgroup has consumed it.  Every workgroup then applies the same FP64
+// update (energy renormalisation, viscosity, forcing) to its own copy of w,
+// in a fixed order, so all copies stay bitwise identical.  Waits are bounded
+// (2 s of device time); on expiry every workgroup leaves the loop and the
+// status word reports it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
 #include "psf_persistent.hpp"
 
 namespace psf {
 
-bool persistent_supported(int, int) { return false; }
-size_t persistent_c_bytes(int, int) { return 0; }
-hipError_t persistent_scene_init(psf_scene*) { return hipSuccess; }
-void persistent_scene_free(psf_scene*) {}
-int persistent_upload_c(psf_scene*, const double*) { return PSF_ERR_UNSUPPORTED; }
-hipError_t launch_persistent_steps(psf_scene*, int, hipStream_t) { return hipErrorNotSupported; }
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTile = 16;
+constexpr int kRB = 40;          // resident MFMA blocks per wave
+constexpr int kMaxK = 1024;      // LDS capacity for w, y
+constexpr int kMaxMeta = 4096;   // blocks per workgroup
+constexpr int kMaxIpw = 8;       // modes per workgroup
+constexpr int kMaxCUs = 256;
+constexpr uint64_t kWaitTicks = 200000000ull;   // s_memrealtime runs at 100 MHz: 2 s
+
+struct PersistentState {
+    int G = 0, ipw = 0, Kp = 0, nb = 0, nBi = 0, bpw = 0;
+    int elem = 0;                // 0 bf16, 1 fp16
+    float col1 = 1.0f;           // scale of the w_lo column
+    float wscale = 1.0f;         // w is multiplied by this before FP16 rounding
+    uint4* blocks = nullptr;     // [G][bpw][64]
+    uint32_t* meta = nullptr;    // [bpw]: t << 24 | p << 16 | q1 << 8 | q2
+    double* inv_scale = nullptr; // [G * ipw]
+    unsigned long long* xbuf = nullptr;   // [2][K][2]
+    uint32_t tag = 0;
+};
+
+struct PArgs {
+    const uint4* blocks;
+    const uint32_t* meta;
+    const double* inv_scale;
+    unsigned long long* xbuf;
+    double* omega;
+    const double* visc;
+    const double* gravity;
+    int* status;
+    double dt;
+    int gravity_active;
+    int K, Kp, G, ipw, bpw, nsteps;
+    uint32_t tag0;
+    float col1, wscale;
+};
+
+static inline PersistentState* st(psf_scene* s) { return reinterpret_cast<PersistentState*>(s->persistent); }
+
+int blocks_per_mode(int prec, int nb)
+{
+    if (prec == PSF_C_BF16X2) return nb * (nb + 1) / 2;
+    int n = 0;
+    for (int p = 0; p < nb; ++p) n += (nb - p + 1) / 2;
+    return n;
+}
+
+static void layout(int prec, int K, int& G, int& ipw, int& nb, int& nBi, int& bpw)
+{
+    G = K < kMaxCUs ? K : kMaxCUs;
+    ipw = (K + G - 1) / G;
+    nb = (K + kTile - 1) / kTile;
+    nBi = blocks_per_mode(prec, nb);
+    bpw = ipw * nBi;
+}
+
+bool persistent_supported(int prec, int K)
+{
+    if (prec != PSF_C_BF16 && prec != PSF_C_FP16 && prec != PSF_C_BF16X2) return false;
+    if (K < 1 || K > kMaxK) return false;
+    int G, ipw, nb, nBi, bpw;
+    layout(prec, K, G, ipw, nb, nBi, bpw);
+    return ipw <= kMaxIpw && bpw <= kMaxMeta;
+}
+
+size_t persistent_c_bytes(int prec, int K)
+{
+    if (!persistent_supported(prec, K)) return 0;
+    int G, ipw, nb, nBi, bpw;
+    layout(prec, K, G, ipw, nb, nBi, bpw);
+    return size_t(G) * bpw * 64 * 16;
+}
+
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return static_cast<uint16_t>(b >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double* red)
+{
+    v = wave_sum_d(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+template <int ELEM>
+__device__ __forceinline__ f32x4 mfma16(const uint4& a, const uint4& b, const f32x4& c)
+{
+    if constexpr (ELEM == 0)
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+}
+
+template <int ELEM>
+__global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
+{
+    __shared__ double s_w[kMaxK];
+    __shared__ double s_y[kMaxK];
+    __shared__ __align__(16) uint16_t s_whi[kMaxK];
+    __shared__ __align__(16) uint16_t s_wlo[kMaxK];
+    __shared__ uint32_t s_meta[kMaxMeta];
+    __shared__ double s_red[4];
+    __shared__ double s_ypart[4][kMaxIpw];
+    __shared__ int s_flag;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = blockIdx.x;
+    const int K = a.K, Kp = a.Kp;
+    const int col = lane & 15, kg = lane >> 4;
+
+    for (int b = tid; b < a.bpw; b += 256) s_meta[b] = a.meta[b];
+    for (int k = tid; k < Kp; k += 256) s_w[k] = k < K ? a.omega[k] : 0.0;
+    if (tid == 0) s_flag = 0;
+
+    // this wave's block range and its resident part
+    const int per = (a.bpw + 3) / 4;
+    const int b0 = wv * per;
+    const int nblk = max(0, min(a.bpw - b0, per));
+    const int nres = min(nblk, kRB);
+    const uint4* gblk = a.blocks + (static_cast<size_t>(g) * a.bpw + b0) * 64 + lane;
+    uint4 areg[kRB];
+#pragma unroll
+    for (int j = 0; j < kRB; ++j)
+        if (j < nres) areg[j] = gblk[static_cast<size_t>(j) * 64];
+    __syncthreads();
+
+    auto split_w = [&]() {
+        for (int k = tid; k < Kp; k += 256) {
+            const double w = s_w[k];
+            uint16_t hi, lo;
+            if constexpr (ELEM == 0) {
+                hi = f32_to_bf16_rne(static_cast<float>(w));
+                lo = f32_to_bf16_rne(static_cast<float>(w - static_cast<double>(bf16_to_f32(hi))));
+            } else {
+                const double ws = w * static_cast<double>(a.wscale);
+                const _Float16 h = static_cast<_Float16>(static_cast<float>(ws));
+                const _Float16 l = static_cast<_Float16>(static_cast<float>((ws - static_cast<double>(h)) * 2048.0));
+                hi = __builtin_bit_cast(uint16_t, h);
+                lo = __builtin_bit_cast(uint16_t, l);
+            }
+            s_whi[k] = hi;
+            s_wlo[k] = lo;
+        }
+    };
+    split_w();
+    __syncthreads();
+
+    for (int s = 0; s < a.nsteps; ++s) {
+        const uint32_t tag = a.tag0 + static_cast<uint32_t>(s) + 1u;
+        if (tid < 4 * kMaxIpw) s_ypart[tid / kMaxIpw][tid % kMaxIpw] = 0.0;
+        __syncthreads();
+        // ---- contraction of this wave's blocks -------------------------------
+        {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            int cur = -1;              // (t << 8) | p of the open accumulator
+            double part = 0.0;
+            auto close_row = [&]() {
+                if (cur < 0) return;
+                const int p = cur & 255;
+                double c = 0.0;
+                if (col < 2) {
+                    const double* wr = s_w + p * kTile + 4 * kg;
+                    c = static_cast<double>(acc[0]) * wr[0] + static_cast<double>(acc[1]) * wr[1] +
+                        static_cast<double>(acc[2]) * wr[2] + static_cast<double>(acc[3]) * wr[3];
+                    if (col == 1) c *= static_cast<double>(a.col1);
+                }
+                part += c;
+            };
+            auto close_mode = [&]() {
+                if (cur < 0) return;
+                const double v = wave_sum_d(part);
+                if (lane == 0) s_ypart[wv][cur >> 8] += v;
+                part = 0.0;
+            };
+            auto process = [&](int j, const uint4& av) {
+                const uint32_t m = s_meta[b0 + j];
+                const int key = static_cast<int>(m >> 16);
+                if (key != cur) {
+                    close_row();
+                    if (cur >= 0 && (key >> 8) != (cur >> 8)) close_mode();
+                    acc = f32x4{0.f, 0.f, 0.f, 0.f};
+                    cur = key;
+                }
+                const int q = (kg < 2) ? static_cast<int>((m >> 8) & 255u) : static_cast<int>(m & 255u);
+                const int off = q * kTile + (kg & 1) * 8;
+                uint4 bv = {0u, 0u, 0u, 0u};
+                if (col == 0) bv = *reinterpret_cast<const uint4*>(s_whi + off);
+                else if (col == 1) bv = *reinterpret_cast<const uint4*>(s_wlo + off);
+                acc = mfma16<ELEM>(av, bv, acc);
+            };
+#pragma unroll
+            for (int j = 0; j < kRB; ++j)
+                if (j < nres) process(j, areg[j]);
+            for (int j = kRB; j < nblk; ++j) process(j, gblk[static_cast<size_t>(j) * 64]);
+            close_row();
+            close_mode();
+        }
+        __syncthreads();
+        // ---- publish y_i of this workgroup ---------------------------------------
+        unsigned long long* xb = a.xbuf + static_cast<size_t>(tag & 1u) * K * 2;
+        if (tid < a.ipw) {
+            const int i = g + a.G * tid;
+            if (i < K) {
+                const double y = (((s_ypart[0][tid] + s_ypart[1][tid]) + s_ypart[2][tid]) + s_ypart[3][tid]) *
+                                 a.inv_scale[g * a.ipw + tid];
+                const unsigned long long bits = __double_as_longlong(y);
+                const unsigned long long tg = static_cast<unsigned long long>(tag) << 32;
+                __hip_atomic_store(xb + 2 * i, tg | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(xb + 2 * i + 1, tg | (bits & 0xFFFFFFFFull), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        // ---- gather all y (bounded wait) ---------------------------------------------
+        {
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            for (int i = tid; i < K; i += 256) {
+                unsigned long long v0, v1;
+                for (;;) {
+                    v0 = __hip_atomic_load(xb + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    v1 = __hip_atomic_load(xb + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((v0 >> 32) == tag && (v1 >> 32) == tag) break;
+                    if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                        s_flag = 2;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                s_y[i] = __longlong_as_double(static_cast<long long>(((v0 & 0xFFFFFFFFull) << 32) | (v1 & 0xFFFFFFFFull)));
+            }
+        }
+        __syncthreads();
+        if (s_flag) break;
+        // ---- FP64 update, identical in every workgroup (psfsolver.cpp:34-59) -----------
+        {
+#pragma clang fp contract(off)
+            double s1 = 0.0;
+            for (int k = tid; k < K; k += 256) s1 += s_w[k] * s_w[k];
+            const double e1 = block_sum_d(s1, s_red);
+            double s2 = 0.0;
+            for (int k = tid; k < K; k += 256) {
+                const double wn = s_w[k] + a.dt * s_y[k];
+                s_y[k] = wn;
+                s2 += wn * wn;
+            }
+            const double e2 = block_sum_d(s2, s_red);
+            if (!(e2 > 0.0) || !__builtin_isfinite(e2) || !__builtin_isfinite(e1)) {
+                if (tid == 0) s_flag = 1;
+                __syncthreads();
+                break;
+            }
+            const double fac = sqrt(e1 / e2);
+            for (int k = tid; k < K; k += 256) {
+                double w = s_y[k] * fac;
+                w = w * a.visc[k];
+                if (a.gravity_active) w = w + a.dt * a.gravity[k];
+                s_w[k] = w;
+            }
+        }
+        __syncthreads();
+        split_w();
+        __syncthreads();
+    }
+    __syncthreads();
+    if (g == 0) {
+        for (int k = tid; k < K; k += 256) a.omega[k] = s_w[k];
+        if (tid == 0 && s_flag) *a.status = s_flag;
+    }
+}
+
+// ---------------------------------------------------------------------------
+hipError_t persistent_scene_init(psf_scene* s)
+{
+    const int prec = s->d.c_precision;
+    if (!persistent_supported(prec, static_cast<int>(s->d.K))) return hipSuccess;
+    PersistentState* p = new (std::nothrow) PersistentState();
+    if (!p) return hipErrorOutOfMemory;
+    s->persistent = p;
+    const int K = static_cast<int>(s->d.K);
+    layout(prec, K, p->G, p->ipw, p->nb, p->nBi, p->bpw);
+    p->Kp = p->nb * kTile;
+    p->elem = prec == PSF_C_FP16 ? 1 : 0;
+    p->col1 = prec == PSF_C_FP16 ? 1.0f / 2048.0f : 1.0f;
+    p->wscale = prec == PSF_C_FP16 ? 256.0f : 1.0f;
+    p->blocks = static_cast<uint4*>(s->dev.C);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->meta), sizeof(uint32_t) * p->bpw);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->inv_scale), sizeof(double) * p->G * p->ipw);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->xbuf), sizeof(unsigned long long) * 4 * K);
+    if (e == hipSuccess) e = hipMemset(p->xbuf, 0, sizeof(unsigned long long) * 4 * K);
+    if (e != hipSuccess) return e;
+    // block metadata (identical for every workgroup)
+    std::vector<uint32_t> meta;
+    meta.reserve(p->bpw);
+    for (int t = 0; t < p->ipw; ++t)
+        for (int q0 = 0; q0 < p->nb; ++q0) {
+            const int pr = q0;
+            if (prec == PSF_C_BF16X2) {
+                for (int q = pr; q < p->nb; ++q)
+                    meta.push_back(uint32_t(t) << 24 | uint32_t(pr) << 16 | uint32_t(q) << 8 | uint32_t(q));
+            } else {
+                for (int q = pr; q < p->nb; q += 2) {
+                    const int q2 = q + 1 < p->nb ? q + 1 : q;
+                    meta.push_back(uint32_t(t) << 24 | uint32_t(pr) << 16 | uint32_t(q) << 8 | uint32_t(q2));
+                }
+            }
+        }
+    return hipMemcpy(p->meta, meta.data(), sizeof(uint32_t) * meta.size(), hipMemcpyHostToDevice);
+}
+
+void persistent_scene_free(psf_scene* s)
+{
+    PersistentState* p = st(s);
+    if (!p) return;
+    if (p->meta) (void)hipFree(p->meta);
+    if (p->inv_scale) (void)hipFree(p->inv_scale);
+    if (p->xbuf) (void)hipFree(p->xbuf);
+    delete p;
+    s->persistent = nullptr;
+}
+
+namespace {
+uint16_t bf16_rne_host(float f)
+{
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return static_cast<uint16_t>(b >> 16);
+}
+float bf16_to_float_host(uint16_t h)
+{
+    uint32_t b = static_cast<uint32_t>(h) << 16;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+uint16_t f16_host(double v)
+{
+    const _Float16 h = static_cast<_Float16>(static_cast<float>(v));
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+}  // namespace
+
+int persistent_upload_c(psf_scene* s, const double* C)
+{
+    PersistentState* p = st(s);
+    if (!p) {
+        s->ctx->err = "persistent path not initialised for this precision";
+        return PSF_ERR_UNSUPPORTED;
+    }
+    const int prec = s->d.c_precision;
+    const int K = static_cast<int>(s->d.K), Kp = p->Kp, nb = p->nb;
+    std::vector<uint16_t> host(size_t(p->G) * p->bpw * 64 * 8, 0);
+    std::vector<double> inv(size_t(p->G) * p->ipw, 0.0);
+    std::vector<double> T(size_t(Kp) * Kp);   // symmetrised, tiled factor applied
+    for (int g = 0; g < p->G; ++g)
+        for (int t = 0; t < p->ipw; ++t) {
+            const int i = g + p->G * t;
+            if (i >= K) continue;
+            const double* Ci = C + size_t(i) * K * K;     // C_i(a, b) at b*K + a
+            std::fill(T.begin(), T.end(), 0.0);
+            double amax = 0.0;
+            for (int a = 0; a < K; ++a)
+                for (int b = 0; b < K; ++b) {
+                    const int pa = a / kTile, qb = b / kTile;
+                    if (pa > qb) continue;
+                    double v = 0.5 * (Ci[size_t(b) * K + a] + Ci[size_t(a) * K + b]);
+                    if (pa < qb) v *= 2.0;
+                    T[size_t(a) * Kp + b] = v;
+                    amax = std::fmax(amax, std::fabs(v));
+                }
+            double scale = 1.0;
+            if (prec == PSF_C_FP16 && amax > 0.0) scale = std::ldexp(1.0, static_cast<int>(std::floor(std::log2(16384.0 / amax))));
+            inv[size_t(g) * p->ipw + t] = 1.0 / (scale * p->wscale);
+            // blocks of mode i in metadata order
+            int bi = 0;
+            for (int pr = 0; pr < nb; ++pr) {
+                auto emit = [&](int q1, int q2, bool hilo) {
+                    const size_t base = ((size_t(g) * p->bpw) + size_t(t) * p->nBi + bi) * 64 * 8;
+                    for (int l = 0; l < 64; ++l) {
+                        const int r = l & 15, kgp = l >> 4;
+                        for (int e = 0; e < 8; ++e) {
+                            const int k = 8 * kgp + e;
+                            const int row = pr * kTile + r;
+                            uint16_t out = 0;
+                            if (hilo) {
+                                const double v = T[size_t(row) * Kp + q1 * kTile + (k & 15)];
+                                const uint16_t hi = bf16_rne_host(static_cast<float>(v));
+                                out = k < 16 ? hi : bf16_rne_host(static_cast<float>(v - bf16_to_float_host(hi)));
+                            } else {
+                                const int q = k < 16 ? q1 : q2;
+                                const bool zero = (k >= 16 && q2 == q1);
+                                const double v = zero ? 0.0 : T[size_t(row) * Kp + q * kTile + (k & 15)] * scale;
+                                out = prec == PSF_C_FP16 ? f16_host(v) : bf16_rne_host(static_cast<float>(v));
+                            }
+                            host[base + size_t(l) * 8 + e] = out;
+                        }
+                    }
+                    ++bi;
+                };
+                if (prec == PSF_C_BF16X2) {
+                    for (int q = pr; q < nb; ++q) emit(q, q, true);
+                } else {
+                    for (int q = pr; q < nb; q += 2) emit(q, q + 1 < nb ? q + 1 : q, false);
+                }
+            }
+        }
+    hipError_t e = hipMemcpy(p->blocks, host.data(), host.size() * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->inv_scale, inv.data(), inv.size() * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        s->ctx->err = std::string("persistent upload: ") + hipGetErrorString(e);
+        return PSF_ERR_HIP;
+    }
+    return PSF_OK;
+}
+
+hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream)
+{
+    PersistentState* p = st(s);
+    if (!p) return hipErrorNotSupported;
+    if (nsteps <= 0) return hipSuccess;
+    PArgs a{};
+    a.blocks = p->blocks;
+    a.meta = p->meta;
+    a.inv_scale = p->inv_scale;
+    a.xbuf = p->xbuf;
+    a.omega = s->dev.omega;
+    a.visc = s->dev.visc;
+    a.gravity = s->dev.gravity;
+    a.status = s->dev.status;
+    a.dt = s->dt;
+    a.gravity_active = s->gravity_active;
+    a.K = static_cast<int>(s->d.K);
+    a.Kp = p->Kp;
+    a.G = p->G;
+    a.ipw = p->ipw;
+    a.bpw = p->bpw;
+    a.nsteps = nsteps;
+    a.tag0 = p->tag;
+    a.col1 = p->col1;
+    a.wscale = p->wscale;
+    void* args[] = {&a};
+    const void* fn = p->elem == 0 ? reinterpret_cast<const void*>(&k_persistent<0>)
+                                  : reinterpret_cast<const void*>(&k_persistent<1>);
+    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(p->G), dim3(256), args, 0, stream);
+    if (e == hipSuccess) p->tag += static_cast<uint32_t>(nsteps);
+    return e;
+}
 
 }  // namespace psf

@@ -5,7 +5,9 @@ pointers and copied by the library.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as ct
+import weakref
 
 import numpy as np
 
@@ -17,6 +19,16 @@ def _ptr(a):
     return a.ctypes.data_as(ct.c_void_p)
 
 
+_LIVE_CONTEXTS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    """Release device memory before the HIP runtime is torn down at exit."""
+    for c in list(_LIVE_CONTEXTS):
+        c.close()
+
+
 class Context:
     def __init__(self, device: int = 0):
         self.lib = L.load()
@@ -25,6 +37,8 @@ class Context:
         if rc != 0:
             raise L.PsfError(rc, f"psf_ctx_create(device={device}) failed")
         self.h = h
+        self.scenes = weakref.WeakSet()
+        _LIVE_CONTEXTS.add(self)
 
     def check(self, rc, what=""):
         if rc != 0:
@@ -36,6 +50,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for sc in list(self.scenes):
+                sc.close()
             self.lib.psf_ctx_destroy(self.h)
             self.h = None
 
@@ -78,16 +94,20 @@ class Scene:
         self.K = int(desc.K)
         self.grid = Grid(desc.nx, desc.ny, desc.nz)
         h = ct.c_void_p()
+        if not ctx.h:
+            raise L.PsfError(L.PSF_ERR_STATE, "context is closed")
         ctx.check(ctx.lib.psf_scene_create(ctx.h, ct.byref(desc), ct.byref(h)), "psf_scene_create")
         self.h = h
         self.n_particles = 0
+        ctx.scenes.add(self)
 
     def _c(self, rc, what):
         self.ctx.check(rc, what)
 
     def close(self):
         if getattr(self, "h", None):
-            self.ctx.lib.psf_scene_destroy(self.h)
+            if self.ctx.h:
+                self.ctx.lib.psf_scene_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -98,12 +118,15 @@ class Scene:
 
     # uploads
     def upload_basis(self, Phi_cm, lam, gravity=None):
-        Phi = np.ascontiguousarray(Phi_cm, dtype=np.float64)
-        assert Phi.shape == (self.K, self.grid.nF)
+        """Phi_cm (K, F) float64, or None for a reduced-step-only scene."""
+        Phi = None
+        if Phi_cm is not None:
+            Phi = np.ascontiguousarray(Phi_cm, dtype=np.float64)
+            assert Phi.shape == (self.K, self.grid.nF)
         lam = np.ascontiguousarray(lam, dtype=np.float64)
         g = None if gravity is None else np.ascontiguousarray(gravity, dtype=np.float64)
-        self._c(self.ctx.lib.psf_scene_upload_basis(self.h, _ptr(Phi), _ptr(lam), None if g is None else _ptr(g)),
-                "upload_basis")
+        self._c(self.ctx.lib.psf_scene_upload_basis(self.h, None if Phi is None else _ptr(Phi), _ptr(lam),
+                                                    None if g is None else _ptr(g)), "upload_basis")
 
     def upload_advection(self, C):
         C = np.ascontiguousarray(C, dtype=np.float64)

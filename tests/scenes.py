@@ -45,12 +45,16 @@ def make_scene(n=8, K=16, basis_kind="analytic", obstacle=False, seed=0, dims=No
     amin = tuple(-x for x in half)
     amax = tuple(half)
     inside = sphere_obstacle_mask(g, h, amin) if obstacle else box_mask(g)
+    tab = None
     if basis_kind == "analytic":
-        Phi, lam, _ = basis.analytic_box_modes(g, K)
+        Phi, lam, tab = basis.analytic_box_modes(g, K)
     else:
         Phi, lam = basis.laplacian_eigenbasis(g, h, inside, K)
     V = (basis.curl_operator(g, h) @ Phi.T).T
-    C = advection.build_advection(g, h, inside, Phi, lam, V)
+    if tab is not None and not obstacle:
+        C = advection.build_advection_box_analytic(g, h, Phi, lam, tab, V)
+    else:
+        C = advection.build_advection(g, h, inside, Phi, lam, V)
     gvec = basis.project(Phi, basis.gravity_field(g, inside))
     omega = basis.unit_energy_omega(K, seed)
     return SceneData(g, h, amin, amax, inside, Phi, lam, C, gvec, omega, V)
