@@ -129,7 +129,10 @@ PSF_API int psf_set_step_path(psf_scene* scene, int path);
 PSF_API int psf_set_omega(psf_scene* scene, const double* omega);
 PSF_API int psf_get_omega(psf_scene* scene, double* omega);
 
-/* particles: n records (x, y, z, w) as glm::vec4 (Graphics/Particle/Particle.h:14) */
+/* particles: n records (x, y, z, w) as glm::vec4 (Graphics/Particle/Particle.h:14).
+   The library stores them sorted by containing voxel; get/trace return the
+   caller's order, and respawn draws use the caller's particle index.
+   psf_get_particles must ask for exactly the count that was set. */
 PSF_API int psf_set_particles(psf_scene* scene, const float* xyzw, uint32_t n);
 PSF_API int psf_get_particles(psf_scene* scene, float* xyzw, uint32_t n);
 

@@ -2,8 +2,10 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
+#include <numeric>
 #include <vector>
 
 #include "psf_kernels.hpp"
@@ -156,7 +158,7 @@ int enqueue_reconstruct(psf_scene* s)
 int enqueue_trace(psf_scene* s, int32_t* dcells)
 {
     TraceParams p = trace_params(s);
-    HIPCHK(s->ctx, launch_trace(p, s->dev.u, s->dev.parts, s->n_particles, dcells, s->ctx->stream));
+    HIPCHK(s->ctx, launch_trace(p, s->dev.u, s->dev.parts, s->dev.pid, s->n_particles, dcells, s->ctx->stream));
     s->frame++;
     return PSF_OK;
 }
@@ -173,9 +175,11 @@ int refresh_visc(psf_scene* s, const std::vector<double>& lam)
 
 }  // namespace
 
-// host copy of lambda per scene, kept beside the scene for viscosity refreshes
+// host-side copies kept beside a scene: lambda (viscosity refreshes) and the
+// caller index of every stored particle
 struct psf_scene_host {
     std::vector<double> lambda;
+    std::vector<uint32_t> pid;
 };
 static psf_scene_host* host_of(psf_scene* s) { return reinterpret_cast<psf_scene_host*>(s->host); }
 
@@ -273,6 +277,7 @@ int psf_scene_create(psf_ctx* ctx, const psf_scene_desc* desc, psf_scene** out)
     chk(dalloc(&s->dev.u, s->F));
     chk(dalloc(&s->dev.parts, desc->max_particles));
     chk(dalloc(&s->dev.cells, size_t(desc->max_particles) * 12));
+    chk(dalloc(&s->dev.pid, desc->max_particles));
     if (e == hipSuccess) e = persistent_scene_init(s);
     if (e != hipSuccess) {
         psf_scene_destroy(s);
@@ -305,7 +310,7 @@ int psf_scene_destroy(psf_scene* s)
     (void)hipStreamSynchronize(s->ctx->stream);
     persistent_scene_free(s);
     dfree(s->dev.omega); dfree(s->dev.lambda); dfree(s->dev.gravity); dfree(s->dev.visc); dfree(s->dev.y);
-    dfree(s->dev.status); dfree(s->dev.C); dfree(s->dev.Phi); dfree(s->dev.u); dfree(s->dev.parts); dfree(s->dev.cells);
+    dfree(s->dev.status); dfree(s->dev.C); dfree(s->dev.Phi); dfree(s->dev.u); dfree(s->dev.parts); dfree(s->dev.cells); dfree(s->dev.pid);
     s->ctx->scenes.erase(s);
     delete host_of(s);
     delete s;
@@ -436,12 +441,37 @@ int psf_get_omega(psf_scene* s, double* omega)
     return check_status(s);
 }
 
+// Particles are stored sorted by the voxel that contains them (stable in the
+// caller's order), so the lanes of a wave gather neighbouring face fluxes;
+// every interface below speaks the caller's order.
 int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
 {
     if (!s || (!xyzw && n)) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
     if (n > s->d.max_particles) return set_err(s->ctx, PSF_ERR_ARG, "%u particles exceed max_particles %u", n, s->d.max_particles);
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
-    if (n) HIPCHK(s->ctx, hipMemcpy(s->dev.parts, xyzw, size_t(n) * 16, hipMemcpyHostToDevice));
+    std::vector<uint32_t>& pid = host_of(s)->pid;
+    pid.resize(n);
+    std::iota(pid.begin(), pid.end(), 0u);
+    std::vector<uint32_t> key(n);
+    const double res = s->d.resolution;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t c[3];
+        const uint32_t dims[3] = {s->d.nx, s->d.ny, s->d.nz};
+        for (int a = 0; a < 3; ++a) {
+            const float sa = (s->d.aabb_min[a] + (s->d.aabb_max[a] - s->d.aabb_min[a])) + xyzw[4 * size_t(i) + a];
+            const double q = static_cast<double>(sa) / res;
+            long v = std::isfinite(q) ? static_cast<long>(std::max(-1.0, std::min(q, 1.0e9))) : 0;
+            c[a] = static_cast<uint32_t>(std::max(0L, std::min(v, static_cast<long>(dims[a]) - 1)));
+        }
+        key[i] = (c[2] * s->d.ny + c[1]) * s->d.nx + c[0];
+    }
+    std::stable_sort(pid.begin(), pid.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    std::vector<float> sorted(size_t(n) * 4);
+    for (uint32_t j = 0; j < n; ++j) std::memcpy(&sorted[4 * size_t(j)], xyzw + 4 * size_t(pid[j]), 16);
+    if (n) {
+        HIPCHK(s->ctx, hipMemcpy(s->dev.parts, sorted.data(), size_t(n) * 16, hipMemcpyHostToDevice));
+        HIPCHK(s->ctx, hipMemcpy(s->dev.pid, pid.data(), size_t(n) * 4, hipMemcpyHostToDevice));
+    }
     s->n_particles = n;
     return PSF_OK;
 }
@@ -449,10 +479,13 @@ int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
 int psf_get_particles(psf_scene* s, float* xyzw, uint32_t n)
 {
     if (!s || (!xyzw && n)) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
-    if (n > s->n_particles) return set_err(s->ctx, PSF_ERR_ARG, "asked for %u particles, scene holds %u", n, s->n_particles);
+    if (n != s->n_particles) return set_err(s->ctx, PSF_ERR_ARG, "asked for %u particles, scene holds %u", n, s->n_particles);
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
-    HIPCHK(s->ctx, hipMemcpyAsync(xyzw, s->dev.parts, size_t(n) * 16, hipMemcpyDeviceToHost, s->ctx->stream));
+    std::vector<float> sorted(size_t(n) * 4);
+    HIPCHK(s->ctx, hipMemcpyAsync(sorted.data(), s->dev.parts, size_t(n) * 16, hipMemcpyDeviceToHost, s->ctx->stream));
     HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    const std::vector<uint32_t>& pid = host_of(s)->pid;
+    for (uint32_t j = 0; j < n; ++j) std::memcpy(xyzw + 4 * size_t(pid[j]), &sorted[4 * size_t(j)], 16);
     return PSF_OK;
 }
 
@@ -488,8 +521,12 @@ int psf_trace(psf_scene* s, int32_t* cells)
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
     int rc = enqueue_trace(s, cells ? s->dev.cells : nullptr);
     if (rc || !cells) return rc;
-    HIPCHK(s->ctx, hipMemcpyAsync(cells, s->dev.cells, size_t(s->n_particles) * 48, hipMemcpyDeviceToHost, s->ctx->stream));
+    const uint32_t n = s->n_particles;
+    std::vector<int32_t> sorted(size_t(n) * 12);
+    HIPCHK(s->ctx, hipMemcpyAsync(sorted.data(), s->dev.cells, size_t(n) * 48, hipMemcpyDeviceToHost, s->ctx->stream));
     HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    const std::vector<uint32_t>& pid = host_of(s)->pid;
+    for (uint32_t j = 0; j < n; ++j) std::memcpy(cells + 12 * size_t(pid[j]), &sorted[12 * size_t(j)], 48);
     return PSF_OK;
 }
 

@@ -32,6 +32,7 @@ struct DeviceScene {
     // particles
     float4* parts = nullptr;     // max_particles
     int32_t* cells = nullptr;    // 12 * max_particles (debug output of cell lookups)
+    uint32_t* pid = nullptr;     // caller index of each stored particle (storage is sorted by voxel)
 };
 
 }  // namespace psf

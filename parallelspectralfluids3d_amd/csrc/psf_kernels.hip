@@ -192,7 +192,7 @@ __device__ __forceinline__ uint32_t respawn_draw(uint32_t seed, uint32_t step, u
 }
 
 __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __restrict__ u, float4* __restrict__ parts,
-                                               uint32_t n, int32_t* __restrict__ cells)
+                                               const uint32_t* __restrict__ pid, uint32_t n, int32_t* __restrict__ cells)
 {
 #pragma clang fp contract(off)
     const uint32_t id = blockIdx.x * 256u + threadIdx.x;
@@ -295,9 +295,10 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
 #pragma unroll
             for (int k = 0; k < 12; ++k) c[k] = -1;
         }
-        const uint32_t r0 = respawn_draw(P.seed, P.frame, id, 0);
-        const uint32_t r1 = respawn_draw(P.seed, P.frame, id, 1);
-        const uint32_t r2 = respawn_draw(P.seed, P.frame, id, 2);
+        const uint32_t who = pid[id];   // draws follow the caller's particle index
+        const uint32_t r0 = respawn_draw(P.seed, P.frame, who, 0);
+        const uint32_t r1 = respawn_draw(P.seed, P.frame, who, 1);
+        const uint32_t r2 = respawn_draw(P.seed, P.frame, who, 2);
         p.x = static_cast<float>((r0 / 1024.0 - 0.5) * 0.25);
         p.y = P.aabb_min[1] + 0.2f;
         p.z = static_cast<float>(-static_cast<double>(P.center_z) + (r1 / 1024.0 - 0.5) * 0.25);
@@ -352,11 +353,11 @@ hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F,
     return hipGetLastError();
 }
 
-hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, uint32_t n, int32_t* cells,
-                        hipStream_t s)
+hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, const uint32_t* pid, uint32_t n,
+                        int32_t* cells, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    k_trace<<<(n + 255) / 256, 256, 0, s>>>(p, u, parts, n, cells);
+    k_trace<<<(n + 255) / 256, 256, 0, s>>>(p, u, parts, pid, n, cells);
     return hipGetLastError();
 }
 

@@ -28,8 +28,8 @@ hipError_t launch_update(double* omega, const double* y, const double* visc, con
 hipError_t launch_reconstruct(int phi_precision, const void* Phi, size_t ldF, uint32_t F, int K,
                               const double* omega, float* u, hipStream_t s);
 // one particle pass
-hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, uint32_t n, int32_t* cells,
-                        hipStream_t s);
+hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, const uint32_t* pid, uint32_t n,
+                        int32_t* cells, hipStream_t s);
 hipError_t launch_zero_int(int* p, int n, hipStream_t s);
 
 }  // namespace psf

@@ -44,7 +44,7 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 16;
-constexpr int kRB = 40;          // resident MFMA blocks per wave
+constexpr int kRB = 36;          // largest register budget, in MFMA blocks per wave (K = 256 hi+lo needs 34)
 constexpr int kMaxK = 1024;      // LDS capacity for w, y
 constexpr int kMaxMeta = 4096;   // blocks per workgroup
 constexpr int kMaxIpw = 8;       // modes per workgroup
@@ -124,20 +124,29 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
-__device__ __forceinline__ double wave_sum_d(double v)
+// Wave-wide FP64 sum on DPP lane moves (no LDS traffic): quad butterflies,
+// half-row and row mirrors, then the row broadcasts of lanes 15 and 31; lane
+// 63 ends with the total, which is broadcast.  The order is fixed, so every
+// wave of every workgroup rounds identically.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_move(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+    return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ double block_sum_d(double v, double* red)
+__device__ __forceinline__ double wave_total(double v)
 {
-    v = wave_sum_d(v);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    return ((red[0] + red[1]) + red[2]) + red[3];
+    v += dpp_move<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+    v += dpp_move<0x141, 0xF>(v);   // row_half_mirror
+    v += dpp_move<0x140, 0xF>(v);   // row_mirror
+    v += dpp_move<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
 
 template <int ELEM>
@@ -149,15 +158,16 @@ __device__ __forceinline__ f32x4 mfma16(const uint4& a, const uint4& b, const f3
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
 }
 
-template <int ELEM>
+template <int ELEM, int RB>
 __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
 {
     __shared__ double s_w[kMaxK];
     __shared__ double s_y[kMaxK];
-    __shared__ __align__(16) uint16_t s_whi[kMaxK];
-    __shared__ __align__(16) uint16_t s_wlo[kMaxK];
+    __shared__ __align__(16) uint16_t s_wb[3 * kMaxK];   // w_hi at [0, Kp), w_lo at [Kp, 2 Kp), zeros after
     __shared__ uint32_t s_meta[kMaxMeta];
-    __shared__ double s_red[4];
+    __shared__ double s_visc[kMaxK];
+    __shared__ double s_grav[kMaxK];
+    __shared__ double s_red[8];
     __shared__ double s_ypart[4][kMaxIpw];
     __shared__ int s_flag;
 
@@ -168,91 +178,129 @@ __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
     const int col = lane & 15, kg = lane >> 4;
 
     for (int b = tid; b < a.bpw; b += 256) s_meta[b] = a.meta[b];
-    for (int k = tid; k < Kp; k += 256) s_w[k] = k < K ? a.omega[k] : 0.0;
+    for (int k = tid; k < Kp; k += 256) {
+        s_w[k] = k < K ? a.omega[k] : 0.0;
+        s_visc[k] = k < K ? a.visc[k] : 1.0;
+        s_grav[k] = k < K ? a.gravity[k] : 0.0;
+    }
+    for (int k = tid; k < Kp; k += 256) s_wb[2 * Kp + k] = 0;
     if (tid == 0) s_flag = 0;
 
-    // this wave's block range and its resident part
+    // this wave's block range; the first RB blocks stay in registers, and
+    // register slots past the wave's last block hold zero blocks, so the
+    // resident MFMA chain runs without guards
     const int per = (a.bpw + 3) / 4;
     const int b0 = wv * per;
     const int nblk = max(0, min(a.bpw - b0, per));
-    const int nres = min(nblk, kRB);
+    const int nres = min(nblk, RB);
     const uint4* gblk = a.blocks + (static_cast<size_t>(g) * a.bpw + b0) * 64 + lane;
-    uint4 areg[kRB];
+    uint4 areg[RB];
 #pragma unroll
-    for (int j = 0; j < kRB; ++j)
+    for (int j = 0; j < RB; ++j) {
+        areg[j] = uint4{0u, 0u, 0u, 0u};
         if (j < nres) areg[j] = gblk[static_cast<size_t>(j) * 64];
+    }
     __syncthreads();
-
-    auto split_w = [&]() {
-        for (int k = tid; k < Kp; k += 256) {
-            const double w = s_w[k];
-            uint16_t hi, lo;
-            if constexpr (ELEM == 0) {
-                hi = f32_to_bf16_rne(static_cast<float>(w));
-                lo = f32_to_bf16_rne(static_cast<float>(w - static_cast<double>(bf16_to_f32(hi))));
-            } else {
-                const double ws = w * static_cast<double>(a.wscale);
-                const _Float16 h = static_cast<_Float16>(static_cast<float>(ws));
-                const _Float16 l = static_cast<_Float16>(static_cast<float>((ws - static_cast<double>(h)) * 2048.0));
-                hi = __builtin_bit_cast(uint16_t, h);
-                lo = __builtin_bit_cast(uint16_t, l);
-            }
-            s_whi[k] = hi;
-            s_wlo[k] = lo;
-        }
+    // metadata of the resident blocks, one word per lane, read back with readlane;
+    // fmask bit j: block j ends a row group (flush), tmask bit j: it ends a mode
+    const uint32_t mreg = lane < nres ? s_meta[b0 + lane] : 0u;
+    uint64_t fmask = 0, tmask = 0;
+    for (int j = 0; j < nres; ++j) {
+        const int kj = __builtin_amdgcn_readlane(static_cast<int>(mreg), j) >> 16;
+        const int kn = j + 1 < nres ? (__builtin_amdgcn_readlane(static_cast<int>(mreg), j + 1) >> 16) : -1;
+        if (kn != kj) fmask |= 1ull << j;
+        if (kn < 0 || (kn >> 8) != (kj >> 8)) tmask |= 1ull << j;
+    }
+    // B operand: lanes of column 0 read w_hi, column 1 w_lo, the others a
+    // zeroed region, so every lane issues the same unpredicated load
+    const int bsel = (col == 0 ? 0 : (col == 1 ? Kp : 2 * Kp)) + (kg & 1) * 8;
+    auto b_operand = [&](uint32_t m) {
+        const int q = (kg < 2) ? static_cast<int>((m >> 8) & 255u) : static_cast<int>(m & 255u);
+        return *reinterpret_cast<const uint4*>(s_wb + bsel + q * kTile);
     };
-    split_w();
+    // FP64 contraction of a 16 x 16 accumulator with w over its row tile p
+    auto row_contract = [&](const f32x4& acc, int p) {
+        double c = 0.0;
+        if (col < 2) {
+            const double* wr = s_w + p * kTile + 4 * kg;
+            c = static_cast<double>(acc[0]) * wr[0] + static_cast<double>(acc[1]) * wr[1] +
+                static_cast<double>(acc[2]) * wr[2] + static_cast<double>(acc[3]) * wr[3];
+            if (col == 1) c *= static_cast<double>(a.col1);
+        }
+        return c;
+    };
+
+    // hi/lo split of one coefficient into the B-operand table
+    auto split_one = [&](int k, double w) {
+        uint16_t hi, lo;
+        if constexpr (ELEM == 0) {
+            hi = f32_to_bf16_rne(static_cast<float>(w));
+            lo = f32_to_bf16_rne(static_cast<float>(w - static_cast<double>(bf16_to_f32(hi))));
+        } else {
+            const double ws = w * static_cast<double>(a.wscale);
+            const _Float16 h = static_cast<_Float16>(static_cast<float>(ws));
+            const _Float16 l = static_cast<_Float16>(static_cast<float>((ws - static_cast<double>(h)) * 2048.0));
+            hi = __builtin_bit_cast(uint16_t, h);
+            lo = __builtin_bit_cast(uint16_t, l);
+        }
+        s_wb[k] = hi;
+        s_wb[Kp + k] = lo;
+    };
+    for (int k = tid; k < Kp; k += 256) split_one(k, s_w[k]);
+    if (tid < 4 * kMaxIpw) s_ypart[tid / kMaxIpw][tid % kMaxIpw] = 0.0;
     __syncthreads();
 
     for (int s = 0; s < a.nsteps; ++s) {
         const uint32_t tag = a.tag0 + static_cast<uint32_t>(s) + 1u;
-        if (tid < 4 * kMaxIpw) s_ypart[tid / kMaxIpw][tid % kMaxIpw] = 0.0;
-        __syncthreads();
         // ---- contraction of this wave's blocks -------------------------------
         {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            int cur = -1;              // (t << 8) | p of the open accumulator
+            uint4 breg[RB];
+#pragma unroll
+            for (int j = 0; j < RB; ++j)
+                breg[j] = b_operand(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mreg), j)));
+            const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
+            f32x4 acc0 = zero4, acc1 = zero4;
             double part = 0.0;
-            auto close_row = [&]() {
-                if (cur < 0) return;
-                const int p = cur & 255;
-                double c = 0.0;
-                if (col < 2) {
-                    const double* wr = s_w + p * kTile + 4 * kg;
-                    c = static_cast<double>(acc[0]) * wr[0] + static_cast<double>(acc[1]) * wr[1] +
-                        static_cast<double>(acc[2]) * wr[2] + static_cast<double>(acc[3]) * wr[3];
-                    if (col == 1) c *= static_cast<double>(a.col1);
+#pragma unroll
+            for (int j = 0; j < RB; ++j) {
+                if (j & 1) acc1 = mfma16<ELEM>(areg[j], breg[j], acc1);
+                else acc0 = mfma16<ELEM>(areg[j], breg[j], acc0);
+                if ((fmask >> j) & 1ull) {
+                    const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mreg), j));
+                    part += row_contract(acc0 + acc1, static_cast<int>((m >> 16) & 255u));
+                    acc0 = zero4;
+                    acc1 = zero4;
+                    if ((tmask >> j) & 1ull) {
+                        const double v = wave_total(part);
+                        if (lane == 0) s_ypart[wv][m >> 24] += v;
+                        part = 0.0;
+                    }
                 }
-                part += c;
-            };
-            auto close_mode = [&]() {
-                if (cur < 0) return;
-                const double v = wave_sum_d(part);
-                if (lane == 0) s_ypart[wv][cur >> 8] += v;
-                part = 0.0;
-            };
-            auto process = [&](int j, const uint4& av) {
+            }
+            // blocks beyond the register budget stream from HBM every step
+            int cur = -1;
+            for (int j = RB; j < nblk; ++j) {
                 const uint32_t m = s_meta[b0 + j];
                 const int key = static_cast<int>(m >> 16);
                 if (key != cur) {
-                    close_row();
-                    if (cur >= 0 && (key >> 8) != (cur >> 8)) close_mode();
-                    acc = f32x4{0.f, 0.f, 0.f, 0.f};
+                    if (cur >= 0) {
+                        part += row_contract(acc0, cur & 255);
+                        if ((key >> 8) != (cur >> 8)) {
+                            const double v = wave_total(part);
+                            if (lane == 0) s_ypart[wv][cur >> 8] += v;
+                            part = 0.0;
+                        }
+                    }
+                    acc0 = zero4;
                     cur = key;
                 }
-                const int q = (kg < 2) ? static_cast<int>((m >> 8) & 255u) : static_cast<int>(m & 255u);
-                const int off = q * kTile + (kg & 1) * 8;
-                uint4 bv = {0u, 0u, 0u, 0u};
-                if (col == 0) bv = *reinterpret_cast<const uint4*>(s_whi + off);
-                else if (col == 1) bv = *reinterpret_cast<const uint4*>(s_wlo + off);
-                acc = mfma16<ELEM>(av, bv, acc);
-            };
-#pragma unroll
-            for (int j = 0; j < kRB; ++j)
-                if (j < nres) process(j, areg[j]);
-            for (int j = kRB; j < nblk; ++j) process(j, gblk[static_cast<size_t>(j) * 64]);
-            close_row();
-            close_mode();
+                acc0 = mfma16<ELEM>(gblk[static_cast<size_t>(j) * 64], b_operand(m), acc0);
+            }
+            if (cur >= 0) {
+                part += row_contract(acc0, cur & 255);
+                const double v = wave_total(part);
+                if (lane == 0) s_ypart[wv][cur >> 8] += v;
+            }
         }
         __syncthreads();
         // ---- publish y_i of this workgroup ---------------------------------------
@@ -292,16 +340,23 @@ __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
         // ---- FP64 update, identical in every workgroup (psfsolver.cpp:34-59) -----------
         {
 #pragma clang fp contract(off)
-            double s1 = 0.0;
-            for (int k = tid; k < K; k += 256) s1 += s_w[k] * s_w[k];
-            const double e1 = block_sum_d(s1, s_red);
-            double s2 = 0.0;
+            double s1 = 0.0, s2 = 0.0;
             for (int k = tid; k < K; k += 256) {
-                const double wn = s_w[k] + a.dt * s_y[k];
+                const double w = s_w[k];
+                const double wn = w + a.dt * s_y[k];
                 s_y[k] = wn;
+                s1 += w * w;
                 s2 += wn * wn;
             }
-            const double e2 = block_sum_d(s2, s_red);
+            s1 = wave_total(s1);
+            s2 = wave_total(s2);
+            if (lane == 0) {
+                s_red[wv] = s1;
+                s_red[4 + wv] = s2;
+            }
+            __syncthreads();
+            const double e1 = ((s_red[0] + s_red[1]) + s_red[2]) + s_red[3];
+            const double e2 = ((s_red[4] + s_red[5]) + s_red[6]) + s_red[7];
             if (!(e2 > 0.0) || !__builtin_isfinite(e2) || !__builtin_isfinite(e1)) {
                 if (tid == 0) s_flag = 1;
                 __syncthreads();
@@ -310,13 +365,13 @@ __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
             const double fac = sqrt(e1 / e2);
             for (int k = tid; k < K; k += 256) {
                 double w = s_y[k] * fac;
-                w = w * a.visc[k];
-                if (a.gravity_active) w = w + a.dt * a.gravity[k];
+                w = w * s_visc[k];
+                if (a.gravity_active) w = w + a.dt * s_grav[k];
                 s_w[k] = w;
+                split_one(k, w);
             }
+            if (tid < 4 * kMaxIpw) s_ypart[tid / kMaxIpw][tid % kMaxIpw] = 0.0;
         }
-        __syncthreads();
-        split_w();
         __syncthreads();
     }
     __syncthreads();
@@ -499,8 +554,17 @@ hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream)
     a.col1 = p->col1;
     a.wscale = p->wscale;
     void* args[] = {&a};
-    const void* fn = p->elem == 0 ? reinterpret_cast<const void*>(&k_persistent<0>)
-                                  : reinterpret_cast<const void*>(&k_persistent<1>);
+    // smallest register budget that holds a wave's blocks (36 when they do not fit)
+    const int per = (p->bpw + 3) / 4;
+    const void* fn;
+    if (p->elem == 0)
+        fn = per <= 4 ? reinterpret_cast<const void*>(&k_persistent<0, 4>)
+           : per <= 12 ? reinterpret_cast<const void*>(&k_persistent<0, 12>)
+                       : reinterpret_cast<const void*>(&k_persistent<0, kRB>);
+    else
+        fn = per <= 4 ? reinterpret_cast<const void*>(&k_persistent<1, 4>)
+           : per <= 12 ? reinterpret_cast<const void*>(&k_persistent<1, 12>)
+                       : reinterpret_cast<const void*>(&k_persistent<1, kRB>);
     hipError_t e = hipLaunchCooperativeKernel(fn, dim3(p->G), dim3(256), args, 0, stream);
     if (e == hipSuccess) p->tag += static_cast<uint32_t>(nsteps);
     return e;
