@@ -18,8 +18,10 @@
 //
 // Exchange.  Workgroup g owns modes i = g, g+G, ...  After the contraction it
 // publishes y_i as two 64-bit words (step tag << 32 | high or low half of the
-// FP64 bits) with relaxed agent-scope atomic stores, and every workgroup
-// polls all K pairs until both tags match.  Each word carries its own
+// FP64 bits) with relaxed agent-scope atomic stores, and one wave of every
+// workgroup polls all K pairs until both tags match.  Each mode's record sits
+// on its own 128-byte line: records that share a line written from several
+// CUs cost more than reading one line per mode.  Each word carries its own
 // validity, so no release/acquire fences (L2 write-back/invalidate) are
 // needed, and the y values arrive exactly.  Buffers alternate by tag parity:
 // a workgroup can only overwrite a buffer two steps later, after every
@@ -50,6 +52,7 @@ constexpr int kMaxMeta = 4096;   // blocks per workgroup
 constexpr int kMaxIpw = 8;       // modes per workgroup
 constexpr int kMaxCUs = 256;
 constexpr uint64_t kWaitTicks = 200000000ull;   // s_memrealtime runs at 100 MHz: 2 s
+constexpr int kXStride = 16;     // 64-bit words per exchange record: one 128-byte line per mode
 
 struct PersistentState {
     int G = 0, ipw = 0, Kp = 0, nb = 0, nBi = 0, bpw = 0;
@@ -59,7 +62,7 @@ struct PersistentState {
     uint4* blocks = nullptr;     // [G][bpw][64]
     uint32_t* meta = nullptr;    // [bpw]: t << 24 | p << 16 | q1 << 8 | q2
     double* inv_scale = nullptr; // [G * ipw]
-    unsigned long long* xbuf = nullptr;   // [2][K][2]
+    unsigned long long* xbuf = nullptr;   // [2][K][kXStride], words 0 and 1 used
     uint32_t tag = 0;
 };
 
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
         }
         __syncthreads();
         // ---- publish y_i of this workgroup ---------------------------------------
-        unsigned long long* xb = a.xbuf + static_cast<size_t>(tag & 1u) * K * 2;
+        unsigned long long* xb = a.xbuf + static_cast<size_t>(tag & 1u) * K * kXStride;
         if (tid < a.ipw) {
             const int i = g + a.G * tid;
             if (i < K) {
@@ -312,25 +315,26 @@ __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
                                  a.inv_scale[g * a.ipw + tid];
                 const unsigned long long bits = __double_as_longlong(y);
                 const unsigned long long tg = static_cast<unsigned long long>(tag) << 32;
-                __hip_atomic_store(xb + 2 * i, tg | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(xb + 2 * i + 1, tg | (bits & 0xFFFFFFFFull), __ATOMIC_RELAXED,
+                __hip_atomic_store(xb + kXStride * i, tg | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(xb + kXStride * i + 1, tg | (bits & 0xFFFFFFFFull), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        // ---- gather all y (bounded wait) ---------------------------------------------
-        {
+        // ---- gather all y (one wave polls; bounded wait) ----------------------------
+        if (wv == 0) {
             const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-            for (int i = tid; i < K; i += 256) {
+            for (int i = lane; i < K; i += 64) {
+                const unsigned long long* rec = xb + static_cast<size_t>(kXStride) * i;
                 unsigned long long v0, v1;
                 for (;;) {
-                    v0 = __hip_atomic_load(xb + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    v1 = __hip_atomic_load(xb + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    v0 = __hip_atomic_load(rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    v1 = __hip_atomic_load(rec + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if ((v0 >> 32) == tag && (v1 >> 32) == tag) break;
                     if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
                         s_flag = 2;
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(1);
+                    __builtin_amdgcn_s_sleep(2);
                 }
                 s_y[i] = __longlong_as_double(static_cast<long long>(((v0 & 0xFFFFFFFFull) << 32) | (v1 & 0xFFFFFFFFull)));
             }
@@ -398,8 +402,8 @@ hipError_t persistent_scene_init(psf_scene* s)
     p->blocks = static_cast<uint4*>(s->dev.C);
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->meta), sizeof(uint32_t) * p->bpw);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->inv_scale), sizeof(double) * p->G * p->ipw);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->xbuf), sizeof(unsigned long long) * 4 * K);
-    if (e == hipSuccess) e = hipMemset(p->xbuf, 0, sizeof(unsigned long long) * 4 * K);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->xbuf), sizeof(unsigned long long) * 2 * kXStride * K);
+    if (e == hipSuccess) e = hipMemset(p->xbuf, 0, sizeof(unsigned long long) * 2 * kXStride * K);
     if (e != hipSuccess) return e;
     // block metadata (identical for every workgroup)
     std::vector<uint32_t> meta;
