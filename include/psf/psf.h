@@ -114,6 +114,10 @@ PSF_API int psf_scene_upload_basis(psf_scene* scene, const double* Phi, const do
 /* C: K matrices K x K, each column-major, matrix i at C + i*K*K
    (std::vector<Eigen::MatrixXd> advection, abstractsolver.h:118). */
 PSF_API int psf_scene_upload_advection(psf_scene* scene, const double* C);
+/* The same tensor as K separate matrices: mats[i] points at the K x K
+   column-major C_i, i.e. advection[i].data() of the reference's
+   std::vector<Eigen::MatrixXd> (abstractsolver.h:118). */
+PSF_API int psf_scene_upload_advection_matrices(psf_scene* scene, const double* const* mats);
 /* Optional: the caller's DECMesh3D::getSignBitString() (decmesh3d.cpp:82-106,
    1390-1399).  The tracer evaluates the sign bits in closed form; this call
    checks the caller's table against that closed form and fails on mismatch. */

@@ -63,6 +63,7 @@ def load():
         "psf_scene_footprint": (i32, [ct.POINTER(SceneDesc), ct.POINTER(ct.c_uint64)]),
         "psf_scene_upload_basis": (i32, [vp, vp, vp, vp]),
         "psf_scene_upload_advection": (i32, [vp, vp]),
+        "psf_scene_upload_advection_matrices": (i32, [vp, vp]),
         "psf_scene_check_signs": (i32, [vp, vp, ct.c_size_t]),
         "psf_set_params": (i32, [vp, dbl, dbl, i32, flt]),
         "psf_set_step_path": (i32, [vp, i32]),

@@ -378,6 +378,23 @@ int psf_scene_upload_advection(psf_scene* s, const double* C)
     return PSF_OK;
 }
 
+int psf_scene_upload_advection_matrices(psf_scene* s, const double* const* mats)
+{
+    if (!s || !mats) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    const size_t K = s->d.K;
+    std::vector<double> C;
+    try {
+        C.resize(K * K * K);
+    } catch (const std::bad_alloc&) {
+        return set_err(s->ctx, PSF_ERR_NOMEM, "host allocation of %zu doubles failed", K * K * K);
+    }
+    for (size_t i = 0; i < K; ++i) {
+        if (!mats[i]) return set_err(s->ctx, PSF_ERR_ARG, "matrix %zu is null", i);
+        std::memcpy(C.data() + i * K * K, mats[i], K * K * sizeof(double));
+    }
+    return psf_scene_upload_advection(s, C.data());
+}
+
 int psf_scene_check_signs(psf_scene* s, const uint32_t* words, size_t nwords)
 {
     if (!s || !words) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");

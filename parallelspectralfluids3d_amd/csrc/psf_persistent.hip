@@ -1,6 +1,6 @@
 // Persistent reduced-step path on MFMA (BF16, FP16, BF16 hi+lo).
 //
-// One cooperative launch runs n steps of psfsolver.cpp:34-59.
+// One launch runs n steps of psfsolver.cpp:34-59.
 //
 // Storage.  Only the symmetric part of C_i enters y_i = w^T C_i w, so C_i is
 // replaced by S_i = (C_i + C_i^T)/2 and stored as its 16x16 tiles T_pq with
@@ -30,6 +30,13 @@
 // in a fixed order, so all copies stay bitwise identical.  Waits are bounded
 // (2 s of device time); on expiry every workgroup leaves the loop and the
 // status word reports it.
+//
+// Residency.  The exchange needs all G workgroups resident at once.  The
+// launch is an ordinary one on the scene's stream: kernels on one stream run
+// one after another, so the GPU holds no other work of the scene when the
+// step starts, and scene set-up checks that G workgroups fit on the device
+// at the kernel's occupancy.  Should a workgroup still not be scheduled, the
+// bounded waits turn that into the timed-out status, not a hang.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -64,6 +71,7 @@ struct PersistentState {
     double* inv_scale = nullptr; // [G * ipw]
     unsigned long long* xbuf = nullptr;   // [2][K][kXStride], words 0 and 1 used
     uint32_t tag = 0;
+    const void* fn = nullptr;    // the k_persistent instance for this layout
 };
 
 struct PArgs {
@@ -386,6 +394,19 @@ __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// smallest register budget that holds a wave's blocks (kRB when they do not fit)
+static const void* pick_kernel(int elem, int bpw)
+{
+    const int per = (bpw + 3) / 4;
+    if (elem == 0)
+        return per <= 4 ? reinterpret_cast<const void*>(&k_persistent<0, 4>)
+             : per <= 12 ? reinterpret_cast<const void*>(&k_persistent<0, 12>)
+                         : reinterpret_cast<const void*>(&k_persistent<0, kRB>);
+    return per <= 4 ? reinterpret_cast<const void*>(&k_persistent<1, 4>)
+         : per <= 12 ? reinterpret_cast<const void*>(&k_persistent<1, 12>)
+                     : reinterpret_cast<const void*>(&k_persistent<1, kRB>);
+}
+
 hipError_t persistent_scene_init(psf_scene* s)
 {
     const int prec = s->d.c_precision;
@@ -400,7 +421,14 @@ hipError_t persistent_scene_init(psf_scene* s)
     p->col1 = prec == PSF_C_FP16 ? 1.0f / 2048.0f : 1.0f;
     p->wscale = prec == PSF_C_FP16 ? 256.0f : 1.0f;
     p->blocks = static_cast<uint4*>(s->dev.C);
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->meta), sizeof(uint32_t) * p->bpw);
+    p->fn = pick_kernel(p->elem, p->bpw);
+    // every workgroup must be resident at once
+    int per_cu = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->fn, 256, 0);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->ctx->device);
+    if (e != hipSuccess) return e;
+    if (per_cu * cus < p->G) return hipErrorCooperativeLaunchTooLarge;
+    e = hipMalloc(reinterpret_cast<void**>(&p->meta), sizeof(uint32_t) * p->bpw);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->inv_scale), sizeof(double) * p->G * p->ipw);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->xbuf), sizeof(unsigned long long) * 2 * kXStride * K);
     if (e == hipSuccess) e = hipMemset(p->xbuf, 0, sizeof(unsigned long long) * 2 * kXStride * K);
@@ -558,18 +586,7 @@ hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream)
     a.col1 = p->col1;
     a.wscale = p->wscale;
     void* args[] = {&a};
-    // smallest register budget that holds a wave's blocks (36 when they do not fit)
-    const int per = (p->bpw + 3) / 4;
-    const void* fn;
-    if (p->elem == 0)
-        fn = per <= 4 ? reinterpret_cast<const void*>(&k_persistent<0, 4>)
-           : per <= 12 ? reinterpret_cast<const void*>(&k_persistent<0, 12>)
-                       : reinterpret_cast<const void*>(&k_persistent<0, kRB>);
-    else
-        fn = per <= 4 ? reinterpret_cast<const void*>(&k_persistent<1, 4>)
-           : per <= 12 ? reinterpret_cast<const void*>(&k_persistent<1, 12>)
-                       : reinterpret_cast<const void*>(&k_persistent<1, kRB>);
-    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(p->G), dim3(256), args, 0, stream);
+    hipError_t e = hipLaunchKernel(p->fn, dim3(p->G), dim3(256), args, 0, stream);
     if (e == hipSuccess) p->tag += static_cast<uint32_t>(nsteps);
     return e;
 }

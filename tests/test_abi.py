@@ -39,3 +39,21 @@ def test_context_errors_are_codes():
     if rc == L.PSF_OK:
         lib.psf_ctx_destroy(h)
     assert lib.psf_ctx_destroy(None) == L.PSF_ERR_ARG
+
+
+def test_cpp_header_compiles(tmp_path):
+    """include/psf/solver.hpp and psf.h compile as plain C++ and C (no HIP headers)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    inc = os.path.join(root, "include")
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "psf/solver.hpp"\nint main(){ return 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", f"-I{inc}", str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    csrc = tmp_path / "t.c"
+    csrc.write_text('#include "psf/psf.h"\nint main(void){ return 0; }\n')
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", f"-I{inc}", str(csrc)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

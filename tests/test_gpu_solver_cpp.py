@@ -1,0 +1,48 @@
+"""The C++ drop-in (include/psf/solver.hpp: HipSolver, a SolverImpl in the
+place of PSFSolverGPU) driven frame by frame like GLCanvas::simulate drives
+integrate() (Widgets/glcanvas.cpp:143-156), checked against the oracle."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from scenes import make_scene, particles_uniform
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "parallelspectralfluids3d_amd", "solver_check")
+
+
+@pytest.mark.parametrize("cprec,phiprec,tol_w", [(0, 0, 1e-12), (4, 1, 2e-6)])
+def test_hip_solver_frames(tmp_path, cprec, phiprec, tol_w):
+    print(f"[gpu-case] solver_check cprec={cprec} phiprec={phiprec}", flush=True)
+    sd = make_scene(n=8, K=16)
+    N, frames, dt = 1024, 20, 0.01
+    p0 = particles_uniform(sd, N, seed=11)
+    d = str(tmp_path)
+    with open(os.path.join(d, "meta.txt"), "w") as fh:
+        fh.write(f"8 8 8 {sd.resolution!r} {sd.K} {N} {frames} {dt!r} 0.0 0 60.0 {cprec} {phiprec}\n")
+    np.ascontiguousarray(sd.Phi, np.float64).tofile(os.path.join(d, "phi.bin"))
+    sd.lam.astype(np.float64).tofile(os.path.join(d, "lambda.bin"))
+    sd.g.astype(np.float64).tofile(os.path.join(d, "gravity.bin"))
+    np.ascontiguousarray(sd.C, np.float64).tofile(os.path.join(d, "C.bin"))
+    sd.omega.astype(np.float64).tofile(os.path.join(d, "omega.bin"))
+    p0.astype(np.float32).tofile(os.path.join(d, "parts.bin"))
+    r = subprocess.run([EXE, d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    w_gpu = np.fromfile(os.path.join(d, "omega_out.bin"), np.float64)
+    u_gpu = np.fromfile(os.path.join(d, "u_out.bin"), np.float32)
+    p_gpu = np.fromfile(os.path.join(d, "parts_out.bin"), np.float32).reshape(N, 4)
+    m = O.Mesh((8, 8, 8), sd.resolution, sd.aabb_min, inside=sd.inside)
+    w, p = sd.omega.copy(), p0.copy()
+    for f in range(frames):
+        w = O.reduced_step(w, sd.C, sd.lam, sd.g, dt, 0.0, False)
+        u = O.reconstruct(sd.Phi, w)
+        p = O.particles(m, u, p, resolution=sd.resolution, timestep=dt, aabb_min=sd.aabb_min,
+                        aabb_max=sd.aabb_max, lifetime=60.0, seed=0, step=f)
+    assert np.linalg.norm(w_gpu - w) / np.linalg.norm(w) <= tol_w
+    assert np.abs(u_gpu - u).max() <= 1e-5 * np.abs(u).max()
+    assert np.abs(p_gpu[:, :3] - p[:, :3]).max() <= 1e-5
