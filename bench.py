@@ -30,6 +30,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "reduced steps/s + particle-steps/s at 64³,K=256; p99 latency; err vs FP64"
 HBM_PEAK = 8.0e12
+# HBM bytes per k_reconstruct<float> launch at 64^3/K=256, measured with
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --counter-pass`
+# (399,390 KiB fetched x 2, the factor a 1 GiB hipMemcpy reads back, plus
+# 3,120.1 KiB written); counters cannot be read from inside this process.
+RECON_TRAFFIC_64_256 = 821_145_723
 
 
 def log(msg):
@@ -117,6 +122,32 @@ def cpu_baseline(sc, budget_s=12.0):
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model}
 
 
+def counter_pass_copies(frames, ms):
+    """Three device-to-device copies of 1 GiB with the HIP runtime's own copy
+    kernel: the reference for the FETCH_SIZE / WRITE_SIZE counters (each copy
+    reads 2^30 bytes and writes 2^30 bytes)."""
+    import ctypes as ct
+    hip = ct.CDLL("libamdhip64.so")
+    n = ct.c_size_t(1 << 30)
+    src, dst = ct.c_void_p(), ct.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(ct.byref(ptr), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, n) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(3):
+            if hip.hipMemcpy(dst, src, n, 3) != 0:        # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    print(json.dumps({"counter_pass": True, "frames": frames, "frame_ms": ms / frames,
+                      "copy_bytes_each_way": 1 << 30, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +159,10 @@ def main():
     ap.add_argument("--reduced-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--counter-pass", action="store_true",
+                    help="short run for hardware-counter collection: the frame loop, one persistent launch of "
+                         "--reduced-steps steps, then three plain 1 GiB device copies that calibrate the byte "
+                         "counters; prints one line and exits")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU); "
@@ -152,6 +187,13 @@ def main():
     wall = time.perf_counter() - t0
     ms_frame = ms / args.steps
     log(f"{args.steps} frames: {ms:.2f} ms (events), {1e3 * wall:.2f} ms (wall)")
+    if args.counter_pass:
+        s.step(args.reduced_steps)     # one persistent launch of that many steps
+        ctx.sync()
+        s.close()
+        ctx.close()
+        counter_pass_copies(args.steps, ms)
+        return
     # per-phase averages: reconstruction is the dominant kernel
     reps = 100
     ms_rec = s.time_phase(L.PHASE_RECON, reps) / reps
@@ -230,7 +272,8 @@ def main():
                              "samples": int(lat.size)},
         "phase_ms": {"reduced_step": ms_st, "reconstruct": ms_rec, "trace": ms_tr},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": RECON_TRAFFIC_64_256 if (n, K) == (64, 256) else None,
                      "kernel": "k_reconstruct<float>", "algorithmic_bytes_per_launch": rec_bytes},
         "cpu_baseline": cpu,
     }
