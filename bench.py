@@ -7,13 +7,18 @@ Laplacian), omega ~ N(0,1) from seed 0 at unit energy, dt = 0.01, no
 viscosity or forcing, 2^20 particles uniform in the box.  A "step" is one
 AbstractSolver::integrate() frame (psfsolver.cpp:21-234): reduced step
 (BF16 hi+lo MFMA contraction, FP64 update), u = Phi w (FP32 Phi, FP64 sums)
-and one particle pass.
+and one particle pass.  With a frame window W (default 32) the reduced
+steps of W frames run in one persistent launch and their W velocity fields
+in one pass over Phi; every frame still gets its own omega, u and particle
+pass, bitwise equal to single frames.
 
 Printed on stdout: one JSON line.  value = particle-steps/s of the frame
-loop; beside it the reduced steps/s of one persistent launch of 1000 steps
-and of the plain FP32 path, omega error against the FP64 path every 100
-steps, frame latency p50/p99, the roofline of the dominant kernel
-(reconstruction) and the CPU oracle timed on this host.
+loop; beside it the same frames one at a time, the reduced steps/s of one
+persistent launch of 1000 steps and of the plain FP32 path, omega error
+against the FP64 path every 100 steps, frame latency p50/p99, per-kernel
+time per frame with the roofline of each kernel (the headline one is the
+kernel with the largest share of a frame) and the CPU oracle timed on this
+host.
 """
 from __future__ import annotations
 
@@ -30,6 +35,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "reduced steps/s + particle-steps/s at 64³,K=256; p99 latency; err vs FP64"
 HBM_PEAK = 8.0e12
+FP64_PEAK = 78.6e12
+# HBM bytes per k_trace launch at 64^3 with 2^20 particles, measured the same
+# way (14,100 KiB fetched x 2 plus 16,384 KiB written)
+TRACE_TRAFFIC_64_1M = 45_654_016
 # HBM bytes per k_reconstruct<float> launch at 64^3/K=256, measured with
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --counter-pass`
 # (399,390 KiB fetched x 2, the factor a 1 GiB hipMemcpy reads back, plus
@@ -157,6 +166,9 @@ def main():
     ap.add_argument("--modes", type=int, default=256)
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--reduced-steps", type=int, default=1000)
+    ap.add_argument("--window", type=int, default=32,
+                    help="frame window: reduced steps and reconstructions of this many frames are computed "
+                         "together (1 = one frame at a time)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--counter-pass", action="store_true",
@@ -179,7 +191,9 @@ def main():
 
     # ---- frame loop (the value) ---------------------------------------------------
     s = make_hip_scene(ctx, sc, L.C_BF16X2, L.PHI_FP32, True, N)
-    log("frame scene uploaded")
+    W = args.window
+    s.set_frame_window(W)
+    log(f"frame scene uploaded (frame window {W})")
     s.time_phase(L.PHASE_FRAME, args.warmup)
     ctx.sync()
     t0 = time.perf_counter()
@@ -194,17 +208,59 @@ def main():
         ctx.close()
         counter_pass_copies(args.steps, ms)
         return
-    # per-phase averages: reconstruction is the dominant kernel
+    # frame latency distribution: one integrate() per call, 8 windows' worth
+    lat = np.array([s.time_phase(L.PHASE_FRAME, 1) for _ in range(max(200, 8 * W))])
+    # per-kernel averages, each per frame
     reps = 100
-    ms_rec = s.time_phase(L.PHASE_RECON, reps) / reps
     ms_tr = s.time_phase(L.PHASE_TRACE, reps) / reps
+    kern = {"k_trace": ms_tr}
+    if W > 1:
+        ms_wrec = s.time_phase(L.PHASE_WINDOW_RECON, 10) / 10
+        ms_wstep = s.time_phase(L.PHASE_NSTEP, W)
+        kern["k_reconstruct_window"] = ms_wrec / W
+        kern["k_persistent"] = ms_wstep / W
+    ms_rec = s.time_phase(L.PHASE_RECON, reps) / reps
     ms_st = s.time_phase(L.PHASE_STEP, reps) / reps
-    # frame latency distribution
-    lat = np.array([s.time_phase(L.PHASE_FRAME, 1) for _ in range(200)])
+    if W <= 1:
+        kern["k_reconstruct"] = ms_rec
+        kern["k_persistent"] = ms_st
+    # the same frames one at a time
+    s.set_frame_window(1)
+    s.time_phase(L.PHASE_FRAME, args.warmup)
+    ms_frame1 = s.time_phase(L.PHASE_FRAME, args.steps) / args.steps
     s.close()
+    log(f"per frame: " + ", ".join(f"{k} {1e3 * v:.1f} us" for k, v in kern.items()) +
+        f"; one frame at a time {1e3 * ms_frame1:.1f} us")
 
+    # roofline of each kernel, per launch; the dominant one (largest share of a
+    # frame) is the headline
+    roof = {}
+    trace_bytes = N * 32 + F * 4
+    roof["k_trace"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9, achieved=trace_bytes / (ms_tr * 1e-3) / 1e9,
+                           algorithmic_bytes_per_launch=trace_bytes, traffic=TRACE_TRAFFIC_64_1M if
+                           (n, N) == (64, 1 << 20) else None)
     rec_bytes = F * K * 4 + F * 4 + K * 8
-    achieved = rec_bytes / (ms_rec * 1e-3)
+    roof["k_reconstruct"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
+                                 achieved=rec_bytes / (ms_rec * 1e-3) / 1e9, algorithmic_bytes_per_launch=rec_bytes,
+                                 traffic=RECON_TRAFFIC_64_256 if (n, K) == (64, 256) else None)
+    if W > 1:
+        wflops = 2.0 * F * K * W
+        roof["k_reconstruct_window"] = dict(bound="fp64", unit="TFLOP/s", peak=FP64_PEAK / 1e12,
+                                            achieved=wflops / (ms_wrec * 1e-3) / 1e12,
+                                            algorithmic_flops_per_launch=wflops,
+                                            algorithmic_bytes_per_launch=F * K * 4 + W * (F * 4 + K * 8),
+                                            traffic=None)
+        cbytes = K * K * (K + 1) // 2 * 4 + 32 * K
+        roof["k_persistent"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
+                                    achieved=cbytes / (kern["k_persistent"] * 1e-3) / 1e9,
+                                    algorithmic_bytes_per_step=cbytes, traffic=None,
+                                    note="C stays in registers across the steps of a launch: effective bandwidth")
+    for r in roof.values():
+        r["frac"] = r["achieved"] / r["peak"]
+    dominant = max(kern, key=kern.get)
+    head = dict(roof[dominant])
+    head["kernel"] = dominant
+    head["ms_per_frame"] = kern[dominant]
 
     # ---- reduced steps/s: one persistent launch vs the plain FP32 path -------------------
     nred = args.reduced_steps
@@ -260,21 +316,23 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16",
         "dtype_detail": "C in BF16 hi+lo on MFMA with FP32 accumulation; FP64 omega update; FP32 Phi with FP64 "
-                        "sums; FP32 particles",
+                        "sums (FP64 FMA, one pass over Phi per frame window); FP32 particles",
         "data": "synthetic",
         "config": {"workload": f"{n}^3 closed box, K={K} analytic DEC eigenbasis, {N} particles, dt=0.01, "
-                               f"one integrate() frame per step", "grid": n, "K": K, "particles": N},
+                               f"one integrate() frame per step", "grid": n, "K": K, "particles": N,
+                   "frame_window": W},
+        "ms_per_frame_one_at_a_time": ms_frame1,
+        "particle_steps_per_s_one_at_a_time": N / (ms_frame1 * 1e-3),
         "reduced_steps_per_s": res.get("bf16x2"),
         "reduced_steps_per_s_by_path": res,
         "reduced_speedup_vs_plain_fp32": res["bf16x2"] / res["plain_fp32"],
         "omega_rel_err_vs_fp64_every_100_steps": errors,
         "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                              "samples": int(lat.size)},
-        "phase_ms": {"reduced_step": ms_st, "reconstruct": ms_rec, "trace": ms_tr},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK,
-                     "traffic": RECON_TRAFFIC_64_256 if (n, K) == (64, 256) else None,
-                     "kernel": "k_reconstruct<float>", "algorithmic_bytes_per_launch": rec_bytes},
+        "kernel_ms_per_frame": kern,
+        "phase_ms_one_at_a_time": {"reduced_step": ms_st, "reconstruct": ms_rec, "trace": ms_tr},
+        "roofline": head,
+        "roofline_by_kernel": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define PSF_ABI_VERSION 1
+#define PSF_ABI_VERSION 2
 
 #if defined(__GNUC__)
 #define PSF_API __attribute__((visibility("default")))
@@ -154,12 +154,25 @@ PSF_API int psf_get_velocity(psf_scene* scene, float* u);
 PSF_API int psf_trace(psf_scene* scene, int32_t* cells);
 /* nframes x AbstractSolver::integrate(): step, reconstruct, trace */
 PSF_API int psf_integrate(psf_scene* scene, int nframes);
+/* Frame window (no counterpart in the reference; a scheduling choice inside
+   integrate()).  With frames = W > 1, when integrate() finds no frame
+   computed ahead it runs the next W reduced steps in one launch and the W
+   reconstructions u_t = float(Phi w_t) in one pass over Phi, then each
+   integrate() traces with its own u_t.  The reduced dynamics do not depend
+   on the particles, so omega, u and the particles are bitwise those of W
+   single frames.  Any call that changes the scene (params, omega, uploads,
+   path, psf_step, psf_reconstruct) first drops the frames computed ahead.
+   W is 1 (default: one frame at a time), 2, 4, 8, 16 or 32; the window
+   holds W * (4 * F' + 8 * K) + 8 * K more bytes, F' = F rounded up to 256. */
+PSF_API int psf_set_frame_window(psf_scene* scene, uint32_t frames);
 
 /* ---- measurement -----------------------------------------------------------
  * Runs `count` launches of one phase back to back on the context stream
  * between two HIP events and returns the elapsed milliseconds.
  * phase: 0 reduced step (one step per launch), 1 reconstruction, 2 trace,
- *        3 full integrate frame, 4 one launch of `count` reduced steps. */
+ *        3 full integrate frame, 4 one launch of `count` reduced steps,
+ *        5 the frame window's reconstruction (all W frames; window > 1,
+ *          recomputes the u of the frames already computed ahead). */
 PSF_API int psf_time_phase(psf_scene* scene, int phase, int count, double* ms);
 
 #ifdef __cplusplus

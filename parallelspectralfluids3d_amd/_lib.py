@@ -17,7 +17,7 @@ PSF_OK, PSF_ERR_ARG, PSF_ERR_HIP, PSF_ERR_NOMEM, PSF_ERR_STATE, PSF_ERR_UNSUPPOR
 C_FP64, C_FP32, C_BF16, C_FP16, C_BF16X2 = 0, 1, 2, 3, 4
 PHI_FP64, PHI_FP32 = 0, 1
 PATH_PLAIN, PATH_PERSISTENT = 0, 1
-PHASE_STEP, PHASE_RECON, PHASE_TRACE, PHASE_FRAME, PHASE_NSTEP = 0, 1, 2, 3, 4
+PHASE_STEP, PHASE_RECON, PHASE_TRACE, PHASE_FRAME, PHASE_NSTEP, PHASE_WINDOW_RECON = 0, 1, 2, 3, 4, 5
 
 _STATUS = {0: "PSF_OK", -1: "PSF_ERR_ARG", -2: "PSF_ERR_HIP", -3: "PSF_ERR_NOMEM", -4: "PSF_ERR_STATE",
            -5: "PSF_ERR_UNSUPPORTED", -6: "PSF_ERR_NONFINITE"}
@@ -76,6 +76,7 @@ def load():
         "psf_get_velocity": (i32, [vp, vp]),
         "psf_trace": (i32, [vp, vp]),
         "psf_integrate": (i32, [vp, i32]),
+        "psf_set_frame_window": (i32, [vp, ct.c_uint32]),
         "psf_time_phase": (i32, [vp, i32, i32, ct.POINTER(ct.c_double)]),
     }
     for name, (res, args) in sig.items():

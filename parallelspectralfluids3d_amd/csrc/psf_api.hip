@@ -129,12 +129,12 @@ int ready_for_step(psf_scene* s)
     return PSF_OK;
 }
 
-int enqueue_steps(psf_scene* s, int n)
+int enqueue_steps(psf_scene* s, int n, double* hist = nullptr, int hist_ld = 0)
 {
     const int K = static_cast<int>(s->d.K);
     hipStream_t st = s->ctx->stream;
     if (s->path == PSF_PATH_PERSISTENT) {
-        hipError_t e = launch_persistent_steps(s, n, st);
+        hipError_t e = launch_persistent_steps(s, n, st, hist, hist_ld);
         if (e != hipSuccess) return set_err(s->ctx, PSF_ERR_HIP, "persistent step launch: %s", hipGetErrorString(e));
         return PSF_OK;
     }
@@ -143,9 +143,15 @@ int enqueue_steps(psf_scene* s, int n)
     for (int it = 0; it < n; ++it) {
         HIPCHK(s->ctx, launch_contract_plain(s->d.c_precision, s->dev.C, s->dev.omega, s->dev.y, K, st));
         HIPCHK(s->ctx, launch_update(s->dev.omega, s->dev.y, s->dev.visc, s->dev.gravity, K, s->dt, s->gravity_active,
-                                     s->dev.status, st));
+                                     s->dev.status, hist, hist_ld, it, st));
     }
     return PSF_OK;
+}
+
+// u of the current frame
+float* current_u(psf_scene* s)
+{
+    return s->win_cursor > 0 ? s->dev.u_hist + size_t(s->win_cursor - 1) * s->dev.ldF : s->dev.u;
 }
 
 int enqueue_reconstruct(psf_scene* s)
@@ -158,9 +164,53 @@ int enqueue_reconstruct(psf_scene* s)
 int enqueue_trace(psf_scene* s, int32_t* dcells)
 {
     TraceParams p = trace_params(s);
-    HIPCHK(s->ctx, launch_trace(p, s->dev.u, s->dev.parts, s->dev.pid, s->n_particles, dcells, s->ctx->stream));
+    HIPCHK(s->ctx, launch_trace(p, current_u(s), s->dev.parts, s->dev.pid, s->n_particles, dcells, s->ctx->stream));
     s->frame++;
     return PSF_OK;
+}
+
+// Ends a frame window early: w and u of the current frame become dev.omega and
+// dev.u again, and the frames computed ahead are dropped.  Every call that
+// changes the scene's state or runs a single phase calls this first.
+int window_flush(psf_scene* s)
+{
+    if (s->win_filled == 0) return PSF_OK;
+    const int t = static_cast<int>(s->win_cursor) - 1;
+    s->win_filled = 0;
+    s->win_cursor = 0;
+    if (t < 0) return PSF_OK;
+    HIPCHK(s->ctx, launch_gather_column(s->dev.hist, static_cast<int>(s->window), t, static_cast<int>(s->d.K),
+                                        s->dev.omega, s->ctx->stream));
+    HIPCHK(s->ctx, hipMemcpyAsync(s->dev.u, s->dev.u_hist + size_t(t) * s->dev.ldF, size_t(s->F) * 4,
+                                  hipMemcpyDeviceToDevice, s->ctx->stream));
+    return PSF_OK;
+}
+
+// One integrate() frame: reduced step, u = Phi w, particle pass
+// (psfsolver.cpp:21-234).  With a frame window of W > 1 the reduced steps and
+// the reconstructions of W frames are computed together when the window runs
+// empty (the reduced dynamics do not depend on the particles), and each frame
+// then only traces with its own u.
+int enqueue_frame(psf_scene* s)
+{
+    int rc;
+    if (s->window <= 1) {
+        if ((rc = enqueue_steps(s, 1))) return rc;
+        if ((rc = enqueue_reconstruct(s))) return rc;
+        return enqueue_trace(s, nullptr);
+    }
+    if (s->win_cursor == s->win_filled) {
+        const int W = static_cast<int>(s->window);
+        s->win_filled = 0;
+        s->win_cursor = 0;
+        if ((rc = enqueue_steps(s, W, s->dev.hist, W))) return rc;
+        HIPCHK(s->ctx, launch_reconstruct_window(s->d.phi_precision, W, s->dev.Phi, s->dev.ldF, s->F,
+                                                 static_cast<int>(s->d.K), s->dev.hist, s->dev.u_hist,
+                                                 s->ctx->stream));
+        s->win_filled = s->window;
+    }
+    s->win_cursor++;
+    return enqueue_trace(s, nullptr);
 }
 
 int refresh_visc(psf_scene* s, const std::vector<double>& lam)
@@ -311,6 +361,7 @@ int psf_scene_destroy(psf_scene* s)
     persistent_scene_free(s);
     dfree(s->dev.omega); dfree(s->dev.lambda); dfree(s->dev.gravity); dfree(s->dev.visc); dfree(s->dev.y);
     dfree(s->dev.status); dfree(s->dev.C); dfree(s->dev.Phi); dfree(s->dev.u); dfree(s->dev.parts); dfree(s->dev.cells); dfree(s->dev.pid);
+    dfree(s->dev.hist); dfree(s->dev.u_hist); dfree(s->dev.omega_view);
     s->ctx->scenes.erase(s);
     delete host_of(s);
     delete s;
@@ -322,6 +373,7 @@ int psf_scene_upload_basis(psf_scene* s, const double* Phi, const double* lambda
     if (!s || !lambda) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
     psf_ctx* ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (int rc0 = window_flush(s)) return rc0;
     const size_t K = s->d.K, F = s->F, ld = s->dev.ldF;
     for (size_t k = 0; k < K; ++k)
         if (!std::isfinite(lambda[k])) return set_err(ctx, PSF_ERR_ARG, "lambda[%zu] is not finite", k);
@@ -363,6 +415,7 @@ int psf_scene_upload_advection(psf_scene* s, const double* C)
     if (!s || !C) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
     psf_ctx* ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (int rc0 = window_flush(s)) return rc0;
     const size_t K = s->d.K, n = K * K * K;
     if (s->d.c_precision == PSF_C_FP64) {
         HIPCHK(ctx, hipMemcpy(s->dev.C, C, n * 8, hipMemcpyHostToDevice));
@@ -419,6 +472,8 @@ int psf_set_params(psf_scene* s, double dt, double viscosity, int gravity_active
 {
     if (!s) return PSF_ERR_ARG;
     if (!std::isfinite(dt) || !std::isfinite(viscosity)) return set_err(s->ctx, PSF_ERR_ARG, "non-finite parameter");
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if (int rc0 = window_flush(s)) return rc0;
     s->dt = dt;
     s->nu = viscosity;
     s->gravity_active = gravity_active ? 1 : 0;
@@ -435,7 +490,36 @@ int psf_set_step_path(psf_scene* s, int path)
         return set_err(s->ctx, PSF_ERR_UNSUPPORTED, "c_precision %d has no plain path", s->d.c_precision);
     if (path == PSF_PATH_PERSISTENT && !persistent_supported(s->d.c_precision, static_cast<int>(s->d.K)))
         return set_err(s->ctx, PSF_ERR_UNSUPPORTED, "c_precision %d has no persistent path", s->d.c_precision);
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if (int rc0 = window_flush(s)) return rc0;
     s->path = path;
+    return PSF_OK;
+}
+
+int psf_set_frame_window(psf_scene* s, uint32_t frames)
+{
+    if (!s) return PSF_ERR_ARG;
+    if (frames != 1 && frames != 2 && frames != 4 && frames != 8 && frames != 16 && frames != 32)
+        return set_err(s->ctx, PSF_ERR_ARG, "frame window must be 1, 2, 4, 8, 16 or 32 (got %u)", frames);
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if (int rc0 = window_flush(s)) return rc0;
+    if (frames == s->window) return PSF_OK;
+    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    dfree(s->dev.hist); dfree(s->dev.u_hist); dfree(s->dev.omega_view);
+    s->dev.hist = nullptr; s->dev.u_hist = nullptr; s->dev.omega_view = nullptr;
+    s->window = 1;
+    if (frames == 1) return PSF_OK;
+    const size_t K = s->d.K;
+    hipError_t e = dalloc(&s->dev.hist, K * frames);
+    if (e == hipSuccess) e = dalloc(&s->dev.u_hist, s->dev.ldF * frames);
+    if (e == hipSuccess) e = dalloc(&s->dev.omega_view, K);
+    if (e != hipSuccess) {
+        dfree(s->dev.hist); dfree(s->dev.u_hist); dfree(s->dev.omega_view);
+        s->dev.hist = nullptr; s->dev.u_hist = nullptr; s->dev.omega_view = nullptr;
+        return set_err(s->ctx, e == hipErrorOutOfMemory ? PSF_ERR_NOMEM : PSF_ERR_HIP, "frame window buffers: %s",
+                       hipGetErrorString(e));
+    }
+    s->window = frames;
     return PSF_OK;
 }
 
@@ -443,6 +527,7 @@ int psf_set_omega(psf_scene* s, const double* omega)
 {
     if (!s || !omega) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if (int rc0 = window_flush(s)) return rc0;
     HIPCHK(s->ctx, hipMemcpyAsync(s->dev.omega, omega, s->d.K * 8, hipMemcpyHostToDevice, s->ctx->stream));
     HIPCHK(s->ctx, launch_zero_int(s->dev.status, 64, s->ctx->stream));
     HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
@@ -454,7 +539,13 @@ int psf_get_omega(psf_scene* s, double* omega)
 {
     if (!s || !omega) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
-    HIPCHK(s->ctx, hipMemcpyAsync(omega, s->dev.omega, s->d.K * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+    const double* src = s->dev.omega;
+    if (s->win_cursor > 0) {
+        HIPCHK(s->ctx, launch_gather_column(s->dev.hist, static_cast<int>(s->window), static_cast<int>(s->win_cursor) - 1,
+                                            static_cast<int>(s->d.K), s->dev.omega_view, s->ctx->stream));
+        src = s->dev.omega_view;
+    }
+    HIPCHK(s->ctx, hipMemcpyAsync(omega, src, s->d.K * 8, hipMemcpyDeviceToHost, s->ctx->stream));
     return check_status(s);
 }
 
@@ -512,6 +603,7 @@ int psf_step(psf_scene* s, int nsteps)
     int rc = ready_for_step(s);
     if (rc) return rc;
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if ((rc = window_flush(s))) return rc;
     return enqueue_steps(s, nsteps);
 }
 
@@ -520,6 +612,7 @@ int psf_reconstruct(psf_scene* s)
     if (!s) return PSF_ERR_ARG;
     if (!s->has_phi || !s->has_omega) return set_err(s->ctx, PSF_ERR_STATE, "Phi or omega missing");
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if (int rc0 = window_flush(s)) return rc0;
     return enqueue_reconstruct(s);
 }
 
@@ -527,7 +620,7 @@ int psf_get_velocity(psf_scene* s, float* u)
 {
     if (!s || !u) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
-    HIPCHK(s->ctx, hipMemcpyAsync(u, s->dev.u, size_t(s->F) * 4, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHK(s->ctx, hipMemcpyAsync(u, current_u(s), size_t(s->F) * 4, hipMemcpyDeviceToHost, s->ctx->stream));
     HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
     return PSF_OK;
 }
@@ -553,25 +646,27 @@ int psf_integrate(psf_scene* s, int nframes)
     int rc = ready_for_step(s);
     if (rc) return rc;
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
-    for (int f = 0; f < nframes; ++f) {
-        if (!s->has_phi) return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
-        if ((rc = enqueue_steps(s, 1))) return rc;
-        if ((rc = enqueue_reconstruct(s))) return rc;
-        if ((rc = enqueue_trace(s, nullptr))) return rc;
-    }
+    if (!s->has_phi) return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
+    for (int f = 0; f < nframes; ++f)
+        if ((rc = enqueue_frame(s))) return rc;
     return PSF_OK;
 }
 
 int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
 {
-    if (!s || !ms || count <= 0 || phase < 0 || phase > 4) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    if (!s || !ms || count <= 0 || phase < 0 || phase > 5) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    if (phase == 5 && s->window <= 1) return set_err(s->ctx, PSF_ERR_STATE, "phase 5 needs a frame window > 1");
     if (phase != 2) {
         int rc = ready_for_step(s);
         if (rc) return rc;
     }
-    if ((phase == 1 || phase == 3) && !s->has_phi) return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
+    if ((phase == 1 || phase == 3 || phase == 5) && !s->has_phi) return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
     psf_ctx* ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (phase == 0 || phase == 1 || phase == 4) {
+        int rc0 = window_flush(s);
+        if (rc0) return rc0;
+    }
     hipEvent_t a, b;
     HIPCHK(ctx, hipEventCreate(&a));
     HIPCHK(ctx, hipEventCreate(&b));
@@ -583,11 +678,13 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
         case 0: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_steps(s, 1); break;
         case 1: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_reconstruct(s); break;
         case 2: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_trace(s, nullptr); break;
-        case 3:
+        case 3: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_frame(s); break;
+        case 5:
             for (int i = 0; i < count && rc == PSF_OK; ++i) {
-                rc = enqueue_steps(s, 1);
-                if (rc == PSF_OK) rc = enqueue_reconstruct(s);
-                if (rc == PSF_OK) rc = enqueue_trace(s, nullptr);
+                hipError_t le = launch_reconstruct_window(s->d.phi_precision, static_cast<int>(s->window), s->dev.Phi,
+                                                          s->dev.ldF, s->F, static_cast<int>(s->d.K), s->dev.hist,
+                                                          s->dev.u_hist, ctx->stream);
+                if (le != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "window reconstruction: %s", hipGetErrorString(le));
             }
             break;
         default: rc = enqueue_steps(s, count); break;
@@ -603,7 +700,7 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    if (rc == PSF_OK && phase != 1 && phase != 2) rc = check_status(s);
+    if (rc == PSF_OK && phase != 1 && phase != 2 && phase != 5) rc = check_status(s);
     return rc;
 }
 

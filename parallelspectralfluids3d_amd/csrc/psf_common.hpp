@@ -33,6 +33,12 @@ struct DeviceScene {
     float4* parts = nullptr;     // max_particles
     int32_t* cells = nullptr;    // 12 * max_particles (debug output of cell lookups)
     uint32_t* pid = nullptr;     // caller index of each stored particle (storage is sorted by voxel)
+    // frame window (psf_set_frame_window): w after each step of the window,
+    // [K][window], u of each of its frames, [window][ldF], and a K-vector for
+    // reading one column back
+    double* hist = nullptr;
+    float* u_hist = nullptr;
+    double* omega_view = nullptr;
 };
 
 }  // namespace psf
@@ -57,6 +63,10 @@ struct psf_scene {
     int path = PSF_PATH_PLAIN;
     uint32_t n_particles = 0;
     uint32_t frame = 0;          // counter of trace passes (respawn draws)
+    // frame window: frames computed ahead (filled) and consumed (cursor);
+    // while filled > 0, the current w and u are column / row cursor-1 of
+    // dev.hist / dev.u_hist and dev.omega holds w at the end of the window
+    uint32_t window = 1, win_filled = 0, win_cursor = 0;
     uint32_t seed = 0;
     void* host = nullptr;        // host-side copies (psf_scene_host)
     void* persistent = nullptr;  // persistent-path state (PersistentState)

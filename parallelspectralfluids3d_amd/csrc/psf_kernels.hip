@@ -75,9 +75,12 @@ __global__ __launch_bounds__(256) void k_contract_plain(const T* __restrict__ C,
 //   e1 = w.w; w += dt*y; e2 = w.w; w *= sqrt(e1/e2); w_k *= exp(nu lambda_k dt);
 //   w += dt*g if forcing.
 // A zero or non-finite energy sets *status and leaves w unchanged.
+// With hist != nullptr the new w is also stored at hist[k * hist_ld + hist_t]
+// (one column of a frame window).
 __global__ __launch_bounds__(256) void k_update(double* __restrict__ omega, const double* __restrict__ y,
                                                 const double* __restrict__ visc, const double* __restrict__ g,
-                                                int K, double dt, int gravity_active, int* __restrict__ status)
+                                                int K, double dt, int gravity_active, int* __restrict__ status,
+                                                double* __restrict__ hist, int hist_ld, int hist_t)
 {
 #pragma clang fp contract(off)
     extern __shared__ double swn[];
@@ -102,6 +105,7 @@ __global__ __launch_bounds__(256) void k_update(double* __restrict__ omega, cons
         w = w * visc[k];
         if (gravity_active) w = w + dt * g[k];
         omega[k] = w;
+        if (hist) hist[static_cast<size_t>(k) * hist_ld + hist_t] = w;
     }
 }
 
@@ -110,7 +114,9 @@ __global__ __launch_bounds__(256) void k_update(double* __restrict__ omega, cons
 // zero-padded leading dimension ldF (a multiple of 256), so every lane issues
 // aligned 16-byte loads of VEC consecutive faces.  The four waves of a block
 // share 64*VEC faces and split the modes (k = wave, wave+4, ...); partial sums
-// are FP64 and combined in a fixed order.
+// are FP64 and combined in a fixed order.  The per-face arithmetic -- an fma
+// chain over the modes of each wave, then ((p0 + p1) + p2) + p3 -- is shared
+// with k_reconstruct_window, so both give the same bits.
 template <typename T>
 __global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
                                                      const double* __restrict__ omega, float* __restrict__ u)
@@ -132,14 +138,14 @@ __global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, 
         const double w = sw[k];
         if constexpr (VEC == 4) {
             const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(base + static_cast<size_t>(k) * ldF));
-            acc[0] += static_cast<double>(v.x) * w;
-            acc[1] += static_cast<double>(v.y) * w;
-            acc[2] += static_cast<double>(v.z) * w;
-            acc[3] += static_cast<double>(v.w) * w;
+            acc[0] = __builtin_fma(static_cast<double>(v.x), w, acc[0]);
+            acc[1] = __builtin_fma(static_cast<double>(v.y), w, acc[1]);
+            acc[2] = __builtin_fma(static_cast<double>(v.z), w, acc[2]);
+            acc[3] = __builtin_fma(static_cast<double>(v.w), w, acc[3]);
         } else {
             const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(base + static_cast<size_t>(k) * ldF));
-            acc[0] += v.x * w;
-            acc[1] += v.y * w;
+            acc[0] = __builtin_fma(v.x, w, acc[0]);
+            acc[1] = __builtin_fma(v.y, w, acc[1]);
         }
     }
     if (wv > 0) {
@@ -155,6 +161,89 @@ __global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, 
             if (f < F) u[f] = static_cast<float>(s);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction for a window of TT frames: u_t = float(Phi w_t), t < TT,
+// from the history hist[k * TT + t] of w.  One pass over Phi serves all TT
+// frames, so the kernel is bound by FP64 FMA throughput instead of HBM.
+// Each lane owns FPT consecutive faces and all TT frames; wave w of a block
+// takes the modes k = w, w+4, ... in increasing order, exactly as
+// k_reconstruct, and the four partial sums are combined in the same order,
+// so u_t has the bits k_reconstruct gives for w_t.  The w_t of one mode are
+// uniform across the wave and come through scalar loads.
+template <typename T, int FPT, int TT>
+__global__ __launch_bounds__(256) void k_reconstruct_window(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
+                                                            const double* __restrict__ hist,
+                                                            float* __restrict__ uh)
+{
+    constexpr int CH = TT < 8 ? TT : 8;      // frames combined per LDS round
+    __shared__ double part[3][CH][kWave * FPT];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t f0 = static_cast<size_t>(blockIdx.x) * (kWave * FPT) + static_cast<size_t>(lane) * FPT;
+    double acc[FPT][TT];
+#pragma unroll
+    for (int j = 0; j < FPT; ++j)
+#pragma unroll
+        for (int t = 0; t < TT; ++t) acc[j][t] = 0.0;
+    const T* base = Phi + f0;
+#pragma unroll 2
+    for (int k = wv; k < K; k += 4) {
+        double ph[FPT];
+        if constexpr (sizeof(T) == 4 && FPT == 4) {
+            const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(base + static_cast<size_t>(k) * ldF));
+            ph[0] = v.x; ph[1] = v.y; ph[2] = v.z; ph[3] = v.w;
+        } else if constexpr (sizeof(T) == 4 && FPT == 2) {
+            typedef float f32x2 __attribute__((ext_vector_type(2)));
+            const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(base + static_cast<size_t>(k) * ldF));
+            ph[0] = v.x; ph[1] = v.y;
+        } else {
+#pragma unroll
+            for (int j = 0; j < FPT; j += 2) {
+                const f64x2 v = __builtin_nontemporal_load(
+                    reinterpret_cast<const f64x2*>(base + static_cast<size_t>(k) * ldF + j));
+                ph[j] = v.x; ph[j + 1] = v.y;
+            }
+        }
+        const double* wk = hist + static_cast<size_t>(k) * TT;
+#pragma unroll
+        for (int t = 0; t < TT; ++t) {
+            const double w = wk[t];
+#pragma unroll
+            for (int j = 0; j < FPT; ++j) acc[j][t] = __builtin_fma(ph[j], w, acc[j][t]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < TT; c += CH) {
+        if (wv > 0) {
+#pragma unroll
+            for (int t = 0; t < CH; ++t)
+#pragma unroll
+                for (int j = 0; j < FPT; ++j) part[wv - 1][t][lane * FPT + j] = acc[j][c + t];
+        }
+        __syncthreads();
+        if (wv == 0) {
+#pragma unroll
+            for (int t = 0; t < CH; ++t)
+#pragma unroll
+                for (int j = 0; j < FPT; ++j) {
+                    const int q = lane * FPT + j;
+                    const double s = ((acc[j][c + t] + part[0][t][q]) + part[1][t][q]) + part[2][t][q];
+                    const size_t f = f0 + j;
+                    if (f < F) uh[static_cast<size_t>(c + t) * ldF + f] = static_cast<float>(s);
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// out[k] = hist[k * ld + t]: one column of a window history
+__global__ __launch_bounds__(256) void k_gather_column(const double* __restrict__ hist, int ld, int t, int K,
+                                                       double* __restrict__ out)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < K) out[k] = hist[static_cast<size_t>(k) * ld + t];
 }
 
 // ---------------------------------------------------------------------------
@@ -333,9 +422,10 @@ hipError_t launch_contract_plain(int prec, const void* C, const double* omega, d
 }
 
 hipError_t launch_update(double* omega, const double* y, const double* visc, const double* g, int K, double dt,
-                         int gravity_active, int* status, hipStream_t s)
+                         int gravity_active, int* status, double* hist, int hist_ld, int hist_t, hipStream_t s)
 {
-    k_update<<<1, 256, sizeof(double) * K, s>>>(omega, y, visc, g, K, dt, gravity_active, status);
+    k_update<<<1, 256, sizeof(double) * K, s>>>(omega, y, visc, g, K, dt, gravity_active, status, hist, hist_ld,
+                                                 hist_t);
     return hipGetLastError();
 }
 
@@ -350,6 +440,43 @@ hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F,
         const unsigned grid = static_cast<unsigned>(ldF / (kWave * 2));
         k_reconstruct<double><<<grid, 256, sh, s>>>(static_cast<const double*>(Phi), ldF, F, K, omega, u);
     }
+    return hipGetLastError();
+}
+
+namespace {
+template <typename T, int FPT, int TT>
+hipError_t window_launch(const void* Phi, size_t ldF, uint32_t F, int K, const double* hist, float* uh, hipStream_t s)
+{
+    const unsigned grid = static_cast<unsigned>(ldF / (kWave * FPT));
+    k_reconstruct_window<T, FPT, TT><<<grid, 256, 0, s>>>(static_cast<const T*>(Phi), ldF, F, K, hist, uh);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t window_dispatch(int window, const void* Phi, size_t ldF, uint32_t F, int K, const double* hist, float* uh,
+                           hipStream_t s)
+{
+    switch (window) {
+    case 2: return window_launch<T, 4, 2>(Phi, ldF, F, K, hist, uh, s);
+    case 4: return window_launch<T, 4, 4>(Phi, ldF, F, K, hist, uh, s);
+    case 8: return window_launch<T, 4, 8>(Phi, ldF, F, K, hist, uh, s);
+    case 16: return window_launch<T, 4, 16>(Phi, ldF, F, K, hist, uh, s);
+    case 32: return window_launch<T, 2, 32>(Phi, ldF, F, K, hist, uh, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+}  // namespace
+
+hipError_t launch_reconstruct_window(int prec, int window, const void* Phi, size_t ldF, uint32_t F, int K,
+                                     const double* hist, float* uh, hipStream_t s)
+{
+    if (ldF % 256 != 0) return hipErrorInvalidValue;
+    return prec == PSF_PHI_FP32 ? window_dispatch<float>(window, Phi, ldF, F, K, hist, uh, s)
+                                : window_dispatch<double>(window, Phi, ldF, F, K, hist, uh, s);
+}
+
+hipError_t launch_gather_column(const double* hist, int ld, int t, int K, double* out, hipStream_t s)
+{
+    k_gather_column<<<(K + 255) / 256, 256, 0, s>>>(hist, ld, t, K, out);
     return hipGetLastError();
 }
 

@@ -22,11 +22,19 @@ struct TraceParams {
 // FP64 update of w (one block)
 hipError_t launch_contract_plain(int c_precision, const void* C, const double* omega, double* y, int K,
                                  hipStream_t s);
+// hist != nullptr also stores the new w at hist[k * hist_ld + hist_t]
 hipError_t launch_update(double* omega, const double* y, const double* visc, const double* gravity, int K,
-                         double dt, int gravity_active, int* status, hipStream_t s);
+                         double dt, int gravity_active, int* status, double* hist, int hist_ld, int hist_t,
+                         hipStream_t s);
 // u = float(Phi w)
 hipError_t launch_reconstruct(int phi_precision, const void* Phi, size_t ldF, uint32_t F, int K,
                               const double* omega, float* u, hipStream_t s);
+// u_t = float(Phi w_t) for t < window (2, 4, 8, 16 or 32), w_t = hist[k * window + t],
+// u_t at uh + t * ldF; bitwise equal to launch_reconstruct for each w_t
+hipError_t launch_reconstruct_window(int phi_precision, int window, const void* Phi, size_t ldF, uint32_t F, int K,
+                                     const double* hist, float* uh, hipStream_t s);
+// out[k] = hist[k * ld + t]
+hipError_t launch_gather_column(const double* hist, int ld, int t, int K, double* out, hipStream_t s);
 // one particle pass
 hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, const uint32_t* pid, uint32_t n,
                         int32_t* cells, hipStream_t s);

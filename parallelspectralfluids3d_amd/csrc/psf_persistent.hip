@@ -88,6 +88,8 @@ struct PArgs {
     int K, Kp, G, ipw, bpw, nsteps;
     uint32_t tag0;
     float col1, wscale;
+    double* hist;                // optional: w after step s at hist[k * hist_ld + s]
+    int hist_ld;
 };
 
 static inline PersistentState* st(psf_scene* s) { return reinterpret_cast<PersistentState*>(s->persistent); }
@@ -381,6 +383,7 @@ __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
                 if (a.gravity_active) w = w + a.dt * s_grav[k];
                 s_w[k] = w;
                 split_one(k, w);
+                if (a.hist && g == 0) a.hist[static_cast<size_t>(k) * a.hist_ld + s] = w;
             }
             if (tid < 4 * kMaxIpw) s_ypart[tid / kMaxIpw][tid % kMaxIpw] = 0.0;
         }
@@ -560,7 +563,7 @@ int persistent_upload_c(psf_scene* s, const double* C)
     return PSF_OK;
 }
 
-hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream)
+hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream, double* hist, int hist_ld)
 {
     PersistentState* p = st(s);
     if (!p) return hipErrorNotSupported;
@@ -585,6 +588,8 @@ hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream)
     a.tag0 = p->tag;
     a.col1 = p->col1;
     a.wscale = p->wscale;
+    a.hist = hist;
+    a.hist_ld = hist_ld;
     void* args[] = {&a};
     hipError_t e = hipLaunchKernel(p->fn, dim3(p->G), dim3(256), args, 0, stream);
     if (e == hipSuccess) p->tag += static_cast<uint32_t>(nsteps);

@@ -188,6 +188,11 @@ class Scene:
     def integrate(self, nframes=1):
         self._c(self.ctx.lib.psf_integrate(self.h, int(nframes)), "integrate")
 
+    def set_frame_window(self, frames):
+        """Frames whose reduced steps and reconstructions are computed together
+        (1 = one frame at a time)."""
+        self._c(self.ctx.lib.psf_set_frame_window(self.h, int(frames)), "set_frame_window")
+
     def time_phase(self, phase, count):
         ms = ct.c_double()
         self._c(self.ctx.lib.psf_time_phase(self.h, int(phase), int(count), ct.byref(ms)), "time_phase")

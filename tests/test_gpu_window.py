@@ -1,0 +1,132 @@
+"""Frame window: W frames' reduced steps in one launch and their
+reconstructions in one pass over Phi, then one trace per integrate().  The
+reduced dynamics (psfsolver.cpp:34-59) do not depend on the particles, so
+omega, u and the particles must be bitwise those of W single frames
+(psfsolver.cpp:21-234), whatever the calls in between."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from scenes import make_scene, particles_uniform
+
+pytestmark = pytest.mark.gpu
+
+C_FP64, C_FP32, C_BF16X2 = 0, 1, 4
+PHI_FP64, PHI_FP32 = 0, 1
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _pair(ctx, sd, c_prec, phi_prec, N, window, lifetime=60.0, seed=3):
+    from parallelspectralfluids3d_amd import hip
+    out = []
+    p = particles_uniform(sd, N, seed=seed, lifetime=10.0)
+    for W in (1, window):
+        d = hip.make_desc(sd.grid, sd.resolution, sd.K, c_prec, phi_prec, N, sd.aabb_min, sd.aabb_max)
+        s = hip.Scene(ctx, d)
+        s.upload_basis(sd.Phi, sd.lam, sd.g)
+        s.upload_advection(sd.C)
+        s.set_omega(sd.omega)
+        s.set_params(0.01, 0.0, False, lifetime=lifetime)
+        s.set_particles(p)
+        s.set_frame_window(W)
+        out.append(s)
+    return out
+
+
+def _same(a, b):
+    wa, wb = a.get_omega(), b.get_omega()
+    ua, ub = a.get_velocity(), b.get_velocity()
+    pa, pb = a.get_particles(), b.get_particles()
+    assert np.array_equal(wa.view(np.uint64), wb.view(np.uint64))
+    assert np.array_equal(ua.view(np.uint32), ub.view(np.uint32))
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+
+
+@pytest.mark.parametrize("window", [2, 8, 32])
+@pytest.mark.parametrize("c_prec,phi_prec", [(C_FP64, PHI_FP64), (C_FP64, PHI_FP32), (C_BF16X2, PHI_FP32)])
+def test_window_bitwise_equal_single_frames(hip_ctx, window, c_prec, phi_prec):
+    """Read back mid-window and at window ends: omega, u and particles equal."""
+    _say(f"window={window} c={c_prec} phi={phi_prec}")
+    sd = make_scene(n=8, K=16)
+    one, win = _pair(hip_ctx, sd, c_prec, phi_prec, 1024, window)
+    for n in (1, window - 1 if window > 2 else 1, window + 3, 2 * window):
+        one.integrate(n)
+        win.integrate(n)
+        _same(one, win)
+
+
+def test_window_state_changes_mid_window(hip_ctx):
+    """set_params, psf_step, psf_reconstruct, set_omega and a window change in
+    the middle of a window give what single frames give."""
+    _say("window state changes")
+    sd = make_scene(n=8, K=16)
+    one, win = _pair(hip_ctx, sd, C_FP64, PHI_FP32, 512, 8)
+    for s in (one, win):
+        s.integrate(3)
+        s.set_params(0.02, 0.5, True, lifetime=60.0)
+        s.integrate(5)
+        s.step(2)
+        s.integrate(9)
+        s.reconstruct()
+        s.trace()
+        s.integrate(4)
+    _same(one, win)
+    w = np.linspace(-1.0, 1.0, sd.K)
+    for s in (one, win):
+        s.integrate(2)
+        s.set_omega(w)
+        s.integrate(11)
+    _same(one, win)
+    win.set_frame_window(32)
+    for s in (one, win):
+        s.integrate(40)
+    _same(one, win)
+    win.set_frame_window(1)
+    for s in (one, win):
+        s.integrate(5)
+    _same(one, win)
+
+
+def test_window_frames_match_oracle(hip_ctx):
+    """40 frames with a window of 16 against the oracle run frame by frame."""
+    _say("window vs oracle")
+    from parallelspectralfluids3d_amd import hip
+    sd = make_scene(n=8, K=16)
+    N = 1024
+    d = hip.make_desc(sd.grid, sd.resolution, sd.K, C_FP64, PHI_FP32, N, sd.aabb_min, sd.aabb_max)
+    s = hip.Scene(hip_ctx, d)
+    s.upload_basis(sd.Phi, sd.lam, sd.g)
+    s.upload_advection(sd.C)
+    s.set_omega(sd.omega)
+    s.set_params(0.01, 0.0, False, lifetime=60.0)
+    p = particles_uniform(sd, N, seed=9)
+    s.set_particles(p)
+    s.set_frame_window(16)
+    m = O.Mesh((sd.grid.nx, sd.grid.ny, sd.grid.nz), sd.resolution, sd.aabb_min, inside=sd.inside)
+    w = sd.omega.copy()
+    for f in range(40):
+        w = O.reduced_step(w, sd.C, sd.lam, sd.g, 0.01, 0.0, False)
+        u = O.reconstruct(sd.Phi, w)
+        p = O.particles(m, u, p, resolution=sd.resolution, timestep=0.01, aabb_min=sd.aabb_min,
+                        aabb_max=sd.aabb_max, lifetime=60.0, seed=0, step=f)
+    s.integrate(40)
+    wg, ug, pg = s.get_omega(), s.get_velocity(), s.get_particles()
+    assert np.linalg.norm(wg - w) / np.linalg.norm(w) <= 1e-12
+    assert np.abs(ug - u).max() <= 2 * np.spacing(np.float32(np.abs(u).max()))
+    assert np.abs(pg[:, :3] - p[:, :3]).max() <= 1e-6
+
+
+def test_window_rejects_bad_sizes(hip_ctx):
+    from parallelspectralfluids3d_amd import _lib as L
+    sd = make_scene(n=8, K=16)
+    one, win = _pair(hip_ctx, sd, C_FP64, PHI_FP32, 64, 4)
+    for bad in (0, 3, 64):
+        with pytest.raises(L.PsfError) as e:
+            win.set_frame_window(bad)
+        assert e.value.status == L.PSF_ERR_ARG
