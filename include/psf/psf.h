@@ -73,6 +73,7 @@ typedef enum psf_step_path {
 
 typedef struct psf_ctx psf_ctx;
 typedef struct psf_scene psf_scene;
+typedef struct psf_server psf_server;
 
 /* Grid and basis description.  Counts follow DECMesh3D (DEC/decmesh3d.cpp:64-70):
    F = (nz+1)nx ny + (ny+1)nx nz + (nx+1)ny nz faces, V = nx ny nz voxels. */
@@ -174,6 +175,61 @@ PSF_API int psf_set_frame_window(psf_scene* scene, uint32_t frames);
  *        5 the frame window's reconstruction (all W frames; window > 1,
  *          recomputes the u of the frames already computed ahead). */
 PSF_API int psf_time_phase(psf_scene* scene, int phase, int count, double* ms);
+
+/* ---- scene server (continuous batching) ----------------------------------------
+ * No counterpart in the reference, which runs one scene per solver.  A server
+ * keeps a set of scenes on the device and advances all of them together: each
+ * round runs up to `window` frames of every active scene -- one persistent
+ * launch for all their reduced steps, one launch for all their velocity
+ * fields, one launch for all their particle passes.  Requests wait in a FIFO
+ * queue and are admitted at the start of a round when a slot, workgroups of
+ * the persistent launch and device memory (the request's footprint against
+ * mem_budget) are free; finished ones leave at the end of their round.
+ * Scenes share one basis per grid (Phi, C of K0 modes); a request uses its
+ * first K modes, packed on the device.  A request's result does not depend
+ * on what else runs in its rounds.  Frames are integrate() frames
+ * (psfsolver.cpp:21-234) with C in BF16 hi+lo, FP32 Phi and FP64 omega. */
+typedef struct psf_server_config {
+    uint32_t window;        /* frames per round: 2, 4, 8, 16 or 32 (0 = 32) */
+    uint32_t max_active;    /* scenes on the device at once, 1..64 (0 = 32) */
+    uint64_t mem_budget;    /* device bytes for admitted requests (0 = 64 GiB) */
+    uint32_t modes_per_wg;  /* modes per workgroup of the persistent layout, 1..8 (0 = 4) */
+} psf_server_config;
+
+typedef struct psf_server_stats {
+    uint64_t submitted, completed, failed, rounds, scene_frames;
+    uint32_t active, queued;
+    uint64_t mem_in_use, mem_peak;   /* bytes held by admitted requests */
+    double busy_ms;                  /* wall time inside rounds */
+} psf_server_stats;
+
+PSF_API int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** out);
+PSF_API int psf_server_destroy(psf_server* server);
+/* A shared basis: desc gives the grid and K0 = desc->K (c_precision and
+   phi_precision are ignored); Phi F x K0 column-major, lambda and gravity K0,
+   C as for psf_scene_upload_advection with K0 modes. */
+PSF_API int psf_server_add_basis(psf_server* server, const psf_scene_desc* desc, const double* Phi,
+                                 const double* lambda, const double* gravity, const double* C, uint32_t* basis_id);
+/* Device bytes a request would hold (its admission footprint). */
+PSF_API int psf_server_request_footprint(psf_server* server, uint32_t basis_id, uint32_t K, uint32_t n_particles,
+                                         uint64_t* bytes);
+/* Queue a request: nframes frames of the first K <= K0 modes of the basis,
+   starting from omega (K) and n particles (x, y, z, w); the caller's buffers
+   are copied. */
+PSF_API int psf_server_submit(psf_server* server, uint32_t basis_id, uint32_t K, uint32_t nframes, double dt,
+                              double viscosity, int gravity_active, float lifetime, const double* omega,
+                              const float* particles, uint32_t n_particles, uint64_t* request_id);
+/* One round: admit, advance every active scene, retire the finished ones.
+   *completed receives the number of requests that finished in this round. */
+PSF_API int psf_server_round(psf_server* server, uint32_t* completed);
+/* 1 if the request finished (its result is ready), 0 if queued or running. */
+PSF_API int psf_server_done(psf_server* server, uint64_t request_id);
+/* Result of a finished request: omega (K), particles (n x 4, caller's order),
+   latency from submit to the end of its last round in milliseconds.  The
+   request is forgotten afterwards. */
+PSF_API int psf_server_result(psf_server* server, uint64_t request_id, double* omega, float* particles,
+                              double* latency_ms);
+PSF_API int psf_server_get_stats(psf_server* server, psf_server_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -37,6 +37,18 @@ class SceneDesc(ct.Structure):
                 ("max_particles", ct.c_uint32)]
 
 
+class ServerConfig(ct.Structure):
+    _fields_ = [("window", ct.c_uint32), ("max_active", ct.c_uint32), ("mem_budget", ct.c_uint64),
+                ("modes_per_wg", ct.c_uint32)]
+
+
+class ServerStats(ct.Structure):
+    _fields_ = [("submitted", ct.c_uint64), ("completed", ct.c_uint64), ("failed", ct.c_uint64),
+                ("rounds", ct.c_uint64), ("scene_frames", ct.c_uint64), ("active", ct.c_uint32),
+                ("queued", ct.c_uint32), ("mem_in_use", ct.c_uint64), ("mem_peak", ct.c_uint64),
+                ("busy_ms", ct.c_double)]
+
+
 _lib = None
 
 
@@ -78,6 +90,16 @@ def load():
         "psf_integrate": (i32, [vp, i32]),
         "psf_set_frame_window": (i32, [vp, ct.c_uint32]),
         "psf_time_phase": (i32, [vp, i32, i32, ct.POINTER(ct.c_double)]),
+        "psf_server_create": (i32, [vp, ct.POINTER(ServerConfig), pp]),
+        "psf_server_destroy": (i32, [vp]),
+        "psf_server_add_basis": (i32, [vp, ct.POINTER(SceneDesc), vp, vp, vp, vp, ct.POINTER(ct.c_uint32)]),
+        "psf_server_request_footprint": (i32, [vp, u32, u32, u32, ct.POINTER(ct.c_uint64)]),
+        "psf_server_submit": (i32, [vp, u32, u32, u32, dbl, dbl, i32, flt, vp, vp, u32,
+                                    ct.POINTER(ct.c_uint64)]),
+        "psf_server_round": (i32, [vp, ct.POINTER(ct.c_uint32)]),
+        "psf_server_done": (i32, [vp, ct.c_uint64]),
+        "psf_server_result": (i32, [vp, ct.c_uint64, vp, vp, ct.POINTER(ct.c_double)]),
+        "psf_server_get_stats": (i32, [vp, ct.POINTER(ServerStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -8,10 +8,85 @@
 #include <numeric>
 #include <vector>
 
+#include <map>
+#include <unordered_map>
+
+#include "psf_internal.hpp"
 #include "psf_kernels.hpp"
 #include "psf_persistent.hpp"
 
 using namespace psf;
+
+namespace psf {
+
+// Size classes (powers of two from 4 KiB) of device memory, recycled
+// between the pooled scenes of one context; returned to HIP with the context.
+struct DevicePool {
+    std::multimap<size_t, void*> idle;
+    std::unordered_map<void*, size_t> live;
+    size_t held = 0;
+};
+
+static size_t pool_class(size_t bytes)
+{
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
+hipError_t scene_alloc(psf_scene* s, void** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return hipSuccess;
+    if (!s->pooled) return hipMalloc(p, bytes);
+    DevicePool*& pool = s->ctx->pool;
+    if (!pool) {
+        pool = new (std::nothrow) DevicePool();
+        if (!pool) return hipErrorOutOfMemory;
+    }
+    const size_t c = pool_class(bytes);
+    auto it = pool->idle.find(c);
+    if (it != pool->idle.end()) {
+        *p = it->second;
+        pool->idle.erase(it);
+    } else {
+        hipError_t e = hipMalloc(p, c);
+        if (e != hipSuccess) return e;
+        pool->held += c;
+    }
+    pool->live[*p] = c;
+    return hipSuccess;
+}
+
+void scene_free(psf_scene* s, void* p)
+{
+    if (!p) return;
+    if (!s->pooled || !s->ctx->pool) {
+        (void)hipFree(p);
+        return;
+    }
+    DevicePool* pool = s->ctx->pool;
+    auto it = pool->live.find(p);
+    if (it == pool->live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    pool->idle.emplace(it->second, p);
+    pool->live.erase(it);
+}
+
+size_t pool_class_bytes(size_t bytes) { return bytes ? pool_class(bytes) : 0; }
+
+static void pool_release(psf_ctx* c)
+{
+    if (!c->pool) return;
+    for (auto& kv : c->pool->idle) (void)hipFree(kv.second);
+    for (auto& kv : c->pool->live) (void)hipFree(kv.first);
+    delete c->pool;
+    c->pool = nullptr;
+}
+
+}  // namespace psf
 
 namespace {
 
@@ -36,17 +111,12 @@ int set_err(psf_ctx* ctx, int code, const char* fmt, ...)
     } while (0)
 
 template <typename T>
-hipError_t dalloc(T** p, size_t n)
+hipError_t salloc(psf_scene* s, T** p, size_t n)
 {
-    *p = nullptr;
-    if (n == 0) return hipSuccess;
-    return hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+    return scene_alloc(s, reinterpret_cast<void**>(p), n * sizeof(T));
 }
 
-void dfree(void* p)
-{
-    if (p) (void)hipFree(p);
-}
+void sfree(psf_scene* s, void* p) { scene_free(s, p); }
 
 size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
@@ -225,6 +295,16 @@ int refresh_visc(psf_scene* s, const std::vector<double>& lam)
 
 }  // namespace
 
+namespace psf {
+TraceParams scene_trace_params(const psf_scene* s) { return trace_params(s); }
+int scene_check_status(psf_scene* s) { return check_status(s); }
+int ctx_set_error(psf_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+}  // namespace psf
+
 // host-side copies kept beside a scene: lambda (viscosity refreshes) and the
 // caller index of every stored particle
 struct psf_scene_host {
@@ -259,7 +339,9 @@ int psf_ctx_destroy(psf_ctx* ctx)
 {
     if (!ctx) return PSF_ERR_ARG;
     (void)hipSetDevice(ctx->device);
+    while (!ctx->servers.empty()) psf_server_destroy(*ctx->servers.begin());
     while (!ctx->scenes.empty()) psf_scene_destroy(*ctx->scenes.begin());
+    pool_release(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PSF_OK;
@@ -291,11 +373,19 @@ int psf_scene_footprint(const psf_scene_desc* d, uint64_t* bytes)
 
 int psf_scene_create(psf_ctx* ctx, const psf_scene_desc* desc, psf_scene** out)
 {
+    return scene_create_ex(ctx, desc, SceneOptions{}, out);
+}
+
+}  // extern "C"
+
+int psf::scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOptions& opt, psf_scene** out)
+{
     if (!ctx || !out) return set_err(ctx, PSF_ERR_ARG, "null argument");
     *out = nullptr;
     int rc = validate_desc(ctx, desc);
     if (rc) return rc;
-    if (!is_plain_precision(desc->c_precision) && !persistent_supported(desc->c_precision, static_cast<int>(desc->K)))
+    if (!is_plain_precision(desc->c_precision) &&
+        !persistent_supported(desc->c_precision, static_cast<int>(desc->K), opt.modes_per_wg))
         return set_err(ctx, PSF_ERR_UNSUPPORTED, "c_precision %d with K=%u is not built", desc->c_precision, desc->K);
     HIPCHK(ctx, hipSetDevice(ctx->device));
     psf_scene* s = new (std::nothrow) psf_scene();
@@ -308,26 +398,36 @@ int psf_scene_create(psf_ctx* ctx, const psf_scene_desc* desc, psf_scene** out)
     s->ctx = ctx;
     s->d = *desc;
     s->host = h;
+    s->pooled = opt.pooled;
+    s->modes_per_wg = opt.modes_per_wg;
     counts(*desc, s->V, s->F, s->nZF, s->nYF, s->nXF);
     s->path = is_plain_precision(desc->c_precision) ? PSF_PATH_PLAIN : PSF_PATH_PERSISTENT;
     const size_t K = desc->K;
     s->dev.ldF = round_up(s->F, 256);
-    s->dev.C_bytes = c_bytes(*desc);
+    s->dev.C_bytes = is_plain_precision(desc->c_precision)
+                         ? c_bytes(*desc)
+                         : persistent_c_bytes(desc->c_precision, static_cast<int>(K), opt.modes_per_wg);
     const size_t phi_bytes = s->dev.ldF * K * (desc->phi_precision == PSF_PHI_FP32 ? 4 : 8);
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
-    chk(dalloc(&s->dev.omega, K));
-    chk(dalloc(&s->dev.lambda, K));
-    chk(dalloc(&s->dev.gravity, K));
-    chk(dalloc(&s->dev.visc, K));
-    chk(dalloc(&s->dev.y, K));
-    chk(dalloc(&s->dev.status, 64));
-    chk(dalloc(reinterpret_cast<unsigned char**>(&s->dev.C), s->dev.C_bytes));
-    chk(dalloc(reinterpret_cast<unsigned char**>(&s->dev.Phi), phi_bytes));
-    chk(dalloc(&s->dev.u, s->F));
-    chk(dalloc(&s->dev.parts, desc->max_particles));
-    chk(dalloc(&s->dev.cells, size_t(desc->max_particles) * 12));
-    chk(dalloc(&s->dev.pid, desc->max_particles));
+    chk(salloc(s, &s->dev.omega, K));
+    chk(salloc(s, &s->dev.lambda, K));
+    chk(salloc(s, &s->dev.gravity, K));
+    chk(salloc(s, &s->dev.visc, K));
+    chk(salloc(s, &s->dev.y, K));
+    chk(salloc(s, &s->dev.status, 64));
+    chk(salloc(s, reinterpret_cast<unsigned char**>(&s->dev.C), s->dev.C_bytes));
+    if (opt.borrowed_phi) {
+        s->dev.Phi = opt.borrowed_phi;
+        s->phi_borrowed = true;
+        s->has_phi = true;
+    } else {
+        chk(salloc(s, reinterpret_cast<unsigned char**>(&s->dev.Phi), phi_bytes));
+    }
+    chk(salloc(s, &s->dev.u, s->F));
+    chk(salloc(s, &s->dev.parts, desc->max_particles));
+    if (opt.cells) chk(salloc(s, &s->dev.cells, size_t(desc->max_particles) * 12));
+    chk(salloc(s, &s->dev.pid, desc->max_particles));
     if (e == hipSuccess) e = persistent_scene_init(s);
     if (e != hipSuccess) {
         psf_scene_destroy(s);
@@ -336,7 +436,7 @@ int psf_scene_create(psf_ctx* ctx, const psf_scene_desc* desc, psf_scene** out)
     }
     // zero padding of Phi (rows F..ldF-1 of each column are read by the
     // vectorised reconstruction) and the status word
-    chk(hipMemsetAsync(s->dev.Phi, 0, phi_bytes, ctx->stream));
+    if (!s->phi_borrowed) chk(hipMemsetAsync(s->dev.Phi, 0, phi_bytes, ctx->stream));
     chk(launch_zero_int(s->dev.status, 64, ctx->stream));
     std::vector<double> ones(K, 1.0), zeros(K, 0.0);
     chk(hipMemcpyAsync(s->dev.visc, ones.data(), K * 8, hipMemcpyHostToDevice, ctx->stream));
@@ -353,15 +453,19 @@ int psf_scene_create(psf_ctx* ctx, const psf_scene_desc* desc, psf_scene** out)
     return PSF_OK;
 }
 
+extern "C" {
+
 int psf_scene_destroy(psf_scene* s)
 {
     if (!s) return PSF_ERR_ARG;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     persistent_scene_free(s);
-    dfree(s->dev.omega); dfree(s->dev.lambda); dfree(s->dev.gravity); dfree(s->dev.visc); dfree(s->dev.y);
-    dfree(s->dev.status); dfree(s->dev.C); dfree(s->dev.Phi); dfree(s->dev.u); dfree(s->dev.parts); dfree(s->dev.cells); dfree(s->dev.pid);
-    dfree(s->dev.hist); dfree(s->dev.u_hist); dfree(s->dev.omega_view);
+    sfree(s, s->dev.omega); sfree(s, s->dev.lambda); sfree(s, s->dev.gravity); sfree(s, s->dev.visc);
+    sfree(s, s->dev.y); sfree(s, s->dev.status); sfree(s, s->dev.C);
+    if (!s->phi_borrowed) sfree(s, s->dev.Phi);
+    sfree(s, s->dev.u); sfree(s, s->dev.parts); sfree(s, s->dev.cells); sfree(s, s->dev.pid);
+    sfree(s, s->dev.hist); sfree(s, s->dev.u_hist); sfree(s, s->dev.omega_view);
     s->ctx->scenes.erase(s);
     delete host_of(s);
     delete s;
@@ -505,16 +609,16 @@ int psf_set_frame_window(psf_scene* s, uint32_t frames)
     if (int rc0 = window_flush(s)) return rc0;
     if (frames == s->window) return PSF_OK;
     HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
-    dfree(s->dev.hist); dfree(s->dev.u_hist); dfree(s->dev.omega_view);
+    sfree(s, s->dev.hist); sfree(s, s->dev.u_hist); sfree(s, s->dev.omega_view);
     s->dev.hist = nullptr; s->dev.u_hist = nullptr; s->dev.omega_view = nullptr;
     s->window = 1;
     if (frames == 1) return PSF_OK;
     const size_t K = s->d.K;
-    hipError_t e = dalloc(&s->dev.hist, K * frames);
-    if (e == hipSuccess) e = dalloc(&s->dev.u_hist, s->dev.ldF * frames);
-    if (e == hipSuccess) e = dalloc(&s->dev.omega_view, K);
+    hipError_t e = salloc(s, &s->dev.hist, K * frames);
+    if (e == hipSuccess) e = salloc(s, &s->dev.u_hist, s->dev.ldF * frames);
+    if (e == hipSuccess) e = salloc(s, &s->dev.omega_view, K);
     if (e != hipSuccess) {
-        dfree(s->dev.hist); dfree(s->dev.u_hist); dfree(s->dev.omega_view);
+        sfree(s, s->dev.hist); sfree(s, s->dev.u_hist); sfree(s, s->dev.omega_view);
         s->dev.hist = nullptr; s->dev.u_hist = nullptr; s->dev.omega_view = nullptr;
         return set_err(s->ctx, e == hipErrorOutOfMemory ? PSF_ERR_NOMEM : PSF_ERR_HIP, "frame window buffers: %s",
                        hipGetErrorString(e));

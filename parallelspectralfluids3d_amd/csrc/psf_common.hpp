@@ -43,11 +43,17 @@ struct DeviceScene {
 
 }  // namespace psf
 
+namespace psf {
+struct DevicePool;
+}
+
 struct psf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
     std::set<psf_scene*> scenes;   // live scenes; destroyed with the context
+    std::set<psf_server*> servers; // live servers; destroyed with the context
+    psf::DevicePool* pool = nullptr;   // recycled device memory of pooled scenes (servers)
 };
 
 struct psf_scene {
@@ -61,6 +67,7 @@ struct psf_scene {
     int gravity_active = 0;
     float lifetime = 60.0f;
     int path = PSF_PATH_PLAIN;
+    int modes_per_wg = 0;        // persistent layout: 0 = one mode per workgroup (up to 256 workgroups)
     uint32_t n_particles = 0;
     uint32_t frame = 0;          // counter of trace passes (respawn draws)
     // frame window: frames computed ahead (filled) and consumed (cursor);
@@ -70,4 +77,13 @@ struct psf_scene {
     uint32_t seed = 0;
     void* host = nullptr;        // host-side copies (psf_scene_host)
     void* persistent = nullptr;  // persistent-path state (PersistentState)
+    bool pooled = false;         // device memory comes from ctx->pool and goes back there
+    bool phi_borrowed = false;   // dev.Phi belongs to someone else (a server's shared basis)
 };
+
+namespace psf {
+// device memory of one scene: hipMalloc / hipFree, or the context's pool for
+// pooled scenes (size classes recycled without hipFree, which synchronises)
+hipError_t scene_alloc(psf_scene* s, void** p, size_t bytes);
+void scene_free(psf_scene* s, void* p);
+}  // namespace psf

@@ -1,0 +1,24 @@
+// Library-internal entry points shared between the C ABI (psf_api.hip) and
+// the scene server (psf_server.hip).
+#pragma once
+#include <string>
+
+#include "psf_common.hpp"
+#include "psf_kernels.hpp"
+
+namespace psf {
+
+struct SceneOptions {
+    int modes_per_wg = 0;          // persistent layout (0 = one mode per workgroup)
+    void* borrowed_phi = nullptr;  // use this Phi (F x K' column-major, K' >= K, leading dim ldF) instead of a copy
+    bool cells = true;             // allocate the cell-lookup debug buffer
+    bool pooled = false;           // take device memory from the context's pool
+};
+
+int scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOptions& opt, psf_scene** out);
+TraceParams scene_trace_params(const psf_scene* s);
+int scene_check_status(psf_scene* s);
+int ctx_set_error(psf_ctx* ctx, int code, const std::string& msg);
+size_t pool_class_bytes(size_t bytes);   // bytes a pooled allocation of this size holds
+
+}  // namespace psf

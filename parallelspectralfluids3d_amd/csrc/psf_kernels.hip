@@ -173,15 +173,14 @@ __global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, 
 // so u_t has the bits k_reconstruct gives for w_t.  The w_t of one mode are
 // uniform across the wave and come through scalar loads.
 template <typename T, int FPT, int TT>
-__global__ __launch_bounds__(256) void k_reconstruct_window(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
-                                                            const double* __restrict__ hist,
-                                                            float* __restrict__ uh)
+__device__ __forceinline__ void window_body(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
+                                            const double* __restrict__ hist, float* __restrict__ uh, uint32_t blk)
 {
     constexpr int CH = TT < 8 ? TT : 8;      // frames combined per LDS round
     __shared__ double part[3][CH][kWave * FPT];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const size_t f0 = static_cast<size_t>(blockIdx.x) * (kWave * FPT) + static_cast<size_t>(lane) * FPT;
+    const size_t f0 = static_cast<size_t>(blk) * (kWave * FPT) + static_cast<size_t>(lane) * FPT;
     double acc[FPT][TT];
 #pragma unroll
     for (int j = 0; j < FPT; ++j)
@@ -238,6 +237,25 @@ __global__ __launch_bounds__(256) void k_reconstruct_window(const T* __restrict_
     }
 }
 
+template <typename T, int FPT, int TT>
+__global__ __launch_bounds__(256) void k_reconstruct_window(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
+                                                            const double* __restrict__ hist,
+                                                            float* __restrict__ uh)
+{
+    window_body<T, FPT, TT>(Phi, ldF, F, K, hist, uh, blockIdx.x);
+}
+
+// several scenes' windows in one launch; scene j owns the blocks from
+// tab[j].block0 (ascending) to the next scene's
+template <typename T, int FPT, int TT>
+__global__ __launch_bounds__(256) void k_reconstruct_window_multi(const WindowScene* __restrict__ tab, int n)
+{
+    int j = 0;
+    while (j + 1 < n && tab[j + 1].block0 <= blockIdx.x) ++j;
+    const WindowScene& w = tab[j];
+    window_body<T, FPT, TT>(static_cast<const T*>(w.Phi), w.ldF, w.F, w.K, w.hist, w.uh, blockIdx.x - w.block0);
+}
+
 // out[k] = hist[k * ld + t]: one column of a window history
 __global__ __launch_bounds__(256) void k_gather_column(const double* __restrict__ hist, int ld, int t, int K,
                                                        double* __restrict__ out)
@@ -280,13 +298,12 @@ __device__ __forceinline__ uint32_t respawn_draw(uint32_t seed, uint32_t step, u
     return static_cast<uint32_t>(z & 1023u);
 }
 
-__global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __restrict__ u, float4* __restrict__ parts,
-                                               const uint32_t* __restrict__ pid, uint32_t n, int32_t* __restrict__ cells)
+// one particle record through one frame; id is its storage slot (cells,
+// caller index)
+__device__ __forceinline__ float4 trace_one(const TraceParams& P, const float* __restrict__ u, float4 p,
+                                            const uint32_t* __restrict__ pid, uint32_t id, int32_t* __restrict__ cells)
 {
 #pragma clang fp contract(off)
-    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
-    if (id >= n) return;
-    float4 p = parts[id];
     const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny), nz = static_cast<int>(P.nz);
     if (p.w > 0.0f) {
         const float sx = (P.aabb_min[0] + P.ext[0]) + p.x;
@@ -393,7 +410,35 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
         p.z = static_cast<float>(-static_cast<double>(P.center_z) + (r1 / 1024.0 - 0.5) * 0.25);
         p.w = static_cast<float>(P.lifetime * (r2 / 1024.0));
     }
-    parts[id] = p;
+    return p;
+}
+
+__global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __restrict__ u, float4* __restrict__ parts,
+                                               const uint32_t* __restrict__ pid, uint32_t n, int32_t* __restrict__ cells)
+{
+    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= n) return;
+    parts[id] = trace_one(P, u, parts[id], pid, id, cells);
+}
+
+// Several scenes, each through nframes frames of its window: particle
+// records do not interact, so each thread carries its particle through the
+// frames with u_t = u_hist + t * ldU and respawn frame P.frame + t, and
+// stores it once.
+__global__ __launch_bounds__(256) void k_trace_window_multi(const TraceScene* __restrict__ tab, int n)
+{
+    int j = 0;
+    while (j + 1 < n && tab[j + 1].block0 <= blockIdx.x) ++j;
+    const TraceScene& ts = tab[j];
+    const uint32_t id = (blockIdx.x - ts.block0) * 256u + threadIdx.x;
+    if (id >= ts.n) return;
+    TraceParams P = ts.P;
+    float4 p = ts.parts[id];
+    for (uint32_t t = 0; t < ts.nframes; ++t) {
+        p = trace_one(P, ts.u_hist + static_cast<size_t>(t) * ts.ldU, p, ts.pid, id, nullptr);
+        P.frame++;
+    }
+    ts.parts[id] = p;
 }
 
 __global__ void k_zero_int(int* p, int n)
@@ -472,6 +517,31 @@ hipError_t launch_reconstruct_window(int prec, int window, const void* Phi, size
     if (ldF % 256 != 0) return hipErrorInvalidValue;
     return prec == PSF_PHI_FP32 ? window_dispatch<float>(window, Phi, ldF, F, K, hist, uh, s)
                                 : window_dispatch<double>(window, Phi, ldF, F, K, hist, uh, s);
+}
+
+hipError_t launch_reconstruct_window_multi(int prec, int window, const WindowScene* d_tab, int n, uint32_t blocks,
+                                           hipStream_t s)
+{
+    if (n <= 0 || blocks == 0) return hipSuccess;
+    if (prec != PSF_PHI_FP32) return hipErrorInvalidValue;
+    switch (window) {
+    case 2: k_reconstruct_window_multi<float, 4, 2><<<blocks, 256, 0, s>>>(d_tab, n); break;
+    case 4: k_reconstruct_window_multi<float, 4, 4><<<blocks, 256, 0, s>>>(d_tab, n); break;
+    case 8: k_reconstruct_window_multi<float, 4, 8><<<blocks, 256, 0, s>>>(d_tab, n); break;
+    case 16: k_reconstruct_window_multi<float, 4, 16><<<blocks, 256, 0, s>>>(d_tab, n); break;
+    case 32: k_reconstruct_window_multi<float, 2, 32><<<blocks, 256, 0, s>>>(d_tab, n); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+uint32_t window_blocks(int window, size_t ldF) { return static_cast<uint32_t>(ldF / (kWave * (window == 32 ? 2 : 4))); }
+
+hipError_t launch_trace_window_multi(const TraceScene* d_tab, int n, uint32_t blocks, hipStream_t s)
+{
+    if (n <= 0 || blocks == 0) return hipSuccess;
+    k_trace_window_multi<<<blocks, 256, 0, s>>>(d_tab, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_gather_column(const double* hist, int ld, int t, int K, double* out, hipStream_t s)

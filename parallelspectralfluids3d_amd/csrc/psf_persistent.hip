@@ -102,30 +102,41 @@ int blocks_per_mode(int prec, int nb)
     return n;
 }
 
-static void layout(int prec, int K, int& G, int& ipw, int& nb, int& nBi, int& bpw)
+// Workgroup g owns modes g, g + G, ...  By default one mode per workgroup up
+// to kMaxCUs workgroups; ipw_req > 0 asks for that many modes per workgroup
+// (fewer workgroups per scene, so more scenes fit in one launch).
+static void layout(int prec, int K, int ipw_req, int& G, int& ipw, int& nb, int& nBi, int& bpw)
 {
-    G = K < kMaxCUs ? K : kMaxCUs;
+    G = ipw_req > 0 ? (K + ipw_req - 1) / ipw_req : (K < kMaxCUs ? K : kMaxCUs);
     ipw = (K + G - 1) / G;
     nb = (K + kTile - 1) / kTile;
     nBi = blocks_per_mode(prec, nb);
     bpw = ipw * nBi;
 }
 
-bool persistent_supported(int prec, int K)
+bool persistent_supported(int prec, int K, int ipw_req)
 {
     if (prec != PSF_C_BF16 && prec != PSF_C_FP16 && prec != PSF_C_BF16X2) return false;
-    if (K < 1 || K > kMaxK) return false;
+    if (K < 1 || K > kMaxK || ipw_req < 0 || ipw_req > kMaxIpw) return false;
     int G, ipw, nb, nBi, bpw;
-    layout(prec, K, G, ipw, nb, nBi, bpw);
-    return ipw <= kMaxIpw && bpw <= kMaxMeta;
+    layout(prec, K, ipw_req, G, ipw, nb, nBi, bpw);
+    return ipw <= kMaxIpw && bpw <= kMaxMeta && G <= 65535;
 }
 
-size_t persistent_c_bytes(int prec, int K)
+size_t persistent_c_bytes(int prec, int K, int ipw_req)
 {
-    if (!persistent_supported(prec, K)) return 0;
+    if (!persistent_supported(prec, K, ipw_req)) return 0;
     int G, ipw, nb, nBi, bpw;
-    layout(prec, K, G, ipw, nb, nBi, bpw);
+    layout(prec, K, ipw_req, G, ipw, nb, nBi, bpw);
     return size_t(G) * bpw * 64 * 16;
+}
+
+int persistent_workgroups(int prec, int K, int ipw_req)
+{
+    if (!persistent_supported(prec, K, ipw_req)) return 0;
+    int G, ipw, nb, nBi, bpw;
+    layout(prec, K, ipw_req, G, ipw, nb, nBi, bpw);
+    return G;
 }
 
 // ---------------------------------------------------------------------------
@@ -171,8 +182,9 @@ __device__ __forceinline__ f32x4 mfma16(const uint4& a, const uint4& b, const f3
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
 }
 
+// The body of one workgroup: local workgroup g of the scene described by a.
 template <int ELEM, int RB>
-__global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
+__device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
 {
     __shared__ double s_w[kMaxK];
     __shared__ double s_y[kMaxK];
@@ -186,7 +198,6 @@ __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int g = blockIdx.x;
     const int K = a.K, Kp = a.Kp;
     const int col = lane & 15, kg = lane >> 4;
 
@@ -396,6 +407,25 @@ __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
     }
 }
 
+// one scene: workgroup g = blockIdx.x
+template <int ELEM, int RB>
+__global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
+{
+    persistent_body<ELEM, RB>(a, static_cast<int>(blockIdx.x));
+}
+
+// several independent scenes in one launch: wg_map[blockIdx.x] = scene << 16 | g.
+// Each scene's workgroups exchange only among themselves (own xbuf, tags,
+// step count), so a scene gives the same bits alone and in a batch.
+template <int ELEM, int RB>
+__global__ __launch_bounds__(256, 1) void k_persistent_multi(const PArgs* __restrict__ table,
+                                                             const uint32_t* __restrict__ wg_map)
+{
+    const uint32_t m = wg_map[blockIdx.x];
+    const PArgs a = table[m >> 16];
+    persistent_body<ELEM, RB>(a, static_cast<int>(m & 0xFFFFu));
+}
+
 // ---------------------------------------------------------------------------
 // smallest register budget that holds a wave's blocks (kRB when they do not fit)
 static const void* pick_kernel(int elem, int bpw)
@@ -413,12 +443,12 @@ static const void* pick_kernel(int elem, int bpw)
 hipError_t persistent_scene_init(psf_scene* s)
 {
     const int prec = s->d.c_precision;
-    if (!persistent_supported(prec, static_cast<int>(s->d.K))) return hipSuccess;
+    if (!persistent_supported(prec, static_cast<int>(s->d.K), s->modes_per_wg)) return hipSuccess;
     PersistentState* p = new (std::nothrow) PersistentState();
     if (!p) return hipErrorOutOfMemory;
     s->persistent = p;
     const int K = static_cast<int>(s->d.K);
-    layout(prec, K, p->G, p->ipw, p->nb, p->nBi, p->bpw);
+    layout(prec, K, s->modes_per_wg, p->G, p->ipw, p->nb, p->nBi, p->bpw);
     p->Kp = p->nb * kTile;
     p->elem = prec == PSF_C_FP16 ? 1 : 0;
     p->col1 = prec == PSF_C_FP16 ? 1.0f / 2048.0f : 1.0f;
@@ -431,9 +461,9 @@ hipError_t persistent_scene_init(psf_scene* s)
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->ctx->device);
     if (e != hipSuccess) return e;
     if (per_cu * cus < p->G) return hipErrorCooperativeLaunchTooLarge;
-    e = hipMalloc(reinterpret_cast<void**>(&p->meta), sizeof(uint32_t) * p->bpw);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->inv_scale), sizeof(double) * p->G * p->ipw);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->xbuf), sizeof(unsigned long long) * 2 * kXStride * K);
+    e = scene_alloc(s, reinterpret_cast<void**>(&p->meta), sizeof(uint32_t) * p->bpw);
+    if (e == hipSuccess) e = scene_alloc(s, reinterpret_cast<void**>(&p->inv_scale), sizeof(double) * p->G * p->ipw);
+    if (e == hipSuccess) e = scene_alloc(s, reinterpret_cast<void**>(&p->xbuf), sizeof(unsigned long long) * 2 * kXStride * K);
     if (e == hipSuccess) e = hipMemset(p->xbuf, 0, sizeof(unsigned long long) * 2 * kXStride * K);
     if (e != hipSuccess) return e;
     // block metadata (identical for every workgroup)
@@ -459,9 +489,9 @@ void persistent_scene_free(psf_scene* s)
 {
     PersistentState* p = st(s);
     if (!p) return;
-    if (p->meta) (void)hipFree(p->meta);
-    if (p->inv_scale) (void)hipFree(p->inv_scale);
-    if (p->xbuf) (void)hipFree(p->xbuf);
+    scene_free(s, p->meta);
+    scene_free(s, p->inv_scale);
+    scene_free(s, p->xbuf);
     delete p;
     s->persistent = nullptr;
 }
@@ -563,11 +593,9 @@ int persistent_upload_c(psf_scene* s, const double* C)
     return PSF_OK;
 }
 
-hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream, double* hist, int hist_ld)
+static PArgs make_pargs(psf_scene* s, int nsteps, double* hist, int hist_ld)
 {
     PersistentState* p = st(s);
-    if (!p) return hipErrorNotSupported;
-    if (nsteps <= 0) return hipSuccess;
     PArgs a{};
     a.blocks = p->blocks;
     a.meta = p->meta;
@@ -590,10 +618,179 @@ hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream,
     a.wscale = p->wscale;
     a.hist = hist;
     a.hist_ld = hist_ld;
+    return a;
+}
+
+hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream, double* hist, int hist_ld)
+{
+    PersistentState* p = st(s);
+    if (!p) return hipErrorNotSupported;
+    if (nsteps <= 0) return hipSuccess;
+    PArgs a = make_pargs(s, nsteps, hist, hist_ld);
     void* args[] = {&a};
     hipError_t e = hipLaunchKernel(p->fn, dim3(p->G), dim3(256), args, 0, stream);
     if (e == hipSuccess) p->tag += static_cast<uint32_t>(nsteps);
     return e;
+}
+
+// ---------------------------------------------------------------------------
+// Several scenes in one launch.
+struct PersistentBatch {
+    int max_scenes = 0, max_wgs = 0;
+    PArgs* d_table = nullptr;
+    uint32_t* d_map = nullptr;
+    PArgs* h_table = nullptr;       // pinned staging
+    uint32_t* h_map = nullptr;
+    int wg_capacity[2] = {0, 0};    // co-resident workgroups of the widest kernel, bf16 / fp16
+};
+
+PersistentBatch* persistent_batch_create(int device, int max_scenes, int max_wgs)
+{
+    PersistentBatch* b = new (std::nothrow) PersistentBatch();
+    if (!b) return nullptr;
+    b->max_scenes = max_scenes;
+    b->max_wgs = max_wgs;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_table), sizeof(PArgs) * max_scenes);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_map), sizeof(uint32_t) * max_wgs);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_table), sizeof(PArgs) * max_scenes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_map), sizeof(uint32_t) * max_wgs, hipHostMallocDefault);
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    for (int el = 0; el < 2 && e == hipSuccess; ++el) {
+        int per_cu = 0;
+        const void* fn = el == 0 ? reinterpret_cast<const void*>(&k_persistent_multi<0, kRB>)
+                                 : reinterpret_cast<const void*>(&k_persistent_multi<1, kRB>);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0);
+        b->wg_capacity[el] = per_cu * cus;
+    }
+    if (e != hipSuccess) {
+        persistent_batch_destroy(b);
+        return nullptr;
+    }
+    return b;
+}
+
+void persistent_batch_destroy(PersistentBatch* b)
+{
+    if (!b) return;
+    if (b->d_table) (void)hipFree(b->d_table);
+    if (b->d_map) (void)hipFree(b->d_map);
+    if (b->h_table) (void)hipHostFree(b->h_table);
+    if (b->h_map) (void)hipHostFree(b->h_map);
+    delete b;
+}
+
+int persistent_batch_wg_capacity(const PersistentBatch* b, int c_precision)
+{
+    const int cap = b->wg_capacity[c_precision == PSF_C_FP16 ? 1 : 0];
+    return cap < b->max_wgs ? cap : b->max_wgs;
+}
+
+hipError_t launch_persistent_batch(PersistentBatch* b, psf_scene* const* sc, const int* nsteps, double* const* hist,
+                                   const int* hist_ld, int n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > b->max_scenes) return hipErrorInvalidValue;
+    int elem = -1, per_max = 0, wgs = 0;
+    for (int j = 0; j < n; ++j) {
+        PersistentState* p = st(sc[j]);
+        if (!p) return hipErrorNotSupported;
+        if (elem >= 0 && p->elem != elem) return hipErrorInvalidValue;
+        elem = p->elem;
+        per_max = std::max(per_max, (p->bpw + 3) / 4);
+        if (nsteps[j] > 0) wgs += p->G;
+    }
+    if (wgs == 0) return hipSuccess;
+    if (wgs > b->max_wgs || wgs > b->wg_capacity[elem]) return hipErrorCooperativeLaunchTooLarge;
+    int w = 0;
+    for (int j = 0; j < n; ++j) {
+        if (nsteps[j] <= 0) continue;
+        PersistentState* p = st(sc[j]);
+        b->h_table[j] = make_pargs(sc[j], nsteps[j], hist ? hist[j] : nullptr, hist_ld ? hist_ld[j] : 0);
+        for (int g = 0; g < p->G; ++g) b->h_map[w++] = static_cast<uint32_t>(j) << 16 | static_cast<uint32_t>(g);
+    }
+    hipError_t e = hipMemcpyAsync(b->d_table, b->h_table, sizeof(PArgs) * n, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->d_map, b->h_map, sizeof(uint32_t) * wgs, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return e;
+    const void* fn;
+    if (elem == 0)
+        fn = per_max <= 4 ? reinterpret_cast<const void*>(&k_persistent_multi<0, 4>)
+           : per_max <= 12 ? reinterpret_cast<const void*>(&k_persistent_multi<0, 12>)
+                           : reinterpret_cast<const void*>(&k_persistent_multi<0, kRB>);
+    else
+        fn = per_max <= 4 ? reinterpret_cast<const void*>(&k_persistent_multi<1, 4>)
+           : per_max <= 12 ? reinterpret_cast<const void*>(&k_persistent_multi<1, 12>)
+                           : reinterpret_cast<const void*>(&k_persistent_multi<1, kRB>);
+    const PArgs* tab = b->d_table;
+    const uint32_t* map = b->d_map;
+    void* args[] = {&tab, &map};
+    e = hipLaunchKernel(fn, dim3(wgs), dim3(256), args, 0, stream);
+    if (e != hipSuccess) return e;
+    for (int j = 0; j < n; ++j)
+        if (nsteps[j] > 0) st(sc[j])->tag += static_cast<uint32_t>(nsteps[j]);
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Packing on the device: C_i(a, b) for a, b, i < K taken from a larger FP64
+// tensor Cf of K0 modes (C_i(a, b) at Cf[(i * K0 + b) * K0 + a]); the first K
+// modes of a basis sorted by eigenvalue give the K-mode tensor.  Same values
+// and rounding as persistent_upload_c for BF16 and BF16 hi+lo.
+__global__ __launch_bounds__(256) void k_pack_c(const double* __restrict__ Cf, int K0, int K, int G, int ipw, int bpw,
+                                                int hilo, const uint32_t* __restrict__ meta, uint4* __restrict__ blocks,
+                                                double* __restrict__ inv_scale)
+{
+    const size_t t = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (t < static_cast<size_t>(G) * ipw) inv_scale[t] = 1.0;
+    const size_t total = static_cast<size_t>(G) * bpw * 64;
+    if (t >= total) return;
+    const int lane = static_cast<int>(t & 63);
+    const size_t gb = t >> 6;
+    const int b = static_cast<int>(gb % bpw), g = static_cast<int>(gb / bpw);
+    const uint32_t m = meta[b];
+    const int tm = static_cast<int>(m >> 24), pr = static_cast<int>((m >> 16) & 255u);
+    const int q1 = static_cast<int>((m >> 8) & 255u), q2 = static_cast<int>(m & 255u);
+    const int i = g + G * tm;
+    const int r = lane & 15, kgp = lane >> 4;
+    const int row = pr * kTile + r;
+    auto T = [&](int a, int c) -> double {
+        if (i >= K || a >= K || c >= K) return 0.0;
+        const double* Ci = Cf + static_cast<size_t>(i) * K0 * K0;
+        double v = 0.5 * (Ci[static_cast<size_t>(c) * K0 + a] + Ci[static_cast<size_t>(a) * K0 + c]);
+        if (a / kTile < c / kTile) v *= 2.0;
+        return v;
+    };
+    uint16_t out[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = 8 * kgp + e;
+        if (hilo) {
+            const double v = T(row, q1 * kTile + (k & 15));
+            const uint16_t hi = f32_to_bf16_rne(static_cast<float>(v));
+            out[e] = k < 16 ? hi : f32_to_bf16_rne(static_cast<float>(v - static_cast<double>(bf16_to_f32(hi))));
+        } else {
+            const int q = k < 16 ? q1 : q2;
+            out[e] = (k >= 16 && q2 == q1) ? uint16_t(0) : f32_to_bf16_rne(static_cast<float>(T(row, q * kTile + (k & 15))));
+        }
+    }
+    uint4 v;
+    v.x = uint32_t(out[0]) | uint32_t(out[1]) << 16;
+    v.y = uint32_t(out[2]) | uint32_t(out[3]) << 16;
+    v.z = uint32_t(out[4]) | uint32_t(out[5]) << 16;
+    v.w = uint32_t(out[6]) | uint32_t(out[7]) << 16;
+    blocks[t] = v;
+}
+
+hipError_t launch_pack_c(psf_scene* s, const double* Cf, int K0, hipStream_t stream)
+{
+    PersistentState* p = st(s);
+    const int prec = s->d.c_precision;
+    if (!p || (prec != PSF_C_BF16 && prec != PSF_C_BF16X2) || K0 < static_cast<int>(s->d.K)) return hipErrorInvalidValue;
+    const size_t total = std::max(static_cast<size_t>(p->G) * p->bpw * 64, static_cast<size_t>(p->G) * p->ipw);
+    const unsigned grid = static_cast<unsigned>((total + 255) / 256);
+    k_pack_c<<<grid, 256, 0, stream>>>(Cf, K0, static_cast<int>(s->d.K), p->G, p->ipw, p->bpw,
+                                       prec == PSF_C_BF16X2 ? 1 : 0, p->meta, p->blocks, p->inv_scale);
+    return hipGetLastError();
 }
 
 }  // namespace psf

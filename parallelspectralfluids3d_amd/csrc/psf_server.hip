@@ -1,0 +1,440 @@
+// Scene server: continuous batching of independent scenes (C ABI in psf.h).
+//
+// Every round advances all active scenes by up to `window` integrate() frames
+// (psfsolver.cpp:21-234) with three launches on the context stream:
+//   1. one persistent launch holding every scene's workgroups; each scene
+//      runs its own number of reduced steps and writes its omega history;
+//   2. one window reconstruction over all scenes' Phi (FP64 sums);
+//   3. one particle launch carrying every scene's particles through their
+//      frames of the round.
+// Scenes never exchange data, and each scene's arithmetic depends only on its
+// own inputs and layout, so a request gives the same bits alone and in a mix.
+// Admission is FIFO at round start, bounded by active slots, the workgroups
+// the persistent launch can hold resident, and a device-memory budget.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "psf_internal.hpp"
+#include "psf_kernels.hpp"
+#include "psf_persistent.hpp"
+
+using namespace psf;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+constexpr uint32_t kMaxActive = 64;
+
+struct Basis {
+    psf_scene_desc desc{};       // grid; K0 = desc.K
+    uint32_t F = 0;
+    size_t ldF = 0;
+    float* Phi = nullptr;        // ldF x K0, FP32, rows F..ldF-1 zero
+    double* C = nullptr;         // K0^3, FP64
+    std::vector<double> lambda, gravity;
+};
+
+enum ReqState { kQueued = 0, kActive = 1, kDone = 2, kFailed = 3 };
+
+struct Request {
+    uint64_t id = 0;
+    uint32_t basis = 0, K = 0, nframes = 0, done = 0, n = 0;
+    double dt = 0.0, nu = 0.0;
+    int grav = 0;
+    float lifetime = 0.0f;
+    std::vector<double> omega;   // input, then result
+    std::vector<float> parts;    // input, then result
+    psf_scene* scene = nullptr;
+    uint64_t footprint = 0;
+    int wgs = 0;
+    int state = kQueued;
+    std::string error;
+    Clock::time_point t_submit, t_done;
+};
+
+}  // namespace
+
+struct psf_server {
+    psf_ctx* ctx = nullptr;
+    psf_server_config cfg{};
+    std::vector<Basis> bases;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    std::deque<Request*> queue;
+    std::vector<Request*> active;
+    uint64_t next_id = 1;
+    PersistentBatch* batch = nullptr;
+    int wg_capacity = 0, wgs_in_use = 0;
+    uint64_t mem_in_use = 0, mem_peak = 0;
+    WindowScene* d_wtab = nullptr;
+    WindowScene* h_wtab = nullptr;
+    TraceScene* d_ttab = nullptr;
+    TraceScene* h_ttab = nullptr;
+    psf_server_stats st{};
+};
+
+namespace {
+
+int fail(psf_ctx* ctx, int code, const std::string& msg) { return ctx_set_error(ctx, code, msg); }
+
+#define SRV_HIPCHK(ctx, expr)                                                                                \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess) return fail((ctx), PSF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device bytes of a request, as the pool holds them.
+uint64_t request_bytes(const psf_server* sv, const Basis& b, uint32_t K, uint32_t n)
+{
+    const int ipw = static_cast<int>(sv->cfg.modes_per_wg);
+    const size_t cb = persistent_c_bytes(PSF_C_BF16X2, static_cast<int>(K), ipw);
+    const int G = persistent_workgroups(PSF_C_BF16X2, static_cast<int>(K), ipw);
+    const size_t bpw = G ? cb / (size_t(G) * 1024) : 0;
+    const size_t Wn = sv->cfg.window;
+    const size_t sizes[] = {
+        K * 8ull, K * 8ull, K * 8ull, K * 8ull, K * 8ull, 64 * 4ull,   // omega lambda gravity visc y status
+        cb, b.F * 4ull, n * 16ull, n * 4ull,                            // C u parts pid
+        bpw * 4, size_t(G) * ((K + G - 1) / G) * 8, 2ull * 16 * K * 8,   // meta inv_scale xbuf
+        K * Wn * 8, b.ldF * Wn * 4, K * 8ull,                           // hist u_hist omega_view
+    };
+    uint64_t t = 0;
+    for (size_t x : sizes) t += pool_class_bytes(x);
+    return t;
+}
+
+int admit(psf_server* sv, Request* r)
+{
+    const Basis& b = sv->bases[r->basis];
+    psf_scene_desc d = b.desc;
+    d.K = r->K;
+    d.c_precision = PSF_C_BF16X2;
+    d.phi_precision = PSF_PHI_FP32;
+    d.max_particles = r->n;
+    SceneOptions o;
+    o.modes_per_wg = static_cast<int>(sv->cfg.modes_per_wg);
+    o.borrowed_phi = b.Phi;
+    o.cells = false;
+    o.pooled = true;
+    psf_scene* s = nullptr;
+    int rc = scene_create_ex(sv->ctx, &d, o, &s);
+    if (rc) return rc;
+    r->scene = s;
+    if ((rc = psf_scene_upload_basis(s, nullptr, b.lambda.data(), b.gravity.data()))) return rc;
+    SRV_HIPCHK(sv->ctx, launch_pack_c(s, b.C, static_cast<int>(b.desc.K), sv->ctx->stream));
+    s->has_C = true;
+    if ((rc = psf_set_params(s, r->dt, r->nu, r->grav, r->lifetime))) return rc;
+    if ((rc = psf_set_omega(s, r->omega.data()))) return rc;
+    if ((rc = psf_set_particles(s, r->parts.data(), r->n))) return rc;
+    if ((rc = psf_set_frame_window(s, sv->cfg.window))) return rc;
+    return PSF_OK;
+}
+
+void release(psf_server* sv, Request* r)
+{
+    if (r->scene) psf_scene_destroy(r->scene);
+    r->scene = nullptr;
+    sv->wgs_in_use -= r->wgs;
+    sv->mem_in_use -= r->footprint;
+}
+
+void admit_queued(psf_server* sv)
+{
+    while (!sv->queue.empty() && sv->active.size() < sv->cfg.max_active) {
+        Request* r = sv->queue.front();
+        if (sv->wgs_in_use + r->wgs > sv->wg_capacity) break;
+        if (sv->mem_in_use + r->footprint > sv->cfg.mem_budget) break;
+        sv->queue.pop_front();
+        sv->wgs_in_use += r->wgs;
+        sv->mem_in_use += r->footprint;
+        sv->mem_peak = std::max(sv->mem_peak, sv->mem_in_use);
+        if (admit(sv, r) != PSF_OK) {
+            r->error = sv->ctx->err;
+            r->state = kFailed;
+            r->t_done = Clock::now();
+            release(sv, r);
+            sv->st.failed++;
+            continue;
+        }
+        r->state = kActive;
+        sv->active.push_back(r);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** out)
+{
+    if (!ctx || !out) return PSF_ERR_ARG;
+    *out = nullptr;
+    psf_server_config c = cfg ? *cfg : psf_server_config{};
+    if (c.window == 0) c.window = 32;
+    if (c.max_active == 0) c.max_active = 32;
+    if (c.mem_budget == 0) c.mem_budget = 64ull << 30;
+    if (c.modes_per_wg == 0) c.modes_per_wg = 4;
+    if (c.window != 2 && c.window != 4 && c.window != 8 && c.window != 16 && c.window != 32)
+        return fail(ctx, PSF_ERR_ARG, "server window must be 2, 4, 8, 16 or 32");
+    if (c.max_active > kMaxActive) return fail(ctx, PSF_ERR_ARG, "max_active must be at most 64");
+    if (c.modes_per_wg > 8) return fail(ctx, PSF_ERR_ARG, "modes_per_wg must be at most 8");
+    SRV_HIPCHK(ctx, hipSetDevice(ctx->device));
+    psf_server* sv = new (std::nothrow) psf_server();
+    if (!sv) return PSF_ERR_NOMEM;
+    sv->ctx = ctx;
+    sv->cfg = c;
+    sv->batch = persistent_batch_create(ctx->device, static_cast<int>(kMaxActive), 8192);
+    hipError_t e = sv->batch ? hipSuccess : hipErrorOutOfMemory;
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sv->d_wtab), sizeof(WindowScene) * kMaxActive);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sv->d_ttab), sizeof(TraceScene) * kMaxActive);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&sv->h_wtab), sizeof(WindowScene) * kMaxActive, hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&sv->h_ttab), sizeof(TraceScene) * kMaxActive, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        ctx->servers.insert(sv);
+        psf_server_destroy(sv);
+        return fail(ctx, PSF_ERR_HIP, std::string("server set-up: ") + hipGetErrorString(e));
+    }
+    sv->wg_capacity = persistent_batch_wg_capacity(sv->batch, PSF_C_BF16X2);
+    ctx->servers.insert(sv);
+    *out = sv;
+    return PSF_OK;
+}
+
+int psf_server_destroy(psf_server* sv)
+{
+    if (!sv) return PSF_ERR_ARG;
+    psf_ctx* ctx = sv->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : sv->reqs)
+        if (kv.second->scene) psf_scene_destroy(kv.second->scene);
+    for (Basis& b : sv->bases) {
+        if (b.Phi) (void)hipFree(b.Phi);
+        if (b.C) (void)hipFree(b.C);
+    }
+    persistent_batch_destroy(sv->batch);
+    if (sv->d_wtab) (void)hipFree(sv->d_wtab);
+    if (sv->d_ttab) (void)hipFree(sv->d_ttab);
+    if (sv->h_wtab) (void)hipHostFree(sv->h_wtab);
+    if (sv->h_ttab) (void)hipHostFree(sv->h_ttab);
+    ctx->servers.erase(sv);
+    delete sv;
+    return PSF_OK;
+}
+
+int psf_server_add_basis(psf_server* sv, const psf_scene_desc* desc, const double* Phi, const double* lambda,
+                         const double* gravity, const double* C, uint32_t* basis_id)
+{
+    if (!sv || !desc || !Phi || !lambda || !C || !basis_id) return fail(sv ? sv->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    psf_ctx* ctx = sv->ctx;
+    psf_scene_desc d = *desc;
+    d.c_precision = PSF_C_BF16X2;
+    d.phi_precision = PSF_PHI_FP32;
+    uint64_t probe = 0;
+    int rc = psf_scene_footprint(&d, &probe);   // validates the grid and K0
+    if (rc) return fail(ctx, rc, "basis descriptor is invalid");
+    if (d.K > 1024) return fail(ctx, PSF_ERR_UNSUPPORTED, "basis K0 must be at most 1024");
+    SRV_HIPCHK(ctx, hipSetDevice(ctx->device));
+    Basis b;
+    b.desc = d;
+    b.F = (d.nx + 1) * d.ny * d.nz + (d.ny + 1) * d.nx * d.nz + (d.nz + 1) * d.nx * d.ny;
+    b.ldF = (size_t(b.F) + 255) / 256 * 256;
+    const size_t K0 = d.K;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&b.Phi), b.ldF * K0 * 4);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b.C), K0 * K0 * K0 * 8);
+    if (e == hipSuccess) e = hipMemset(b.Phi, 0, b.ldF * K0 * 4);
+    if (e == hipSuccess) {
+        std::vector<float> col(b.F);
+        for (size_t k = 0; k < K0 && e == hipSuccess; ++k) {
+            for (uint32_t f = 0; f < b.F; ++f) col[f] = static_cast<float>(Phi[k * b.F + f]);
+            e = hipMemcpy(b.Phi + k * b.ldF, col.data(), size_t(b.F) * 4, hipMemcpyHostToDevice);
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpy(b.C, C, K0 * K0 * K0 * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (b.Phi) (void)hipFree(b.Phi);
+        if (b.C) (void)hipFree(b.C);
+        return fail(ctx, e == hipErrorOutOfMemory ? PSF_ERR_NOMEM : PSF_ERR_HIP,
+                    std::string("basis upload: ") + hipGetErrorString(e));
+    }
+    b.lambda.assign(lambda, lambda + K0);
+    if (gravity)
+        b.gravity.assign(gravity, gravity + K0);
+    else
+        b.gravity.assign(K0, 0.0);
+    sv->bases.push_back(std::move(b));
+    *basis_id = static_cast<uint32_t>(sv->bases.size() - 1);
+    return PSF_OK;
+}
+
+int psf_server_request_footprint(psf_server* sv, uint32_t basis_id, uint32_t K, uint32_t n, uint64_t* bytes)
+{
+    if (!sv || !bytes) return PSF_ERR_ARG;
+    if (basis_id >= sv->bases.size()) return fail(sv->ctx, PSF_ERR_ARG, "unknown basis");
+    const Basis& b = sv->bases[basis_id];
+    if (K == 0 || K > b.desc.K) return fail(sv->ctx, PSF_ERR_ARG, "K must be in [1, K0]");
+    *bytes = request_bytes(sv, b, K, n);
+    return PSF_OK;
+}
+
+int psf_server_submit(psf_server* sv, uint32_t basis_id, uint32_t K, uint32_t nframes, double dt, double viscosity,
+                      int gravity_active, float lifetime, const double* omega, const float* particles, uint32_t n,
+                      uint64_t* request_id)
+{
+    if (!sv || !omega || (!particles && n) || !request_id) return fail(sv ? sv->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    psf_ctx* ctx = sv->ctx;
+    if (basis_id >= sv->bases.size()) return fail(ctx, PSF_ERR_ARG, "unknown basis");
+    const Basis& b = sv->bases[basis_id];
+    if (K == 0 || K > b.desc.K) return fail(ctx, PSF_ERR_ARG, "K must be in [1, K0]");
+    if (nframes == 0) return fail(ctx, PSF_ERR_ARG, "nframes must be positive");
+    if (!std::isfinite(dt) || !std::isfinite(viscosity)) return fail(ctx, PSF_ERR_ARG, "non-finite parameter");
+    const int wgs = persistent_workgroups(PSF_C_BF16X2, static_cast<int>(K), static_cast<int>(sv->cfg.modes_per_wg));
+    const uint64_t fp = request_bytes(sv, b, K, n);
+    if (wgs > sv->wg_capacity) return fail(ctx, PSF_ERR_UNSUPPORTED, "request needs more workgroups than one launch holds");
+    if (fp > sv->cfg.mem_budget) return fail(ctx, PSF_ERR_NOMEM, "request footprint exceeds the server's memory budget");
+    auto r = std::make_unique<Request>();
+    r->id = sv->next_id++;
+    r->basis = basis_id;
+    r->K = K;
+    r->nframes = nframes;
+    r->n = n;
+    r->dt = dt;
+    r->nu = viscosity;
+    r->grav = gravity_active ? 1 : 0;
+    r->lifetime = lifetime;
+    r->omega.assign(omega, omega + K);
+    r->parts.assign(particles, particles + size_t(n) * 4);
+    r->footprint = fp;
+    r->wgs = wgs;
+    r->t_submit = Clock::now();
+    Request* raw = r.get();
+    sv->reqs.emplace(raw->id, std::move(r));
+    sv->queue.push_back(raw);
+    sv->st.submitted++;
+    *request_id = raw->id;
+    return PSF_OK;
+}
+
+int psf_server_round(psf_server* sv, uint32_t* completed)
+{
+    if (!sv) return PSF_ERR_ARG;
+    psf_ctx* ctx = sv->ctx;
+    const Clock::time_point t0 = Clock::now();
+    if (completed) *completed = 0;
+    SRV_HIPCHK(ctx, hipSetDevice(ctx->device));
+    admit_queued(sv);
+    const int n = static_cast<int>(sv->active.size());
+    if (n == 0) return PSF_OK;
+    const int W = static_cast<int>(sv->cfg.window);
+    hipStream_t stream = ctx->stream;
+    std::vector<psf_scene*> sc(n);
+    std::vector<int> nsteps(n), ld(n, W);
+    std::vector<double*> hist(n);
+    uint32_t wblocks = 0, tblocks = 0;
+    for (int j = 0; j < n; ++j) {
+        Request* r = sv->active[j];
+        psf_scene* s = r->scene;
+        sc[j] = s;
+        nsteps[j] = static_cast<int>(std::min<uint32_t>(static_cast<uint32_t>(W), r->nframes - r->done));
+        hist[j] = s->dev.hist;
+        const Basis& b = sv->bases[r->basis];
+        sv->h_wtab[j] = WindowScene{b.Phi, s->dev.ldF, s->F, static_cast<int>(s->d.K), s->dev.hist, s->dev.u_hist, wblocks};
+        wblocks += window_blocks(W, s->dev.ldF);
+        TraceScene t{};
+        t.P = scene_trace_params(s);
+        t.u_hist = s->dev.u_hist;
+        t.ldU = s->dev.ldF;
+        t.parts = s->dev.parts;
+        t.pid = s->dev.pid;
+        t.n = s->n_particles;
+        t.nframes = static_cast<uint32_t>(nsteps[j]);
+        t.block0 = tblocks;
+        sv->h_ttab[j] = t;
+        tblocks += (s->n_particles + 255) / 256;
+    }
+    SRV_HIPCHK(ctx, launch_persistent_batch(sv->batch, sc.data(), nsteps.data(), hist.data(), ld.data(), n, stream));
+    SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_wtab, sv->h_wtab, sizeof(WindowScene) * n, hipMemcpyHostToDevice, stream));
+    SRV_HIPCHK(ctx, launch_reconstruct_window_multi(PSF_PHI_FP32, W, sv->d_wtab, n, wblocks, stream));
+    SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_ttab, sv->h_ttab, sizeof(TraceScene) * n, hipMemcpyHostToDevice, stream));
+    SRV_HIPCHK(ctx, launch_trace_window_multi(sv->d_ttab, n, tblocks, stream));
+    SRV_HIPCHK(ctx, hipStreamSynchronize(stream));
+    sv->st.rounds++;
+    std::vector<Request*> keep;
+    keep.reserve(n);
+    for (int j = 0; j < n; ++j) {
+        Request* r = sv->active[j];
+        r->scene->frame += static_cast<uint32_t>(nsteps[j]);
+        r->done += static_cast<uint32_t>(nsteps[j]);
+        sv->st.scene_frames += static_cast<uint64_t>(nsteps[j]);
+        if (r->done < r->nframes) {
+            keep.push_back(r);
+            continue;
+        }
+        int rc = scene_check_status(r->scene);
+        if (rc == PSF_OK) rc = psf_get_omega(r->scene, r->omega.data());
+        if (rc == PSF_OK) rc = psf_get_particles(r->scene, r->parts.data(), r->n);
+        r->t_done = Clock::now();
+        if (rc == PSF_OK) {
+            r->state = kDone;
+            sv->st.completed++;
+            if (completed) (*completed)++;
+        } else {
+            r->state = kFailed;
+            r->error = ctx->err;
+            sv->st.failed++;
+        }
+        release(sv, r);
+    }
+    sv->active.swap(keep);
+    sv->st.busy_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    return PSF_OK;
+}
+
+int psf_server_done(psf_server* sv, uint64_t id)
+{
+    if (!sv) return PSF_ERR_ARG;
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return fail(sv->ctx, PSF_ERR_ARG, "unknown request");
+    return it->second->state == kDone || it->second->state == kFailed ? 1 : 0;
+}
+
+int psf_server_result(psf_server* sv, uint64_t id, double* omega, float* particles, double* latency_ms)
+{
+    if (!sv) return PSF_ERR_ARG;
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return fail(sv->ctx, PSF_ERR_ARG, "unknown request");
+    Request* r = it->second.get();
+    if (r->state == kQueued || r->state == kActive) return fail(sv->ctx, PSF_ERR_STATE, "request has not finished");
+    if (r->state == kFailed) {
+        const std::string msg = "request failed: " + r->error;
+        sv->reqs.erase(it);
+        return fail(sv->ctx, PSF_ERR_HIP, msg);
+    }
+    if (omega) std::memcpy(omega, r->omega.data(), r->omega.size() * 8);
+    if (particles) std::memcpy(particles, r->parts.data(), r->parts.size() * 4);
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    sv->reqs.erase(it);
+    return PSF_OK;
+}
+
+int psf_server_get_stats(psf_server* sv, psf_server_stats* out)
+{
+    if (!sv || !out) return PSF_ERR_ARG;
+    psf_server_stats s = sv->st;
+    s.active = static_cast<uint32_t>(sv->active.size());
+    s.queued = static_cast<uint32_t>(sv->queue.size());
+    s.mem_in_use = sv->mem_in_use;
+    s.mem_peak = sv->mem_peak;
+    *out = s;
+    return PSF_OK;
+}
+
+}  // extern "C"

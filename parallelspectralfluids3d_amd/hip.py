@@ -197,3 +197,96 @@ class Scene:
         ms = ct.c_double()
         self._c(self.ctx.lib.psf_time_phase(self.h, int(phase), int(count), ct.byref(ms)), "time_phase")
         return float(ms.value)
+
+
+class Server:
+    """Continuous batching of independent scenes on one context (psf_server_*).
+
+    add_basis() registers one grid with K0 modes (Phi, lambda, gravity, C);
+    submit() queues a request on the first K modes of a basis; round()
+    admits queued requests and advances every active scene by up to `window`
+    frames; result() returns omega, particles and the latency of a finished
+    request."""
+
+    def __init__(self, ctx: Context, window=32, max_active=32, mem_budget=0, modes_per_wg=4):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        cfg = L.ServerConfig()
+        cfg.window, cfg.max_active, cfg.mem_budget, cfg.modes_per_wg = window, max_active, mem_budget, modes_per_wg
+        h = ct.c_void_p()
+        ctx.check(self.lib.psf_server_create(ctx.h, ct.byref(cfg), ct.byref(h)), "psf_server_create")
+        self.h = h
+        self.window = window
+        self._n = {}
+        self._K = {}
+        ctx.scenes.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            if self.ctx.h:
+                self.lib.psf_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_basis(self, grid: Grid, resolution, Phi_cm, lam, gravity, C, aabb_min=(-0.5, -0.5, -0.5),
+                  aabb_max=(0.5, 0.5, 0.5)) -> int:
+        K0 = int(np.asarray(lam).size)
+        d = make_desc(grid, resolution, K0, L.C_BF16X2, L.PHI_FP32, 0, aabb_min, aabb_max)
+        Phi = np.ascontiguousarray(Phi_cm, np.float64)
+        lam = np.ascontiguousarray(lam, np.float64)
+        g = np.ascontiguousarray(gravity if gravity is not None else np.zeros(K0), np.float64)
+        C = np.ascontiguousarray(C, np.float64)
+        assert Phi.shape == (K0, grid.nF) and C.size == K0 ** 3
+        bid = ct.c_uint32()
+        self.ctx.check(self.lib.psf_server_add_basis(self.h, ct.byref(d), _ptr(Phi), _ptr(lam), _ptr(g), _ptr(C),
+                                                     ct.byref(bid)), "psf_server_add_basis")
+        return int(bid.value)
+
+    def footprint(self, basis, K, n_particles) -> int:
+        out = ct.c_uint64()
+        self.ctx.check(self.lib.psf_server_request_footprint(self.h, basis, K, n_particles, ct.byref(out)),
+                       "psf_server_request_footprint")
+        return int(out.value)
+
+    def submit(self, basis, omega, particles, nframes, dt=0.01, viscosity=0.0, gravity_active=False,
+               lifetime=60.0) -> int:
+        w = np.ascontiguousarray(omega, np.float64)
+        p = np.ascontiguousarray(particles, np.float32).reshape(-1, 4)
+        rid = ct.c_uint64()
+        self.ctx.check(self.lib.psf_server_submit(self.h, basis, w.size, int(nframes), float(dt), float(viscosity),
+                                                  1 if gravity_active else 0, float(lifetime), _ptr(w), _ptr(p),
+                                                  p.shape[0], ct.byref(rid)), "psf_server_submit")
+        self._n[rid.value] = p.shape[0]
+        self._K[rid.value] = w.size
+        return int(rid.value)
+
+    def round(self) -> int:
+        c = ct.c_uint32()
+        self.ctx.check(self.lib.psf_server_round(self.h, ct.byref(c)), "psf_server_round")
+        return int(c.value)
+
+    def done(self, rid) -> bool:
+        rc = self.lib.psf_server_done(self.h, ct.c_uint64(rid))
+        if rc < 0:
+            self.ctx.check(rc, "psf_server_done")
+        return rc == 1
+
+    def result(self, rid):
+        """(omega, particles, latency_ms) of a finished request."""
+        w = np.empty(self._K[rid], np.float64)
+        p = np.empty((self._n[rid], 4), np.float32)
+        lat = ct.c_double()
+        self.ctx.check(self.lib.psf_server_result(self.h, ct.c_uint64(rid), _ptr(w), _ptr(p), ct.byref(lat)),
+                       "psf_server_result")
+        del self._n[rid], self._K[rid]
+        return w, p, float(lat.value)
+
+    def stats(self) -> dict:
+        st = L.ServerStats()
+        self.ctx.check(self.lib.psf_server_get_stats(self.h, ct.byref(st)), "psf_server_get_stats")
+        return {name: getattr(st, name) for name, _ in L.ServerStats._fields_}
