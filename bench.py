@@ -131,6 +131,118 @@ def cpu_baseline(sc, budget_s=12.0):
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model}
 
 
+SERVE_GRIDS = (16, 24, 32, 48)
+SERVE_K0 = 192
+
+
+def serving_workload(n_requests, rate, seed=0, particles=1 << 14):
+    """The serving stream: per request a closed box from {16,24,32,48}^3, K
+    uniform in [16, 192], a frame count uniform in [100, 2000], omega ~ N(0,1)
+    at unit energy and `particles` uniform in the box; Poisson arrivals at
+    `rate` per second.  All draws from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    reqs = []
+    for j in range(n_requests):
+        n = int(rng.choice(SERVE_GRIDS))
+        K = int(rng.integers(16, SERVE_K0 + 1))
+        frames = int(rng.integers(100, 2001))
+        w = rng.standard_normal(K)
+        w /= np.linalg.norm(w)
+        p = np.empty((particles, 4), np.float32)
+        p[:, :3] = rng.random((particles, 3)) - 0.5
+        p[:, 3] = np.inf
+        reqs.append(dict(arrival=float(t[j]), grid=n, K=K, frames=frames, omega=w, parts=p))
+    return reqs
+
+
+def serving_bases():
+    """One K0 = 192 basis per grid: analytic box modes, their C and forcing."""
+    from parallelspectralfluids3d_amd import advection, basis
+    from parallelspectralfluids3d_amd.grid import Grid, box_mask
+    out = {}
+    for n in SERVE_GRIDS:
+        g = Grid(n, n, n)
+        h = 1.0 / n
+        Phi, lam, tab = basis.analytic_box_modes(g, SERVE_K0)
+        V = (basis.curl_operator(g, h) @ Phi.T).T
+        C = advection.build_advection_box_analytic(g, h, Phi, lam, tab, V)
+        gv = basis.project(Phi, basis.gravity_field(g, box_mask(g)))
+        out[n] = dict(grid=g, h=h, Phi=Phi, lam=lam, C=C, g=gv)
+    return out
+
+
+def run_stream(ctx, bases, reqs, max_active, window=32, saturate=False):
+    """Feeds the requests to a server at their arrival times (all at once when
+    saturate) and runs rounds until every request has finished.  Returns the
+    throughput over the makespan and the latency of each request from its
+    arrival."""
+    from parallelspectralfluids3d_amd import hip
+    sv = hip.Server(ctx, window=window, max_active=max_active)
+    bid = {n: sv.add_basis(b["grid"], b["h"], b["Phi"], b["lam"], b["g"], b["C"]) for n, b in bases.items()}
+    pending = sorted(range(len(reqs)), key=lambda j: reqs[j]["arrival"])
+    outstanding = {}
+    lat = np.zeros(len(reqs))
+    frames_done = 0
+    nxt = 0
+    t0 = time.perf_counter()
+    while nxt < len(pending) or outstanding:
+        now = time.perf_counter() - t0
+        while nxt < len(pending) and (saturate or reqs[pending[nxt]]["arrival"] <= now):
+            j = pending[nxt]
+            r = reqs[j]
+            rid = sv.submit(bid[r["grid"]], r["omega"], r["parts"], r["frames"])
+            outstanding[rid] = (j, time.perf_counter() - t0)
+            nxt += 1
+        st = sv.stats()
+        if st["active"] == 0 and st["queued"] == 0:
+            if nxt < len(pending):
+                time.sleep(max(0.0, reqs[pending[nxt]]["arrival"] - (time.perf_counter() - t0)))
+            continue
+        if sv.round():
+            for rid in [k for k in outstanding if sv.done(k)]:
+                j, t_sub = outstanding.pop(rid)
+                _, _, ms = sv.result(rid)
+                arrival = 0.0 if saturate else reqs[j]["arrival"]
+                lat[j] = t_sub + ms * 1e-3 - arrival
+                frames_done += reqs[j]["frames"]
+    makespan = time.perf_counter() - t0
+    st = sv.stats()
+    sv.close()
+    first = 0.0 if saturate else min(r["arrival"] for r in reqs)
+    span = makespan - first
+    return dict(makespan_s=makespan, scene_frames=frames_done, scene_frames_per_s=frames_done / span,
+                particle_steps_per_s=frames_done * reqs[0]["parts"].shape[0] / span,
+                latency_ms_p50=float(np.percentile(lat, 50) * 1e3), latency_ms_p99=float(np.percentile(lat, 99) * 1e3),
+                rounds=int(st["rounds"]), busy_s=st["busy_ms"] * 1e-3, admit_s=st["admit_ms"] * 1e-3,
+                retire_s=st["retire_ms"] * 1e-3, failed=int(st["failed"]),
+                mem_peak_gb=st["mem_peak"] / 1e9)
+
+
+def serving_bench(ctx, n_requests, rate):
+    t = time.time()
+    bases = serving_bases()
+    reqs = serving_workload(n_requests, rate)
+    log(f"serving: {len(bases)} bases (K0={SERVE_K0}) and {n_requests} requests in {time.time() - t:.1f}s")
+    batched = run_stream(ctx, bases, reqs, max_active=64)
+    log(f"serving batched: {batched['scene_frames_per_s']:.0f} scene-frames/s, "
+        f"p50 {batched['latency_ms_p50']:.1f} ms, p99 {batched['latency_ms_p99']:.1f} ms")
+    alone = run_stream(ctx, bases, reqs, max_active=1)
+    log(f"serving one at a time: {alone['scene_frames_per_s']:.0f} scene-frames/s, "
+        f"p50 {alone['latency_ms_p50']:.1f} ms, p99 {alone['latency_ms_p99']:.1f} ms")
+    sat = run_stream(ctx, bases, reqs, max_active=64, saturate=True)
+    log(f"serving batched, all queued at once: {sat['scene_frames_per_s']:.0f} scene-frames/s")
+    return {
+        "workload": f"{n_requests} requests, Poisson arrivals at {rate:g}/s, grids {SERVE_GRIDS}^3, K in [16, 192], "
+                    f"frames in [100, 2000], 2^14 particles each; one K0={SERVE_K0} basis per grid",
+        "batched": batched,
+        "one_at_a_time": alone,
+        "batched_all_queued": sat,
+        "throughput_vs_one_at_a_time": batched["scene_frames_per_s"] / alone["scene_frames_per_s"],
+        "capacity_vs_one_at_a_time": sat["scene_frames_per_s"] / alone["scene_frames_per_s"],
+    }
+
+
 def counter_pass_copies(frames, ms):
     """Three device-to-device copies of 1 GiB with the HIP runtime's own copy
     kernel: the reference for the FETCH_SIZE / WRITE_SIZE counters (each copy
@@ -171,6 +283,9 @@ def main():
                          "together (1 = one frame at a time)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=512)
+    ap.add_argument("--serving-rate", type=float, default=200.0)
     ap.add_argument("--counter-pass", action="store_true",
                     help="short run for hardware-counter collection: the frame loop, one persistent launch of "
                          "--reduced-steps steps, then three plain 1 GiB device copies that calibrate the byte "
@@ -297,6 +412,7 @@ def main():
             errors[name] = e
             r.close()
         log("omega rel err at 1000 steps: " + ", ".join(f"{k}={v[-1]:.2e}" for k, v in errors.items()))
+    serving = None if args.no_serving else serving_bench(ctx, args.serving_requests, args.serving_rate)
     ctx.close()
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
@@ -333,6 +449,7 @@ def main():
         "phase_ms_one_at_a_time": {"reduced_step": ms_st, "reconstruct": ms_rec, "trace": ms_tr},
         "roofline": head,
         "roofline_by_kernel": roof,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -193,7 +193,8 @@ typedef struct psf_server_config {
     uint32_t window;        /* frames per round: 2, 4, 8, 16 or 32 (0 = 32) */
     uint32_t max_active;    /* scenes on the device at once, 1..64 (0 = 32) */
     uint64_t mem_budget;    /* device bytes for admitted requests (0 = 64 GiB) */
-    uint32_t modes_per_wg;  /* modes per workgroup of the persistent layout, 1..8 (0 = 4) */
+    uint32_t modes_per_wg;  /* modes per workgroup of the persistent layout, 1..8 (0 = per request, the
+                               most that keep all of its C blocks in registers) */
 } psf_server_config;
 
 typedef struct psf_server_stats {
@@ -201,6 +202,7 @@ typedef struct psf_server_stats {
     uint32_t active, queued;
     uint64_t mem_in_use, mem_peak;   /* bytes held by admitted requests */
     double busy_ms;                  /* wall time inside rounds */
+    double admit_ms, retire_ms;      /* of which: admitting requests, reading back finished ones */
 } psf_server_stats;
 
 PSF_API int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** out);

@@ -46,7 +46,7 @@ class ServerStats(ct.Structure):
     _fields_ = [("submitted", ct.c_uint64), ("completed", ct.c_uint64), ("failed", ct.c_uint64),
                 ("rounds", ct.c_uint64), ("scene_frames", ct.c_uint64), ("active", ct.c_uint32),
                 ("queued", ct.c_uint32), ("mem_in_use", ct.c_uint64), ("mem_peak", ct.c_uint64),
-                ("busy_ms", ct.c_double)]
+                ("busy_ms", ct.c_double), ("admit_ms", ct.c_double), ("retire_ms", ct.c_double)]
 
 
 _lib = None

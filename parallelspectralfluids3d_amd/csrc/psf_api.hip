@@ -295,7 +295,35 @@ int refresh_visc(psf_scene* s, const std::vector<double>& lam)
 
 }  // namespace
 
+// host-side copies kept beside a scene: lambda (viscosity refreshes) and the
+// caller index of every stored particle
+struct psf_scene_host {
+    std::vector<double> lambda;
+    std::vector<uint32_t> pid;
+};
+static psf_scene_host* host_of(psf_scene* s) { return reinterpret_cast<psf_scene_host*>(s->host); }
+
 namespace psf {
+// omega (dev.omega) and the particles in the caller's order, read with
+// synchronous copies on the null stream (the scene's work must be complete);
+// reports the scene's status word like psf_get_omega
+int scene_read_results(psf_scene* s, double* omega, float* xyzw)
+{
+    int st = 0;
+    HIPCHK(s->ctx, hipMemcpy(&st, s->dev.status, sizeof(int), hipMemcpyDeviceToHost));
+    if (st == 1) return set_err(s->ctx, PSF_ERR_NONFINITE, "energy renormalisation saw a zero or non-finite energy");
+    if (st) return set_err(s->ctx, PSF_ERR_HIP, "persistent step: the exchange of y between workgroups timed out");
+    if (omega) HIPCHK(s->ctx, hipMemcpy(omega, s->dev.omega, s->d.K * 8, hipMemcpyDeviceToHost));
+    const uint32_t n = s->n_particles;
+    if (xyzw && n) {
+        std::vector<float> sorted(size_t(n) * 4);
+        HIPCHK(s->ctx, hipMemcpy(sorted.data(), s->dev.parts, size_t(n) * 16, hipMemcpyDeviceToHost));
+        const std::vector<uint32_t>& pid = host_of(s)->pid;
+        for (uint32_t j = 0; j < n; ++j) std::memcpy(xyzw + 4 * size_t(pid[j]), &sorted[4 * size_t(j)], 16);
+    }
+    return PSF_OK;
+}
+
 TraceParams scene_trace_params(const psf_scene* s) { return trace_params(s); }
 int scene_check_status(psf_scene* s) { return check_status(s); }
 int ctx_set_error(psf_ctx* ctx, int code, const std::string& msg)
@@ -305,13 +333,7 @@ int ctx_set_error(psf_ctx* ctx, int code, const std::string& msg)
 }
 }  // namespace psf
 
-// host-side copies kept beside a scene: lambda (viscosity refreshes) and the
-// caller index of every stored particle
-struct psf_scene_host {
-    std::vector<double> lambda;
-    std::vector<uint32_t> pid;
-};
-static psf_scene_host* host_of(psf_scene* s) { return reinterpret_cast<psf_scene_host*>(s->host); }
+
 
 extern "C" {
 
@@ -460,6 +482,16 @@ int psf_scene_destroy(psf_scene* s)
     if (!s) return PSF_ERR_ARG;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
+    scene_release(s);
+    return PSF_OK;
+}
+
+}  // extern "C"
+
+// Frees a scene whose device work is known to be complete, without waiting
+// on the context stream.
+void psf::scene_release(psf_scene* s)
+{
     persistent_scene_free(s);
     sfree(s, s->dev.omega); sfree(s, s->dev.lambda); sfree(s, s->dev.gravity); sfree(s, s->dev.visc);
     sfree(s, s->dev.y); sfree(s, s->dev.status); sfree(s, s->dev.C);
@@ -469,8 +501,9 @@ int psf_scene_destroy(psf_scene* s)
     s->ctx->scenes.erase(s);
     delete host_of(s);
     delete s;
-    return PSF_OK;
 }
+
+extern "C" {
 
 int psf_scene_upload_basis(psf_scene* s, const double* Phi, const double* lambda, const double* gravity)
 {
@@ -677,7 +710,15 @@ int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
         }
         key[i] = (c[2] * s->d.ny + c[1]) * s->d.nx + c[0];
     }
-    std::stable_sort(pid.begin(), pid.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    if (size_t(s->V) <= std::max<size_t>(size_t(4) * n, size_t(1) << 16)) {
+        // stable counting sort by voxel
+        std::vector<uint32_t> start(size_t(s->V) + 1, 0u);
+        for (uint32_t i = 0; i < n; ++i) start[key[i] + 1]++;
+        for (size_t v = 0; v < s->V; ++v) start[v + 1] += start[v];
+        for (uint32_t i = 0; i < n; ++i) pid[start[key[i]]++] = i;
+    } else {
+        std::stable_sort(pid.begin(), pid.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    }
     std::vector<float> sorted(size_t(n) * 4);
     for (uint32_t j = 0; j < n; ++j) std::memcpy(&sorted[4 * size_t(j)], xyzw + 4 * size_t(pid[j]), 16);
     if (n) {

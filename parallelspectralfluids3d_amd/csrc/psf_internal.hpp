@@ -16,6 +16,10 @@ struct SceneOptions {
 };
 
 int scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOptions& opt, psf_scene** out);
+// destroy a scene whose device work is complete (no wait on the context stream)
+void scene_release(psf_scene* s);
+// omega and particles (caller's order) by synchronous null-stream copies
+int scene_read_results(psf_scene* s, double* omega, float* xyzw);
 TraceParams scene_trace_params(const psf_scene* s);
 int scene_check_status(psf_scene* s);
 int ctx_set_error(psf_ctx* ctx, int code, const std::string& msg);

@@ -13,6 +13,18 @@ namespace psf {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
+// Address-space views of pointers that batched launches read from tables in
+// memory (the compiler then cannot infer them): global memory, and constant
+// memory for data a kernel only reads, so uniform loads become scalar loads.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(T* p) { return (gptr<T>)(p); }
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T* p) { return (cptr<T>)(p); }
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -111,31 +123,29 @@ __global__ __launch_bounds__(256) void k_update(double* __restrict__ omega, cons
 
 // ---------------------------------------------------------------------------
 // Reconstruction u_f = float(sum_k Phi[k][f] w_k).  Phi is column-major with a
-// zero-padded leading dimension ldF (a multiple of 256), so every lane issues
-// aligned 16-byte loads of VEC consecutive faces.  The four waves of a block
-// share 64*VEC faces and split the modes (k = wave, wave+4, ...); partial sums
-// are FP64 and combined in a fixed order.  The per-face arithmetic -- an fma
-// chain over the modes of each wave, then ((p0 + p1) + p2) + p3 -- is shared
-// with k_reconstruct_window, so both give the same bits.
+// zero-padded leading dimension ldF (a multiple of 256).  Every wave owns
+// 64*VEC consecutive faces (VEC per lane, one aligned 16-byte load per lane
+// and mode) and runs the whole sum for them: an FP64 fma chain over
+// k = 0, 1, ..., K-1 from 0.0, rounded to float at the end.  The w_k are
+// uniform across the wave and come through scalar loads.  k_reconstruct_window
+// runs the same chain per face and frame, so both give the same bits.
 template <typename T>
 __global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
                                                      const double* __restrict__ omega, float* __restrict__ u)
 {
     constexpr int VEC = 16 / sizeof(T);
-    extern __shared__ double sw[];
-    __shared__ double part[3][kWave * VEC];
-    for (int k = threadIdx.x; k < K; k += 256) sw[k] = omega[k];
-    __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const size_t f0 = static_cast<size_t>(blockIdx.x) * (kWave * VEC) + static_cast<size_t>(lane) * VEC;
+    const size_t fw = (static_cast<size_t>(blockIdx.x) * 4 + wv) * (kWave * VEC);
+    if (fw >= ldF) return;
+    const size_t f0 = fw + static_cast<size_t>(lane) * VEC;
     double acc[VEC];
 #pragma unroll
     for (int j = 0; j < VEC; ++j) acc[j] = 0.0;
     const T* base = Phi + f0;
 #pragma unroll 8
-    for (int k = wv; k < K; k += 4) {
-        const double w = sw[k];
+    for (int k = 0; k < K; ++k) {
+        const double w = omega[k];
         if constexpr (VEC == 4) {
             const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(base + static_cast<size_t>(k) * ldF));
             acc[0] = __builtin_fma(static_cast<double>(v.x), w, acc[0]);
@@ -148,64 +158,66 @@ __global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, 
             acc[1] = __builtin_fma(v.y, w, acc[1]);
         }
     }
-    if (wv > 0) {
 #pragma unroll
-        for (int j = 0; j < VEC; ++j) part[wv - 1][lane * VEC + j] = acc[j];
-    }
-    __syncthreads();
-    if (wv == 0) {
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) {
-            const double s = ((acc[j] + part[0][lane * VEC + j]) + part[1][lane * VEC + j]) + part[2][lane * VEC + j];
-            const size_t f = f0 + j;
-            if (f < F) u[f] = static_cast<float>(s);
-        }
-    }
+    for (int j = 0; j < VEC; ++j)
+        if (f0 + j < F) u[f0 + j] = static_cast<float>(acc[j]);
 }
 
 // ---------------------------------------------------------------------------
 // Reconstruction for a window of TT frames: u_t = float(Phi w_t), t < TT,
 // from the history hist[k * TT + t] of w.  One pass over Phi serves all TT
 // frames, so the kernel is bound by FP64 FMA throughput instead of HBM.
-// Each lane owns FPT consecutive faces and all TT frames; wave w of a block
-// takes the modes k = w, w+4, ... in increasing order, exactly as
-// k_reconstruct, and the four partial sums are combined in the same order,
-// so u_t has the bits k_reconstruct gives for w_t.  The w_t of one mode are
-// uniform across the wave and come through scalar loads.
-template <typename T, int FPT, int TT>
-__device__ __forceinline__ void window_body(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
-                                            const double* __restrict__ hist, float* __restrict__ uh, uint32_t blk)
+// Every wave owns 64*FPT consecutive faces, each lane FPT of them for all TT
+// frames, and runs the same per-face fma chain over k = 0..K-1 as
+// k_reconstruct.  The w_t of one mode are uniform across the wave and come
+// through scalar loads.
+// Scene of a block in a batched launch: tab[0..n) (n <= 64) by ascending
+// block0; one table load per lane and a ballot.
+template <typename Tab>
+__device__ __forceinline__ int scene_of_block(const Tab* __restrict__ tab, int n, uint32_t blk)
 {
-    constexpr int CH = TT < 8 ? TT : 8;      // frames combined per LDS round
-    __shared__ double part[3][CH][kWave * FPT];
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool le = lane < n && tab[lane].block0 <= blk;
+    return __popcll(__ballot(le)) - 1;
+}
+
+template <typename T, int FPT, int TT>
+__device__ __forceinline__ void window_body(const T* Phi_, size_t ldF, uint32_t F, int K, const double* hist_,
+                                            float* uh_, uint32_t blk)
+{
+    const gptr<const T> Phi = as_global(Phi_);
+    const cptr<double> hist = as_const(hist_);
+    const gptr<float> uh = as_global(uh_);
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const size_t f0 = static_cast<size_t>(blk) * (kWave * FPT) + static_cast<size_t>(lane) * FPT;
+    const size_t fw = (static_cast<size_t>(blk) * 4 + wv) * (kWave * FPT);
+    if (fw >= ldF) return;
+    const size_t f0 = fw + static_cast<size_t>(lane) * FPT;
     double acc[FPT][TT];
 #pragma unroll
     for (int j = 0; j < FPT; ++j)
 #pragma unroll
         for (int t = 0; t < TT; ++t) acc[j][t] = 0.0;
-    const T* base = Phi + f0;
+    const gptr<const T> base = Phi + f0;
 #pragma unroll 2
-    for (int k = wv; k < K; k += 4) {
+    for (int k = 0; k < K; ++k) {
         double ph[FPT];
         if constexpr (sizeof(T) == 4 && FPT == 4) {
-            const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(base + static_cast<size_t>(k) * ldF));
+            const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4>>(base + static_cast<size_t>(k) * ldF));
             ph[0] = v.x; ph[1] = v.y; ph[2] = v.z; ph[3] = v.w;
         } else if constexpr (sizeof(T) == 4 && FPT == 2) {
             typedef float f32x2 __attribute__((ext_vector_type(2)));
-            const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(base + static_cast<size_t>(k) * ldF));
+            const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x2>>(base + static_cast<size_t>(k) * ldF));
             ph[0] = v.x; ph[1] = v.y;
         } else {
 #pragma unroll
             for (int j = 0; j < FPT; j += 2) {
                 const f64x2 v = __builtin_nontemporal_load(
-                    reinterpret_cast<const f64x2*>(base + static_cast<size_t>(k) * ldF + j));
+                    reinterpret_cast<gptr<const f64x2>>(base + static_cast<size_t>(k) * ldF + j));
                 ph[j] = v.x; ph[j + 1] = v.y;
             }
         }
-        const double* wk = hist + static_cast<size_t>(k) * TT;
+        const cptr<double> wk = hist + static_cast<size_t>(k) * TT;
 #pragma unroll
         for (int t = 0; t < TT; ++t) {
             const double w = wk[t];
@@ -214,27 +226,10 @@ __device__ __forceinline__ void window_body(const T* __restrict__ Phi, size_t ld
         }
     }
 #pragma unroll
-    for (int c = 0; c < TT; c += CH) {
-        if (wv > 0) {
+    for (int t = 0; t < TT; ++t)
 #pragma unroll
-            for (int t = 0; t < CH; ++t)
-#pragma unroll
-                for (int j = 0; j < FPT; ++j) part[wv - 1][t][lane * FPT + j] = acc[j][c + t];
-        }
-        __syncthreads();
-        if (wv == 0) {
-#pragma unroll
-            for (int t = 0; t < CH; ++t)
-#pragma unroll
-                for (int j = 0; j < FPT; ++j) {
-                    const int q = lane * FPT + j;
-                    const double s = ((acc[j][c + t] + part[0][t][q]) + part[1][t][q]) + part[2][t][q];
-                    const size_t f = f0 + j;
-                    if (f < F) uh[static_cast<size_t>(c + t) * ldF + f] = static_cast<float>(s);
-                }
-        }
-        __syncthreads();
-    }
+        for (int j = 0; j < FPT; ++j)
+            if (f0 + j < F) uh[static_cast<size_t>(t) * ldF + f0 + j] = static_cast<float>(acc[j][t]);
 }
 
 template <typename T, int FPT, int TT>
@@ -250,9 +245,7 @@ __global__ __launch_bounds__(256) void k_reconstruct_window(const T* __restrict_
 template <typename T, int FPT, int TT>
 __global__ __launch_bounds__(256) void k_reconstruct_window_multi(const WindowScene* __restrict__ tab, int n)
 {
-    int j = 0;
-    while (j + 1 < n && tab[j + 1].block0 <= blockIdx.x) ++j;
-    const WindowScene& w = tab[j];
+    const WindowScene& w = tab[scene_of_block(tab, n, blockIdx.x)];
     window_body<T, FPT, TT>(static_cast<const T*>(w.Phi), w.ldF, w.F, w.K, w.hist, w.uh, blockIdx.x - w.block0);
 }
 
@@ -300,8 +293,8 @@ __device__ __forceinline__ uint32_t respawn_draw(uint32_t seed, uint32_t step, u
 
 // one particle record through one frame; id is its storage slot (cells,
 // caller index)
-__device__ __forceinline__ float4 trace_one(const TraceParams& P, const float* __restrict__ u, float4 p,
-                                            const uint32_t* __restrict__ pid, uint32_t id, int32_t* __restrict__ cells)
+__device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<const float> u, float4 p,
+                                            const gptr<const uint32_t> pid, uint32_t id, int32_t* __restrict__ cells)
 {
 #pragma clang fp contract(off)
     const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny), nz = static_cast<int>(P.nz);
@@ -418,7 +411,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
 {
     const uint32_t id = blockIdx.x * 256u + threadIdx.x;
     if (id >= n) return;
-    parts[id] = trace_one(P, u, parts[id], pid, id, cells);
+    parts[id] = trace_one(P, as_global(u), parts[id], as_global(pid), id, cells);
 }
 
 // Several scenes, each through nframes frames of its window: particle
@@ -427,18 +420,21 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
 // stores it once.
 __global__ __launch_bounds__(256) void k_trace_window_multi(const TraceScene* __restrict__ tab, int n)
 {
-    int j = 0;
-    while (j + 1 < n && tab[j + 1].block0 <= blockIdx.x) ++j;
-    const TraceScene& ts = tab[j];
+    const TraceScene& ts = tab[scene_of_block(tab, n, blockIdx.x)];
     const uint32_t id = (blockIdx.x - ts.block0) * 256u + threadIdx.x;
     if (id >= ts.n) return;
     TraceParams P = ts.P;
-    float4 p = ts.parts[id];
+    const gptr<f32x4> parts = as_global(reinterpret_cast<f32x4*>(ts.parts));
+    const gptr<const float> uh = as_global(ts.u_hist);
+    const gptr<const uint32_t> pid = as_global(ts.pid);
+    const f32x4 v = parts[id];
+    float4 p = make_float4(v.x, v.y, v.z, v.w);
     for (uint32_t t = 0; t < ts.nframes; ++t) {
-        p = trace_one(P, ts.u_hist + static_cast<size_t>(t) * ts.ldU, p, ts.pid, id, nullptr);
+        p = trace_one(P, uh + static_cast<size_t>(t) * ts.ldU, p, pid, id, nullptr);
         P.frame++;
     }
-    ts.parts[id] = p;
+    const f32x4 o = {p.x, p.y, p.z, p.w};
+    parts[id] = o;
 }
 
 __global__ void k_zero_int(int* p, int n)
@@ -477,13 +473,12 @@ hipError_t launch_update(double* omega, const double* y, const double* visc, con
 hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F, int K, const double* omega,
                               float* u, hipStream_t s)
 {
-    const size_t sh = sizeof(double) * K;
     if (prec == PSF_PHI_FP32) {
-        const unsigned grid = static_cast<unsigned>(ldF / (kWave * 4));
-        k_reconstruct<float><<<grid, 256, sh, s>>>(static_cast<const float*>(Phi), ldF, F, K, omega, u);
+        const unsigned grid = static_cast<unsigned>((ldF + 256 * 4 - 1) / (256 * 4));
+        k_reconstruct<float><<<grid, 256, 0, s>>>(static_cast<const float*>(Phi), ldF, F, K, omega, u);
     } else {
-        const unsigned grid = static_cast<unsigned>(ldF / (kWave * 2));
-        k_reconstruct<double><<<grid, 256, sh, s>>>(static_cast<const double*>(Phi), ldF, F, K, omega, u);
+        const unsigned grid = static_cast<unsigned>((ldF + 256 * 2 - 1) / (256 * 2));
+        k_reconstruct<double><<<grid, 256, 0, s>>>(static_cast<const double*>(Phi), ldF, F, K, omega, u);
     }
     return hipGetLastError();
 }
@@ -492,7 +487,7 @@ namespace {
 template <typename T, int FPT, int TT>
 hipError_t window_launch(const void* Phi, size_t ldF, uint32_t F, int K, const double* hist, float* uh, hipStream_t s)
 {
-    const unsigned grid = static_cast<unsigned>(ldF / (kWave * FPT));
+    const unsigned grid = static_cast<unsigned>((ldF + 256 * FPT - 1) / (256 * FPT));
     k_reconstruct_window<T, FPT, TT><<<grid, 256, 0, s>>>(static_cast<const T*>(Phi), ldF, F, K, hist, uh);
     return hipGetLastError();
 }
@@ -535,7 +530,11 @@ hipError_t launch_reconstruct_window_multi(int prec, int window, const WindowSce
     return hipGetLastError();
 }
 
-uint32_t window_blocks(int window, size_t ldF) { return static_cast<uint32_t>(ldF / (kWave * (window == 32 ? 2 : 4))); }
+uint32_t window_blocks(int window, size_t ldF)
+{
+    const size_t per_block = 256 * (window == 32 ? 2 : 4);
+    return static_cast<uint32_t>((ldF + per_block - 1) / per_block);
+}
 
 hipError_t launch_trace_window_multi(const TraceScene* d_tab, int n, uint32_t blocks, hipStream_t s)
 {

@@ -131,6 +131,14 @@ size_t persistent_c_bytes(int prec, int K, int ipw_req)
     return size_t(G) * bpw * 64 * 16;
 }
 
+int persistent_auto_ipw(int prec, int K)
+{
+    const int nb = (K + kTile - 1) / kTile;
+    const int nBi = blocks_per_mode(prec, nb);
+    int ipw = (4 * kRB) / nBi;
+    return ipw < 1 ? 1 : (ipw > kMaxIpw ? kMaxIpw : ipw);
+}
+
 int persistent_workgroups(int prec, int K, int ipw_req)
 {
     if (!persistent_supported(prec, K, ipw_req)) return 0;

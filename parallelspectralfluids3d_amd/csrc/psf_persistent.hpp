@@ -8,6 +8,8 @@ namespace psf {
 bool persistent_supported(int c_precision, int K, int ipw_req = 0);
 size_t persistent_c_bytes(int c_precision, int K, int ipw_req = 0);
 int persistent_workgroups(int c_precision, int K, int ipw_req = 0);
+// largest modes-per-workgroup (<= 8) whose blocks all stay in registers
+int persistent_auto_ipw(int c_precision, int K);
 hipError_t persistent_scene_init(psf_scene* s);
 void persistent_scene_free(psf_scene* s);
 int persistent_upload_c(psf_scene* s, const double* C);

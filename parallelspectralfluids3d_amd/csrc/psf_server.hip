@@ -1,12 +1,19 @@
 // Scene server: continuous batching of independent scenes (C ABI in psf.h).
 //
 // Every round advances all active scenes by up to `window` integrate() frames
-// (psfsolver.cpp:21-234) with three launches on the context stream:
-//   1. one persistent launch holding every scene's workgroups; each scene
-//      runs its own number of reduced steps and writes its omega history;
-//   2. one window reconstruction over all scenes' Phi (FP64 sums);
-//   3. one particle launch carrying every scene's particles through their
-//      frames of the round.
+// (psfsolver.cpp:21-234) with three launches:
+//   1. on the context stream, one persistent launch holding every scene's
+//      workgroups; each scene runs its own number of reduced steps and
+//      writes its omega history;
+//   2. on the server's second stream, after 1, one window reconstruction
+//      over all scenes' Phi (FP64 sums);
+//   3. on the second stream, one particle launch carrying every scene's
+//      particles through their frames of the round.
+// The reduced steps of round r+1 need only omega, so they run while round r
+// reconstructs and traces; each scene has two omega-history buffers, and a
+// round's launches wait for the round two back to release its buffer.  The
+// host keeps one round in flight: a call launches round r, then waits for
+// round r-1 and hands out the requests that finished in it.
 // Scenes never exchange data, and each scene's arithmetic depends only on its
 // own inputs and layout, so a request gives the same bits alone and in a mix.
 // Admission is FIFO at round start, bounded by active slots, the workgroups
@@ -46,7 +53,9 @@ enum ReqState { kQueued = 0, kActive = 1, kDone = 2, kFailed = 3 };
 
 struct Request {
     uint64_t id = 0;
-    uint32_t basis = 0, K = 0, nframes = 0, done = 0, n = 0;
+    uint32_t basis = 0, K = 0, nframes = 0, sched = 0, n = 0;
+    int64_t last_round = -1;     // round that runs its last frames
+    double* hist2 = nullptr;     // second omega-history buffer (the first is the scene's)
     double dt = 0.0, nu = 0.0;
     int grav = 0;
     float lifetime = 0.0f;
@@ -54,7 +63,7 @@ struct Request {
     std::vector<float> parts;    // input, then result
     psf_scene* scene = nullptr;
     uint64_t footprint = 0;
-    int wgs = 0;
+    int wgs = 0, ipw = 0;
     int state = kQueued;
     std::string error;
     Clock::time_point t_submit, t_done;
@@ -70,13 +79,19 @@ struct psf_server {
     std::deque<Request*> queue;
     std::vector<Request*> active;
     uint64_t next_id = 1;
-    PersistentBatch* batch = nullptr;
+    // per round parity: launch tables (pinned staging and device copies)
+    PersistentBatch* batch[2] = {nullptr, nullptr};
+    WindowScene* d_wtab[2] = {nullptr, nullptr};
+    WindowScene* h_wtab[2] = {nullptr, nullptr};
+    TraceScene* d_ttab[2] = {nullptr, nullptr};
+    TraceScene* h_ttab[2] = {nullptr, nullptr};
+    hipEvent_t ev_step[2] = {nullptr, nullptr};   // after the persistent launch of a round
+    hipEvent_t ev_done[2] = {nullptr, nullptr};   // after the particle launch of a round
+    hipStream_t second = nullptr;                 // reconstruction and particles
+    int64_t launched = 0;                         // rounds launched
+    int64_t retired = -1;                         // last round whose requests were handed out
     int wg_capacity = 0, wgs_in_use = 0;
     uint64_t mem_in_use = 0, mem_peak = 0;
-    WindowScene* d_wtab = nullptr;
-    WindowScene* h_wtab = nullptr;
-    TraceScene* d_ttab = nullptr;
-    TraceScene* h_ttab = nullptr;
     psf_server_stats st{};
 };
 
@@ -90,10 +105,17 @@ int fail(psf_ctx* ctx, int code, const std::string& msg) { return ctx_set_error(
         if (e_ != hipSuccess) return fail((ctx), PSF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// modes per workgroup of a request's persistent layout
+int request_ipw(const psf_server* sv, uint32_t K)
+{
+    return sv->cfg.modes_per_wg ? static_cast<int>(sv->cfg.modes_per_wg)
+                                : persistent_auto_ipw(PSF_C_BF16X2, static_cast<int>(K));
+}
+
 // Device bytes of a request, as the pool holds them.
 uint64_t request_bytes(const psf_server* sv, const Basis& b, uint32_t K, uint32_t n)
 {
-    const int ipw = static_cast<int>(sv->cfg.modes_per_wg);
+    const int ipw = request_ipw(sv, K);
     const size_t cb = persistent_c_bytes(PSF_C_BF16X2, static_cast<int>(K), ipw);
     const int G = persistent_workgroups(PSF_C_BF16X2, static_cast<int>(K), ipw);
     const size_t bpw = G ? cb / (size_t(G) * 1024) : 0;
@@ -102,7 +124,7 @@ uint64_t request_bytes(const psf_server* sv, const Basis& b, uint32_t K, uint32_
         K * 8ull, K * 8ull, K * 8ull, K * 8ull, K * 8ull, 64 * 4ull,   // omega lambda gravity visc y status
         cb, b.F * 4ull, n * 16ull, n * 4ull,                            // C u parts pid
         bpw * 4, size_t(G) * ((K + G - 1) / G) * 8, 2ull * 16 * K * 8,   // meta inv_scale xbuf
-        K * Wn * 8, b.ldF * Wn * 4, K * 8ull,                           // hist u_hist omega_view
+        K * Wn * 8, b.ldF * Wn * 4, K * 8ull, K * Wn * 8,               // hist u_hist omega_view hist2
     };
     uint64_t t = 0;
     for (size_t x : sizes) t += pool_class_bytes(x);
@@ -118,7 +140,7 @@ int admit(psf_server* sv, Request* r)
     d.phi_precision = PSF_PHI_FP32;
     d.max_particles = r->n;
     SceneOptions o;
-    o.modes_per_wg = static_cast<int>(sv->cfg.modes_per_wg);
+    o.modes_per_wg = r->ipw;
     o.borrowed_phi = b.Phi;
     o.cells = false;
     o.pooled = true;
@@ -126,21 +148,28 @@ int admit(psf_server* sv, Request* r)
     int rc = scene_create_ex(sv->ctx, &d, o, &s);
     if (rc) return rc;
     r->scene = s;
+    // particles first: host sorting and null-stream copies overlap the GPU's work
+    if ((rc = psf_set_particles(s, r->parts.data(), r->n))) return rc;
+    if ((rc = psf_set_frame_window(s, sv->cfg.window))) return rc;
+    SRV_HIPCHK(sv->ctx, scene_alloc(s, reinterpret_cast<void**>(&r->hist2), sizeof(double) * r->K * sv->cfg.window));
     if ((rc = psf_scene_upload_basis(s, nullptr, b.lambda.data(), b.gravity.data()))) return rc;
     SRV_HIPCHK(sv->ctx, launch_pack_c(s, b.C, static_cast<int>(b.desc.K), sv->ctx->stream));
     s->has_C = true;
     if ((rc = psf_set_params(s, r->dt, r->nu, r->grav, r->lifetime))) return rc;
     if ((rc = psf_set_omega(s, r->omega.data()))) return rc;
-    if ((rc = psf_set_particles(s, r->parts.data(), r->n))) return rc;
-    if ((rc = psf_set_frame_window(s, sv->cfg.window))) return rc;
     return PSF_OK;
 }
 
+// frees a request's scene; its device work must be complete
 void release(psf_server* sv, Request* r)
 {
-    if (r->scene) psf_scene_destroy(r->scene);
+    if (r->scene) {
+        scene_free(r->scene, r->hist2);
+        scene_release(r->scene);
+    }
     r->scene = nullptr;
-    sv->wgs_in_use -= r->wgs;
+    r->hist2 = nullptr;
+    if (r->last_round < 0) sv->wgs_in_use -= r->wgs;   // otherwise freed when its last round launched
     sv->mem_in_use -= r->footprint;
 }
 
@@ -156,6 +185,7 @@ void admit_queued(psf_server* sv)
         sv->mem_peak = std::max(sv->mem_peak, sv->mem_in_use);
         if (admit(sv, r) != PSF_OK) {
             r->error = sv->ctx->err;
+            (void)hipStreamSynchronize(sv->ctx->stream);   // uploads already queued for the scene
             r->state = kFailed;
             r->t_done = Clock::now();
             release(sv, r);
@@ -179,7 +209,6 @@ int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** o
     if (c.window == 0) c.window = 32;
     if (c.max_active == 0) c.max_active = 32;
     if (c.mem_budget == 0) c.mem_budget = 64ull << 30;
-    if (c.modes_per_wg == 0) c.modes_per_wg = 4;
     if (c.window != 2 && c.window != 4 && c.window != 8 && c.window != 16 && c.window != 32)
         return fail(ctx, PSF_ERR_ARG, "server window must be 2, 4, 8, 16 or 32");
     if (c.max_active > kMaxActive) return fail(ctx, PSF_ERR_ARG, "max_active must be at most 64");
@@ -189,20 +218,27 @@ int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** o
     if (!sv) return PSF_ERR_NOMEM;
     sv->ctx = ctx;
     sv->cfg = c;
-    sv->batch = persistent_batch_create(ctx->device, static_cast<int>(kMaxActive), 8192);
-    hipError_t e = sv->batch ? hipSuccess : hipErrorOutOfMemory;
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sv->d_wtab), sizeof(WindowScene) * kMaxActive);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sv->d_ttab), sizeof(TraceScene) * kMaxActive);
-    if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void**>(&sv->h_wtab), sizeof(WindowScene) * kMaxActive, hipHostMallocDefault);
-    if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void**>(&sv->h_ttab), sizeof(TraceScene) * kMaxActive, hipHostMallocDefault);
+    hipError_t e = hipStreamCreateWithFlags(&sv->second, hipStreamNonBlocking);
+    for (int q = 0; q < 2 && e == hipSuccess; ++q) {
+        sv->batch[q] = persistent_batch_create(ctx->device, static_cast<int>(kMaxActive), 8192);
+        if (!sv->batch[q]) e = hipErrorOutOfMemory;
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sv->d_wtab[q]), sizeof(WindowScene) * kMaxActive);
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sv->d_ttab[q]), sizeof(TraceScene) * kMaxActive);
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&sv->h_wtab[q]), sizeof(WindowScene) * kMaxActive,
+                              hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&sv->h_ttab[q]), sizeof(TraceScene) * kMaxActive,
+                              hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->ev_step[q], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->ev_done[q], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         ctx->servers.insert(sv);
         psf_server_destroy(sv);
         return fail(ctx, PSF_ERR_HIP, std::string("server set-up: ") + hipGetErrorString(e));
     }
-    sv->wg_capacity = persistent_batch_wg_capacity(sv->batch, PSF_C_BF16X2);
+    sv->wg_capacity = persistent_batch_wg_capacity(sv->batch[0], PSF_C_BF16X2);
     ctx->servers.insert(sv);
     *out = sv;
     return PSF_OK;
@@ -214,17 +250,26 @@ int psf_server_destroy(psf_server* sv)
     psf_ctx* ctx = sv->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (sv->second) (void)hipStreamSynchronize(sv->second);
     for (auto& kv : sv->reqs)
-        if (kv.second->scene) psf_scene_destroy(kv.second->scene);
+        if (kv.second->scene) {
+            scene_free(kv.second->scene, kv.second->hist2);
+            scene_release(kv.second->scene);
+        }
     for (Basis& b : sv->bases) {
         if (b.Phi) (void)hipFree(b.Phi);
         if (b.C) (void)hipFree(b.C);
     }
-    persistent_batch_destroy(sv->batch);
-    if (sv->d_wtab) (void)hipFree(sv->d_wtab);
-    if (sv->d_ttab) (void)hipFree(sv->d_ttab);
-    if (sv->h_wtab) (void)hipHostFree(sv->h_wtab);
-    if (sv->h_ttab) (void)hipHostFree(sv->h_ttab);
+    for (int q = 0; q < 2; ++q) {
+        persistent_batch_destroy(sv->batch[q]);
+        if (sv->d_wtab[q]) (void)hipFree(sv->d_wtab[q]);
+        if (sv->d_ttab[q]) (void)hipFree(sv->d_ttab[q]);
+        if (sv->h_wtab[q]) (void)hipHostFree(sv->h_wtab[q]);
+        if (sv->h_ttab[q]) (void)hipHostFree(sv->h_ttab[q]);
+        if (sv->ev_step[q]) (void)hipEventDestroy(sv->ev_step[q]);
+        if (sv->ev_done[q]) (void)hipEventDestroy(sv->ev_done[q]);
+    }
+    if (sv->second) (void)hipStreamDestroy(sv->second);
     ctx->servers.erase(sv);
     delete sv;
     return PSF_OK;
@@ -296,7 +341,8 @@ int psf_server_submit(psf_server* sv, uint32_t basis_id, uint32_t K, uint32_t nf
     if (K == 0 || K > b.desc.K) return fail(ctx, PSF_ERR_ARG, "K must be in [1, K0]");
     if (nframes == 0) return fail(ctx, PSF_ERR_ARG, "nframes must be positive");
     if (!std::isfinite(dt) || !std::isfinite(viscosity)) return fail(ctx, PSF_ERR_ARG, "non-finite parameter");
-    const int wgs = persistent_workgroups(PSF_C_BF16X2, static_cast<int>(K), static_cast<int>(sv->cfg.modes_per_wg));
+    const int ipw = request_ipw(sv, K);
+    const int wgs = persistent_workgroups(PSF_C_BF16X2, static_cast<int>(K), ipw);
     const uint64_t fp = request_bytes(sv, b, K, n);
     if (wgs > sv->wg_capacity) return fail(ctx, PSF_ERR_UNSUPPORTED, "request needs more workgroups than one launch holds");
     if (fp > sv->cfg.mem_budget) return fail(ctx, PSF_ERR_NOMEM, "request footprint exceeds the server's memory budget");
@@ -314,6 +360,7 @@ int psf_server_submit(psf_server* sv, uint32_t basis_id, uint32_t K, uint32_t nf
     r->parts.assign(particles, particles + size_t(n) * 4);
     r->footprint = fp;
     r->wgs = wgs;
+    r->ipw = ipw;
     r->t_submit = Clock::now();
     Request* raw = r.get();
     sv->reqs.emplace(raw->id, std::move(r));
@@ -331,69 +378,98 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
     if (completed) *completed = 0;
     SRV_HIPCHK(ctx, hipSetDevice(ctx->device));
     admit_queued(sv);
-    const int n = static_cast<int>(sv->active.size());
-    if (n == 0) return PSF_OK;
-    const int W = static_cast<int>(sv->cfg.window);
-    hipStream_t stream = ctx->stream;
-    std::vector<psf_scene*> sc(n);
-    std::vector<int> nsteps(n), ld(n, W);
-    std::vector<double*> hist(n);
-    uint32_t wblocks = 0, tblocks = 0;
-    for (int j = 0; j < n; ++j) {
-        Request* r = sv->active[j];
-        psf_scene* s = r->scene;
-        sc[j] = s;
-        nsteps[j] = static_cast<int>(std::min<uint32_t>(static_cast<uint32_t>(W), r->nframes - r->done));
-        hist[j] = s->dev.hist;
-        const Basis& b = sv->bases[r->basis];
-        sv->h_wtab[j] = WindowScene{b.Phi, s->dev.ldF, s->F, static_cast<int>(s->d.K), s->dev.hist, s->dev.u_hist, wblocks};
-        wblocks += window_blocks(W, s->dev.ldF);
-        TraceScene t{};
-        t.P = scene_trace_params(s);
-        t.u_hist = s->dev.u_hist;
-        t.ldU = s->dev.ldF;
-        t.parts = s->dev.parts;
-        t.pid = s->dev.pid;
-        t.n = s->n_particles;
-        t.nframes = static_cast<uint32_t>(nsteps[j]);
-        t.block0 = tblocks;
-        sv->h_ttab[j] = t;
-        tblocks += (s->n_particles + 255) / 256;
-    }
-    SRV_HIPCHK(ctx, launch_persistent_batch(sv->batch, sc.data(), nsteps.data(), hist.data(), ld.data(), n, stream));
-    SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_wtab, sv->h_wtab, sizeof(WindowScene) * n, hipMemcpyHostToDevice, stream));
-    SRV_HIPCHK(ctx, launch_reconstruct_window_multi(PSF_PHI_FP32, W, sv->d_wtab, n, wblocks, stream));
-    SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_ttab, sv->h_ttab, sizeof(TraceScene) * n, hipMemcpyHostToDevice, stream));
-    SRV_HIPCHK(ctx, launch_trace_window_multi(sv->d_ttab, n, tblocks, stream));
-    SRV_HIPCHK(ctx, hipStreamSynchronize(stream));
-    sv->st.rounds++;
-    std::vector<Request*> keep;
-    keep.reserve(n);
-    for (int j = 0; j < n; ++j) {
-        Request* r = sv->active[j];
-        r->scene->frame += static_cast<uint32_t>(nsteps[j]);
-        r->done += static_cast<uint32_t>(nsteps[j]);
-        sv->st.scene_frames += static_cast<uint64_t>(nsteps[j]);
-        if (r->done < r->nframes) {
-            keep.push_back(r);
-            continue;
+    const Clock::time_point t1 = Clock::now();
+    sv->st.admit_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+
+    // ---- launch the next round for every scene with frames left ----------------
+    std::vector<Request*> run;
+    for (Request* r : sv->active)
+        if (r->sched < r->nframes) run.push_back(r);
+    const int n = static_cast<int>(run.size());
+    const bool launch = n > 0;
+    if (launch) {
+        const int W = static_cast<int>(sv->cfg.window);
+        const int q = static_cast<int>(sv->launched & 1);
+        hipStream_t sa = ctx->stream, sb = sv->second;
+        std::vector<psf_scene*> sc(n);
+        std::vector<int> nsteps(n), ld(n, W);
+        std::vector<double*> hist(n);
+        uint32_t wblocks = 0, tblocks = 0;
+        for (int j = 0; j < n; ++j) {
+            Request* r = run[j];
+            psf_scene* s = r->scene;
+            sc[j] = s;
+            nsteps[j] = static_cast<int>(std::min<uint32_t>(static_cast<uint32_t>(W), r->nframes - r->sched));
+            hist[j] = q == 0 ? s->dev.hist : r->hist2;
+            const Basis& b = sv->bases[r->basis];
+            sv->h_wtab[q][j] = WindowScene{b.Phi, s->dev.ldF, s->F, static_cast<int>(s->d.K), hist[j], s->dev.u_hist,
+                                           wblocks};
+            wblocks += window_blocks(W, s->dev.ldF);
+            TraceScene t{};
+            t.P = scene_trace_params(s);
+            t.u_hist = s->dev.u_hist;
+            t.ldU = s->dev.ldF;
+            t.parts = s->dev.parts;
+            t.pid = s->dev.pid;
+            t.n = s->n_particles;
+            t.nframes = static_cast<uint32_t>(nsteps[j]);
+            t.block0 = tblocks;
+            sv->h_ttab[q][j] = t;
+            tblocks += (s->n_particles + 255) / 256;
         }
-        int rc = scene_check_status(r->scene);
-        if (rc == PSF_OK) rc = psf_get_omega(r->scene, r->omega.data());
-        if (rc == PSF_OK) rc = psf_get_particles(r->scene, r->parts.data(), r->n);
-        r->t_done = Clock::now();
-        if (rc == PSF_OK) {
-            r->state = kDone;
-            sv->st.completed++;
-            if (completed) (*completed)++;
-        } else {
-            r->state = kFailed;
-            r->error = ctx->err;
-            sv->st.failed++;
+        // the round two back read this parity's history buffers
+        if (sv->launched >= 2) SRV_HIPCHK(ctx, hipStreamWaitEvent(sa, sv->ev_done[q], 0));
+        SRV_HIPCHK(ctx, launch_persistent_batch(sv->batch[q], sc.data(), nsteps.data(), hist.data(), ld.data(), n, sa));
+        SRV_HIPCHK(ctx, hipEventRecord(sv->ev_step[q], sa));
+        SRV_HIPCHK(ctx, hipStreamWaitEvent(sb, sv->ev_step[q], 0));
+        SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_wtab[q], sv->h_wtab[q], sizeof(WindowScene) * n, hipMemcpyHostToDevice, sb));
+        SRV_HIPCHK(ctx, launch_reconstruct_window_multi(PSF_PHI_FP32, W, sv->d_wtab[q], n, wblocks, sb));
+        SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_ttab[q], sv->h_ttab[q], sizeof(TraceScene) * n, hipMemcpyHostToDevice, sb));
+        SRV_HIPCHK(ctx, launch_trace_window_multi(sv->d_ttab[q], n, tblocks, sb));
+        SRV_HIPCHK(ctx, hipEventRecord(sv->ev_done[q], sb));
+        for (int j = 0; j < n; ++j) {
+            Request* r = run[j];
+            r->scene->frame += static_cast<uint32_t>(nsteps[j]);
+            r->sched += static_cast<uint32_t>(nsteps[j]);
+            sv->st.scene_frames += static_cast<uint64_t>(nsteps[j]);
+            if (r->sched == r->nframes) {
+                r->last_round = sv->launched;
+                sv->wgs_in_use -= r->wgs;
+            }
         }
-        release(sv, r);
+        sv->launched++;
+        sv->st.rounds++;
     }
-    sv->active.swap(keep);
+
+    // ---- hand out the requests of the round before (all rounds when idle) ----------
+    const int64_t target = launch ? sv->launched - 2 : sv->launched - 1;
+    if (target > sv->retired) {
+        SRV_HIPCHK(ctx, hipEventSynchronize(sv->ev_done[target & 1]));
+        const Clock::time_point t2 = Clock::now();
+        std::vector<Request*> keep;
+        keep.reserve(sv->active.size());
+        for (Request* r : sv->active) {
+            if (r->last_round < 0 || r->last_round > target) {
+                keep.push_back(r);
+                continue;
+            }
+            const int rc = scene_read_results(r->scene, r->omega.data(), r->parts.data());
+            r->t_done = Clock::now();
+            if (rc == PSF_OK) {
+                r->state = kDone;
+                sv->st.completed++;
+                if (completed) (*completed)++;
+            } else {
+                r->state = kFailed;
+                r->error = ctx->err;
+                sv->st.failed++;
+            }
+            release(sv, r);
+        }
+        sv->active.swap(keep);
+        sv->retired = target;
+        sv->st.retire_ms += std::chrono::duration<double, std::milli>(Clock::now() - t2).count();
+    }
     sv->st.busy_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     return PSF_OK;
 }

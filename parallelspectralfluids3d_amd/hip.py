@@ -208,7 +208,7 @@ class Server:
     frames; result() returns omega, particles and the latency of a finished
     request."""
 
-    def __init__(self, ctx: Context, window=32, max_active=32, mem_budget=0, modes_per_wg=4):
+    def __init__(self, ctx: Context, window=32, max_active=32, mem_budget=0, modes_per_wg=0):
         self.ctx = ctx
         self.lib = ctx.lib
         cfg = L.ServerConfig()
