@@ -181,6 +181,7 @@ TraceParams trace_params(const psf_scene* s)
         p.ext[c] = s->d.aabb_max[c] - s->d.aabb_min[c];
     }
     p.res = s->d.resolution;
+    p.inv_res = 1.0 / s->d.resolution;
     p.hf = static_cast<float>(s->d.resolution);
     p.inv_h = static_cast<float>(1.0 / s->d.resolution);
     p.dtf = static_cast<float>(s->dt);

@@ -129,11 +129,12 @@ __global__ __launch_bounds__(256) void k_update(double* __restrict__ omega, cons
 // k = 0, 1, ..., K-1 from 0.0, rounded to float at the end.  The w_k are
 // uniform across the wave and come through scalar loads.  k_reconstruct_window
 // runs the same chain per face and frame, so both give the same bits.
-template <typename T>
+// FP32 Phi: one face per lane (measured faster than 2 or 4 here), 16 modes in flight
+constexpr int kGemvVecF32 = 1;
+template <typename T, int VEC>
 __global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
                                                      const double* __restrict__ omega, float* __restrict__ u)
 {
-    constexpr int VEC = 16 / sizeof(T);
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const size_t fw = (static_cast<size_t>(blockIdx.x) * 4 + wv) * (kWave * VEC);
@@ -143,15 +144,23 @@ __global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, 
 #pragma unroll
     for (int j = 0; j < VEC; ++j) acc[j] = 0.0;
     const T* base = Phi + f0;
-#pragma unroll 8
+#pragma unroll 16
     for (int k = 0; k < K; ++k) {
         const double w = omega[k];
-        if constexpr (VEC == 4) {
+        if constexpr (sizeof(T) == 4 && VEC == 4) {
             const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(base + static_cast<size_t>(k) * ldF));
             acc[0] = __builtin_fma(static_cast<double>(v.x), w, acc[0]);
             acc[1] = __builtin_fma(static_cast<double>(v.y), w, acc[1]);
             acc[2] = __builtin_fma(static_cast<double>(v.z), w, acc[2]);
             acc[3] = __builtin_fma(static_cast<double>(v.w), w, acc[3]);
+        } else if constexpr (sizeof(T) == 4 && VEC == 2) {
+            typedef float f32x2 __attribute__((ext_vector_type(2)));
+            const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(base + static_cast<size_t>(k) * ldF));
+            acc[0] = __builtin_fma(static_cast<double>(v.x), w, acc[0]);
+            acc[1] = __builtin_fma(static_cast<double>(v.y), w, acc[1]);
+        } else if constexpr (VEC == 1) {
+            const T v = __builtin_nontemporal_load(base + static_cast<size_t>(k) * ldF);
+            acc[0] = __builtin_fma(static_cast<double>(v), w, acc[0]);
         } else {
             const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(base + static_cast<size_t>(k) * ldF));
             acc[0] = __builtin_fma(v.x, w, acc[0]);
@@ -279,6 +288,23 @@ __device__ __forceinline__ float mixf(float x, float y, float a)
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// int(a / b) with a / b rounded to double first, as the CPU path computes it,
+// from rb = RN(1/b): q = RN(a * rb) is within 3 * 2^-53 |q| of RN(a / b), so
+// when q lies farther than 2^-40 max(|q|, 1) from every integer, no integer
+// separates the two and they truncate alike.  Otherwise (particles within
+// ~1e-12 cells of a voxel plane, or NaN) the IEEE division decides.
+__device__ __forceinline__ int trunc_div(double a, double b, double rb)
+{
+    const double q = a * rb;
+    if (__builtin_fabs(q - __builtin_rint(q)) > 0x1p-40 * __builtin_fmax(__builtin_fabs(q), 1.0))
+        return static_cast<int>(q);
+    return static_cast<int>(a / b);
+}
+
+// two consecutive face fluxes u[i], u[i + 1] in one 4-byte-aligned load (i + 1 < F
+// always holds for the faces paired below)
+typedef float FluxPair __attribute__((ext_vector_type(2))) __attribute__((aligned(4)));
+
 __device__ __forceinline__ uint32_t respawn_draw(uint32_t seed, uint32_t step, uint32_t particle, uint32_t k)
 {
     uint64_t z = (static_cast<uint64_t>(seed) << 32) ^ (static_cast<uint64_t>(step) * 0x9E3779B97F4A7C15ull) ^
@@ -302,16 +328,16 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
         const float sx = (P.aabb_min[0] + P.ext[0]) + p.x;
         const float sy = (P.aabb_min[1] + P.ext[1]) + p.y;
         const float sz = (P.aabb_min[2] + P.ext[2]) + p.z;
-        const double res = P.res, hres = P.res / 2;
-        int yO = static_cast<int>(static_cast<double>(sy) / res);
-        int xO = static_cast<int>(static_cast<double>(sx) / res);
-        int zO = static_cast<int>(static_cast<double>(sz) / res);
-        int zM = static_cast<int>((static_cast<double>(sz) - hres) / res);
-        int zP = static_cast<int>((static_cast<double>(sz) + hres) / res);
-        int yM = static_cast<int>((static_cast<double>(sy) - hres) / res);
-        int yP = static_cast<int>((static_cast<double>(sy) + hres) / res);
-        int xM = static_cast<int>((static_cast<double>(sx) - hres) / res);
-        int xP = static_cast<int>((static_cast<double>(sx) + hres) / res);
+        const double res = P.res, hres = P.res / 2, rr = P.inv_res;
+        int yO = trunc_div(static_cast<double>(sy), res, rr);
+        int xO = trunc_div(static_cast<double>(sx), res, rr);
+        int zO = trunc_div(static_cast<double>(sz), res, rr);
+        int zM = trunc_div(static_cast<double>(sz) - hres, res, rr);
+        int zP = trunc_div(static_cast<double>(sz) + hres, res, rr);
+        int yM = trunc_div(static_cast<double>(sy) - hres, res, rr);
+        int yP = trunc_div(static_cast<double>(sy) + hres, res, rr);
+        int xM = trunc_div(static_cast<double>(sx) - hres, res, rr);
+        int xP = trunc_div(static_cast<double>(sx) + hres, res, rr);
         xO = clampi(xO, 0, nx - 1); yO = clampi(yO, 0, ny - 1); zO = clampi(zO, 0, nz - 1);
         xM = clampi(xM, 0, nx - 1); xP = clampi(xP, 0, nx - 1);
         yM = clampi(yM, 0, ny - 1); yP = clampi(yP, 0, ny - 1);
@@ -330,24 +356,36 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
         auto xf = [&](int x, int y, int z) {
             return static_cast<uint32_t>(z) * (P.ny * (P.nx + 1)) + static_cast<uint32_t>(y) * (P.nx + 1) + x + P.nZF + P.nYF;
         };
+        // Faces of neighbouring x are adjacent in every block of the layout, so
+        // each (x, x+1) couple comes in one load: the x faces of voxel xO, and
+        // the y and z faces of columns xM and xP (xP = xM + 1 but at clamps).
+        auto pair = [&](uint32_t i) { return *reinterpret_cast<gptr<const FluxPair>>(u + i); };
+        const int dx = xP - xM;
+        auto second = [&](const FluxPair& q, uint32_t i) { return dx == 1 ? q.y : (dx == 0 ? q.x : u[i + dx]); };
         // x component: voxels (xO, yM|yP, zM|zP), faces f5 (left) / f6 (right)
         const float s4 = face_sign(4, xO, 0, 0), s5 = face_sign(5, xO, 0, 0);
-        const float vel1x = -s4 * u[xf(xO, yM, zM)], vel3x = s5 * u[xf(xO + 1, yM, zM)];
-        const float vel2x = -s4 * u[xf(xO, yP, zM)], vel4x = s5 * u[xf(xO + 1, yP, zM)];
-        const float vel5x = -s4 * u[xf(xO, yM, zP)], vel7x = s5 * u[xf(xO + 1, yM, zP)];
-        const float vel6x = -s4 * u[xf(xO, yP, zP)], vel8x = s5 * u[xf(xO + 1, yP, zP)];
+        const FluxPair x1 = pair(xf(xO, yM, zM)), x2 = pair(xf(xO, yP, zM));
+        const FluxPair x3 = pair(xf(xO, yM, zP)), x4 = pair(xf(xO, yP, zP));
+        const float vel1x = -s4 * x1.x, vel3x = s5 * x1.y;
+        const float vel2x = -s4 * x2.x, vel4x = s5 * x2.y;
+        const float vel5x = -s4 * x3.x, vel7x = s5 * x3.y;
+        const float vel6x = -s4 * x4.x, vel8x = s5 * x4.y;
         // y component: voxels (xM|xP, yO, zM|zP), faces f3 (bottom) / f4 (top)
         const float s2 = face_sign(2, 0, yO, 0), s3 = face_sign(3, 0, yO, 0);
-        const float vel1y = -s2 * u[yf(xM, yO, zM)], vel2y = s3 * u[yf(xM, yO + 1, zM)];
-        const float vel3y = -s2 * u[yf(xP, yO, zM)], vel4y = s3 * u[yf(xP, yO + 1, zM)];
-        const float vel5y = -s2 * u[yf(xM, yO, zP)], vel6y = s3 * u[yf(xM, yO + 1, zP)];
-        const float vel7y = -s2 * u[yf(xP, yO, zP)], vel8y = s3 * u[yf(xP, yO + 1, zP)];
+        const uint32_t iy1 = yf(xM, yO, zM), iy2 = yf(xM, yO + 1, zM), iy3 = yf(xM, yO, zP), iy4 = yf(xM, yO + 1, zP);
+        const FluxPair y1 = pair(iy1), y2 = pair(iy2), y3 = pair(iy3), y4 = pair(iy4);
+        const float vel1y = -s2 * y1.x, vel2y = s3 * y2.x;
+        const float vel3y = -s2 * second(y1, iy1), vel4y = s3 * second(y2, iy2);
+        const float vel5y = -s2 * y3.x, vel6y = s3 * y4.x;
+        const float vel7y = -s2 * second(y3, iy3), vel8y = s3 * second(y4, iy4);
         // z component: voxels (xM|xP, yM|yP, zO), faces f1 (front) / f2 (back)
         const float s0 = face_sign(0, 0, 0, zO), s1 = face_sign(1, 0, 0, zO);
-        const float vel1z = s0 * u[zf(xM, yM, zO)], vel2z = s0 * u[zf(xM, yP, zO)];
-        const float vel3z = s0 * u[zf(xP, yM, zO)], vel4z = s0 * u[zf(xP, yP, zO)];
-        const float vel5z = -s1 * u[zf(xM, yM, zO + 1)], vel6z = -s1 * u[zf(xM, yP, zO + 1)];
-        const float vel7z = -s1 * u[zf(xP, yM, zO + 1)], vel8z = -s1 * u[zf(xP, yP, zO + 1)];
+        const uint32_t iz1 = zf(xM, yM, zO), iz2 = zf(xM, yP, zO), iz3 = zf(xM, yM, zO + 1), iz4 = zf(xM, yP, zO + 1);
+        const FluxPair z1 = pair(iz1), z2 = pair(iz2), z3 = pair(iz3), z4 = pair(iz4);
+        const float vel1z = s0 * z1.x, vel2z = s0 * z2.x;
+        const float vel3z = s0 * second(z1, iz1), vel4z = s0 * second(z2, iz2);
+        const float vel5z = -s1 * z3.x, vel6z = -s1 * z4.x;
+        const float vel7z = -s1 * second(z3, iz3), vel8z = -s1 * second(z4, iz4);
 
         const float hf = P.hf;
         float cx0 = (P.aabb_min[0] + hf * static_cast<float>(xO)) + 0.5f * 0.0f;
@@ -474,11 +512,12 @@ hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F,
                               float* u, hipStream_t s)
 {
     if (prec == PSF_PHI_FP32) {
-        const unsigned grid = static_cast<unsigned>((ldF + 256 * 4 - 1) / (256 * 4));
-        k_reconstruct<float><<<grid, 256, 0, s>>>(static_cast<const float*>(Phi), ldF, F, K, omega, u);
+        constexpr int V = kGemvVecF32;
+        const unsigned grid = static_cast<unsigned>((ldF + 256 * V - 1) / (256 * V));
+        k_reconstruct<float, V><<<grid, 256, 0, s>>>(static_cast<const float*>(Phi), ldF, F, K, omega, u);
     } else {
         const unsigned grid = static_cast<unsigned>((ldF + 256 * 2 - 1) / (256 * 2));
-        k_reconstruct<double><<<grid, 256, 0, s>>>(static_cast<const double*>(Phi), ldF, F, K, omega, u);
+        k_reconstruct<double, 2><<<grid, 256, 0, s>>>(static_cast<const double*>(Phi), ldF, F, K, omega, u);
     }
     return hipGetLastError();
 }

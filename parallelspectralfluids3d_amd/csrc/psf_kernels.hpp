@@ -10,6 +10,7 @@ struct TraceParams {
     float aabb_min[3], aabb_max[3];
     float ext[3];               // aabb_max - aabb_min (AABB::getExtent, float)
     double res;                 // AbstractSolver::resolution
+    double inv_res;             // RN(1 / res)
     float hf;                   // (float)resolution
     float inv_h;                // float(1.0 / resolution)
     float dtf;                  // float(timeStep)

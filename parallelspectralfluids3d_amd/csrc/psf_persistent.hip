@@ -60,6 +60,7 @@ constexpr int kMaxIpw = 8;       // modes per workgroup
 constexpr int kMaxCUs = 256;
 constexpr uint64_t kWaitTicks = 200000000ull;   // s_memrealtime runs at 100 MHz: 2 s
 constexpr int kXStride = 16;     // 64-bit words per exchange record: one 128-byte line per mode
+constexpr int kBChunk = 6;       // B-operand blocks loaded ahead of the MFMA chain
 
 struct PersistentState {
     int G = 0, ipw = 0, Kp = 0, nb = 0, nBi = 0, bpw = 0;
@@ -286,26 +287,39 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
         const uint32_t tag = a.tag0 + static_cast<uint32_t>(s) + 1u;
         // ---- contraction of this wave's blocks -------------------------------
         {
-            uint4 breg[RB];
+            // B operands come from LDS in chunks of kBChunk blocks, the next
+            // chunk loading while the current one feeds the MFMAs
+            uint4 bq[2][kBChunk];
 #pragma unroll
-            for (int j = 0; j < RB; ++j)
-                breg[j] = b_operand(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mreg), j)));
+            for (int jj = 0; jj < kBChunk && jj < RB; ++jj)
+                bq[0][jj] = b_operand(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mreg), jj)));
             const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
             f32x4 acc0 = zero4, acc1 = zero4;
             double part = 0.0;
 #pragma unroll
-            for (int j = 0; j < RB; ++j) {
-                if (j & 1) acc1 = mfma16<ELEM>(areg[j], breg[j], acc1);
-                else acc0 = mfma16<ELEM>(areg[j], breg[j], acc0);
-                if ((fmask >> j) & 1ull) {
-                    const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mreg), j));
-                    part += row_contract(acc0 + acc1, static_cast<int>((m >> 16) & 255u));
-                    acc0 = zero4;
-                    acc1 = zero4;
-                    if ((tmask >> j) & 1ull) {
-                        const double v = wave_total(part);
-                        if (lane == 0) s_ypart[wv][m >> 24] += v;
-                        part = 0.0;
+            for (int c = 0; c < RB; c += kBChunk) {
+                const int cb = (c / kBChunk) & 1;
+#pragma unroll
+                for (int jj = 0; jj < kBChunk; ++jj)
+                    if (c + kBChunk + jj < RB)
+                        bq[cb ^ 1][jj] = b_operand(
+                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mreg), c + kBChunk + jj)));
+#pragma unroll
+                for (int jj = 0; jj < kBChunk; ++jj) {
+                    const int j = c + jj;
+                    if (j >= RB) break;
+                    if (j & 1) acc1 = mfma16<ELEM>(areg[j], bq[cb][jj], acc1);
+                    else acc0 = mfma16<ELEM>(areg[j], bq[cb][jj], acc0);
+                    if ((fmask >> j) & 1ull) {
+                        const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mreg), j));
+                        part += row_contract(acc0 + acc1, static_cast<int>((m >> 16) & 255u));
+                        acc0 = zero4;
+                        acc1 = zero4;
+                        if ((tmask >> j) & 1ull) {
+                            const double v = wave_total(part);
+                            if (lane == 0) s_ypart[wv][m >> 24] += v;
+                            part = 0.0;
+                        }
                     }
                 }
             }
@@ -426,7 +440,7 @@ __global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
 // Each scene's workgroups exchange only among themselves (own xbuf, tags,
 // step count), so a scene gives the same bits alone and in a batch.
 template <int ELEM, int RB>
-__global__ __launch_bounds__(256, 1) void k_persistent_multi(const PArgs* __restrict__ table,
+__global__ __launch_bounds__(256, 2) void k_persistent_multi(const PArgs* __restrict__ table,
                                                              const uint32_t* __restrict__ wg_map)
 {
     const uint32_t m = wg_map[blockIdx.x];

@@ -377,9 +377,11 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
     const Clock::time_point t0 = Clock::now();
     if (completed) *completed = 0;
     SRV_HIPCHK(ctx, hipSetDevice(ctx->device));
-    admit_queued(sv);
-    const Clock::time_point t1 = Clock::now();
-    sv->st.admit_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    // with nothing on the device, admit first so that this call launches work
+    if (sv->active.empty()) {
+        admit_queued(sv);
+        sv->st.admit_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    }
 
     // ---- launch the next round for every scene with frames left ----------------
     std::vector<Request*> run;
@@ -439,6 +441,13 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
         }
         sv->launched++;
         sv->st.rounds++;
+    }
+
+    // ---- admit while the device runs the round; admitted scenes join the next ------
+    {
+        const Clock::time_point ta = Clock::now();
+        admit_queued(sv);
+        sv->st.admit_ms += std::chrono::duration<double, std::milli>(Clock::now() - ta).count();
     }
 
     // ---- hand out the requests of the round before (all rounds when idle) ----------
