@@ -118,6 +118,15 @@ hipError_t salloc(psf_scene* s, T** p, size_t n)
 
 void sfree(psf_scene* s, void* p) { scene_free(s, p); }
 
+void release_side(psf_scene* s)
+{
+    if (s->side) (void)hipStreamDestroy(s->side);
+    for (int q = 0; q < 2; ++q)
+        if (s->ev_ready[q]) (void)hipEventDestroy(s->ev_ready[q]);
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    s->side = nullptr;
+}
+
 size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
 bool is_plain_precision(int p) { return p == PSF_C_FP64 || p == PSF_C_FP32; }
@@ -200,10 +209,10 @@ int ready_for_step(psf_scene* s)
     return PSF_OK;
 }
 
-int enqueue_steps(psf_scene* s, int n, double* hist = nullptr, int hist_ld = 0)
+int enqueue_steps(psf_scene* s, int n, double* hist = nullptr, int hist_ld = 0, hipStream_t st = nullptr)
 {
     const int K = static_cast<int>(s->d.K);
-    hipStream_t st = s->ctx->stream;
+    if (!st) st = s->ctx->stream;
     if (s->path == PSF_PATH_PERSISTENT) {
         hipError_t e = launch_persistent_steps(s, n, st, hist, hist_ld);
         if (e != hipSuccess) return set_err(s->ctx, PSF_ERR_HIP, "persistent step launch: %s", hipGetErrorString(e));
@@ -219,10 +228,14 @@ int enqueue_steps(psf_scene* s, int n, double* hist = nullptr, int hist_ld = 0)
     return PSF_OK;
 }
 
+// buffers of the frame window: 0 is dev.hist / dev.u_hist, 1 dev.hist_b / dev.u_hist_b
+double* win_hist(psf_scene* s, uint32_t b) { return b ? s->dev.hist_b : s->dev.hist; }
+float* win_u(psf_scene* s, uint32_t b) { return b ? s->dev.u_hist_b : s->dev.u_hist; }
+
 // u of the current frame
 float* current_u(psf_scene* s)
 {
-    return s->win_cursor > 0 ? s->dev.u_hist + size_t(s->win_cursor - 1) * s->dev.ldF : s->dev.u;
+    return s->win_cursor > 0 ? win_u(s, s->win_buf) + size_t(s->win_cursor - 1) * s->dev.ldF : s->dev.u;
 }
 
 int enqueue_reconstruct(psf_scene* s)
@@ -240,28 +253,46 @@ int enqueue_trace(psf_scene* s, int32_t* dcells)
     return PSF_OK;
 }
 
+// W reduced steps from dev.omega into buffer b's history, then the W velocity
+// fields into buffer b's u rows, on stream st
+int enqueue_window_fill(psf_scene* s, uint32_t b, hipStream_t st)
+{
+    const int W = static_cast<int>(s->window);
+    int rc = enqueue_steps(s, W, win_hist(s, b), W, st);
+    if (rc) return rc;
+    HIPCHK(s->ctx, launch_reconstruct_window(s->d.phi_precision, W, s->dev.Phi, s->dev.ldF, s->F,
+                                             static_cast<int>(s->d.K), win_hist(s, b), win_u(s, b), st));
+    return PSF_OK;
+}
+
 // Ends a frame window early: w and u of the current frame become dev.omega and
 // dev.u again, and the frames computed ahead are dropped.  Every call that
 // changes the scene's state or runs a single phase calls this first.
 int window_flush(psf_scene* s)
 {
+    hipStream_t m = s->ctx->stream;
+    if (s->win_ahead) {   // the side stream's window also moved dev.omega: wait for it
+        HIPCHK(s->ctx, hipStreamWaitEvent(m, s->ev_ready[s->win_buf ^ 1u], 0));
+        s->win_ahead = false;
+    }
     if (s->win_filled == 0) return PSF_OK;
     const int t = static_cast<int>(s->win_cursor) - 1;
     s->win_filled = 0;
     s->win_cursor = 0;
     if (t < 0) return PSF_OK;
-    HIPCHK(s->ctx, launch_gather_column(s->dev.hist, static_cast<int>(s->window), t, static_cast<int>(s->d.K),
-                                        s->dev.omega, s->ctx->stream));
-    HIPCHK(s->ctx, hipMemcpyAsync(s->dev.u, s->dev.u_hist + size_t(t) * s->dev.ldF, size_t(s->F) * 4,
-                                  hipMemcpyDeviceToDevice, s->ctx->stream));
+    HIPCHK(s->ctx, launch_gather_column(win_hist(s, s->win_buf), static_cast<int>(s->window), t,
+                                        static_cast<int>(s->d.K), s->dev.omega, m));
+    HIPCHK(s->ctx, hipMemcpyAsync(s->dev.u, win_u(s, s->win_buf) + size_t(t) * s->dev.ldF, size_t(s->F) * 4,
+                                  hipMemcpyDeviceToDevice, m));
     return PSF_OK;
 }
 
 // One integrate() frame: reduced step, u = Phi w, particle pass
 // (psfsolver.cpp:21-234).  With a frame window of W > 1 the reduced steps and
-// the reconstructions of W frames are computed together when the window runs
-// empty (the reduced dynamics do not depend on the particles), and each frame
-// then only traces with its own u.
+// the reconstructions of W frames are computed together (the reduced dynamics
+// do not depend on the particles), and each frame then only traces with its
+// own u.  With the second buffer, the next window is computed on the side
+// stream while the frames of the current one trace on the context stream.
 int enqueue_frame(psf_scene* s)
 {
     int rc;
@@ -270,15 +301,29 @@ int enqueue_frame(psf_scene* s)
         if ((rc = enqueue_reconstruct(s))) return rc;
         return enqueue_trace(s, nullptr);
     }
+    hipStream_t m = s->ctx->stream;
     if (s->win_cursor == s->win_filled) {
-        const int W = static_cast<int>(s->window);
+        uint32_t next = s->win_buf;
+        if (s->win_filled > 0 && s->win_pipeline) next = s->win_buf ^ 1u;
         s->win_filled = 0;
         s->win_cursor = 0;
-        if ((rc = enqueue_steps(s, W, s->dev.hist, W))) return rc;
-        HIPCHK(s->ctx, launch_reconstruct_window(s->d.phi_precision, W, s->dev.Phi, s->dev.ldF, s->F,
-                                                 static_cast<int>(s->d.K), s->dev.hist, s->dev.u_hist,
-                                                 s->ctx->stream));
+        if (s->win_ahead) {
+            HIPCHK(s->ctx, hipStreamWaitEvent(m, s->ev_ready[next], 0));
+            s->win_ahead = false;
+        } else if ((rc = enqueue_window_fill(s, next, m))) {
+            return rc;
+        }
+        s->win_buf = next;
         s->win_filled = s->window;
+        if (s->win_pipeline) {
+            // the window after this one, on the side stream, behind everything
+            // queued so far on the context stream (omega, the buffer's last traces)
+            HIPCHK(s->ctx, hipEventRecord(s->ev_fork, m));
+            HIPCHK(s->ctx, hipStreamWaitEvent(s->side, s->ev_fork, 0));
+            if ((rc = enqueue_window_fill(s, next ^ 1u, s->side))) return rc;
+            HIPCHK(s->ctx, hipEventRecord(s->ev_ready[next ^ 1u], s->side));
+            s->win_ahead = true;
+        }
     }
     s->win_cursor++;
     return enqueue_trace(s, nullptr);
@@ -423,6 +468,7 @@ int psf::scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOp
     s->host = h;
     s->pooled = opt.pooled;
     s->modes_per_wg = opt.modes_per_wg;
+    s->win_pipeline = opt.frame_pipeline;
     counts(*desc, s->V, s->F, s->nZF, s->nYF, s->nXF);
     s->path = is_plain_precision(desc->c_precision) ? PSF_PATH_PLAIN : PSF_PATH_PERSISTENT;
     const size_t K = desc->K;
@@ -483,6 +529,7 @@ int psf_scene_destroy(psf_scene* s)
     if (!s) return PSF_ERR_ARG;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
+    if (s->side) (void)hipStreamSynchronize(s->side);
     scene_release(s);
     return PSF_OK;
 }
@@ -499,6 +546,8 @@ void psf::scene_release(psf_scene* s)
     if (!s->phi_borrowed) sfree(s, s->dev.Phi);
     sfree(s, s->dev.u); sfree(s, s->dev.parts); sfree(s, s->dev.cells); sfree(s, s->dev.pid);
     sfree(s, s->dev.hist); sfree(s, s->dev.u_hist); sfree(s, s->dev.omega_view);
+    sfree(s, s->dev.hist_b); sfree(s, s->dev.u_hist_b);
+    release_side(s);
     s->ctx->scenes.erase(s);
     delete host_of(s);
     delete s;
@@ -643,17 +692,33 @@ int psf_set_frame_window(psf_scene* s, uint32_t frames)
     if (int rc0 = window_flush(s)) return rc0;
     if (frames == s->window) return PSF_OK;
     HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
-    sfree(s, s->dev.hist); sfree(s, s->dev.u_hist); sfree(s, s->dev.omega_view);
-    s->dev.hist = nullptr; s->dev.u_hist = nullptr; s->dev.omega_view = nullptr;
+    if (s->side) HIPCHK(s->ctx, hipStreamSynchronize(s->side));
+    auto drop = [&]() {
+        sfree(s, s->dev.hist); sfree(s, s->dev.u_hist); sfree(s, s->dev.omega_view);
+        sfree(s, s->dev.hist_b); sfree(s, s->dev.u_hist_b);
+        s->dev.hist = nullptr; s->dev.u_hist = nullptr; s->dev.omega_view = nullptr;
+        s->dev.hist_b = nullptr; s->dev.u_hist_b = nullptr;
+    };
+    drop();
     s->window = 1;
+    s->win_buf = 0;
     if (frames == 1) return PSF_OK;
     const size_t K = s->d.K;
     hipError_t e = salloc(s, &s->dev.hist, K * frames);
     if (e == hipSuccess) e = salloc(s, &s->dev.u_hist, s->dev.ldF * frames);
     if (e == hipSuccess) e = salloc(s, &s->dev.omega_view, K);
+    if (s->win_pipeline) {
+        if (e == hipSuccess) e = salloc(s, &s->dev.hist_b, K * frames);
+        if (e == hipSuccess) e = salloc(s, &s->dev.u_hist_b, s->dev.ldF * frames);
+        if (e == hipSuccess && !s->side) {
+            e = hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking);
+            for (int q = 0; q < 2 && e == hipSuccess; ++q)
+                e = hipEventCreateWithFlags(&s->ev_ready[q], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming);
+        }
+    }
     if (e != hipSuccess) {
-        sfree(s, s->dev.hist); sfree(s, s->dev.u_hist); sfree(s, s->dev.omega_view);
-        s->dev.hist = nullptr; s->dev.u_hist = nullptr; s->dev.omega_view = nullptr;
+        drop();
         return set_err(s->ctx, e == hipErrorOutOfMemory ? PSF_ERR_NOMEM : PSF_ERR_HIP, "frame window buffers: %s",
                        hipGetErrorString(e));
     }
@@ -679,8 +744,9 @@ int psf_get_omega(psf_scene* s, double* omega)
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
     const double* src = s->dev.omega;
     if (s->win_cursor > 0) {
-        HIPCHK(s->ctx, launch_gather_column(s->dev.hist, static_cast<int>(s->window), static_cast<int>(s->win_cursor) - 1,
-                                            static_cast<int>(s->d.K), s->dev.omega_view, s->ctx->stream));
+        HIPCHK(s->ctx, launch_gather_column(win_hist(s, s->win_buf), static_cast<int>(s->window),
+                                            static_cast<int>(s->win_cursor) - 1, static_cast<int>(s->d.K),
+                                            s->dev.omega_view, s->ctx->stream));
         src = s->dev.omega_view;
     }
     HIPCHK(s->ctx, hipMemcpyAsync(omega, src, s->d.K * 8, hipMemcpyDeviceToHost, s->ctx->stream));
@@ -809,7 +875,7 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
     if ((phase == 1 || phase == 3 || phase == 5) && !s->has_phi) return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
     psf_ctx* ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    if (phase == 0 || phase == 1 || phase == 4) {
+    if (phase == 0 || phase == 1 || phase == 4 || phase == 5) {
         int rc0 = window_flush(s);
         if (rc0) return rc0;
     }
@@ -829,7 +895,7 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
             for (int i = 0; i < count && rc == PSF_OK; ++i) {
                 hipError_t le = launch_reconstruct_window(s->d.phi_precision, static_cast<int>(s->window), s->dev.Phi,
                                                           s->dev.ldF, s->F, static_cast<int>(s->d.K), s->dev.hist,
-                                                          s->dev.u_hist, ctx->stream);
+                                                          s->dev.u_hist, ctx->stream);   // buffer 0, after the flush
                 if (le != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "window reconstruction: %s", hipGetErrorString(le));
             }
             break;

@@ -39,6 +39,10 @@ struct DeviceScene {
     double* hist = nullptr;
     float* u_hist = nullptr;
     double* omega_view = nullptr;
+    // the second buffer pair: the next window is computed into it while the
+    // current one is traced
+    double* hist_b = nullptr;
+    float* u_hist_b = nullptr;
 };
 
 }  // namespace psf
@@ -74,6 +78,12 @@ struct psf_scene {
     // while filled > 0, the current w and u are column / row cursor-1 of
     // dev.hist / dev.u_hist and dev.omega holds w at the end of the window
     uint32_t window = 1, win_filled = 0, win_cursor = 0;
+    uint32_t win_buf = 0;        // buffer of the frames being consumed (0: hist/u_hist, 1: hist_b/u_hist_b)
+    bool win_pipeline = true;    // compute the next window on the side stream during this one
+    bool win_ahead = false;      // the next window is queued on the side stream into the other buffer
+    hipStream_t side = nullptr;
+    hipEvent_t ev_ready[2] = {nullptr, nullptr};   // side stream: buffer filled
+    hipEvent_t ev_fork = nullptr;                  // context stream position the side stream starts from
     uint32_t seed = 0;
     void* host = nullptr;        // host-side copies (psf_scene_host)
     void* persistent = nullptr;  // persistent-path state (PersistentState)

@@ -13,6 +13,7 @@ struct SceneOptions {
     void* borrowed_phi = nullptr;  // use this Phi (F x K' column-major, K' >= K, leading dim ldF) instead of a copy
     bool cells = true;             // allocate the cell-lookup debug buffer
     bool pooled = false;           // take device memory from the context's pool
+    bool frame_pipeline = true;    // frame windows keep a second buffer pair and a side stream
 };
 
 int scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOptions& opt, psf_scene** out);

@@ -144,6 +144,7 @@ int admit(psf_server* sv, Request* r)
     o.borrowed_phi = b.Phi;
     o.cells = false;
     o.pooled = true;
+    o.frame_pipeline = false;   // the server pipelines rounds itself
     psf_scene* s = nullptr;
     int rc = scene_create_ex(sv->ctx, &d, o, &s);
     if (rc) return rc;

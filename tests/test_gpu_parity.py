@@ -158,3 +158,39 @@ def test_check_signs_and_errors(hip_ctx):
     s2.set_params(0.01)
     s2.step(3)
     assert np.all(np.isfinite(s2.get_omega()))
+
+
+@pytest.mark.parametrize("case", [dict(n=8, K=16), dict(n=12, K=16)])
+def test_trace_on_voxel_planes(hip_ctx, case):
+    """Particles on voxel planes, half-cell planes and the AABB walls, and a few
+    float ulps either side: the cell index math (double quotient truncated,
+    psfsolver.cpp:80-100) and the clamped neighbour columns match the oracle
+    bit for bit.  n=12 makes h = 1/12 inexact in binary."""
+    _say(f"trace on voxel planes {case}")
+    sd = make_scene(**case)
+    n = sd.grid.nx
+    h = sd.resolution
+    rng = np.random.default_rng(7)
+    planes = np.concatenate([np.arange(n + 1) * h, (np.arange(n) + 0.5) * h]) + sd.aabb_min[0]
+    N = 4096
+    p0 = np.zeros((N, 4), np.float32)
+    for a in range(3):
+        c = rng.choice(planes, N).astype(np.float32)
+        steps = rng.integers(-3, 4, N)
+        for j in range(N):
+            for _ in range(abs(int(steps[j]))):
+                c[j] = np.nextafter(c[j], np.float32(np.sign(steps[j]) * 10.0))
+        p0[:, a] = np.clip(c, sd.aabb_min[a], sd.aabb_max[a])
+    p0[:, 3] = 60.0
+    s = _scene(hip_ctx, sd, max_particles=N)
+    s.set_params(0.01, 0.0, False, lifetime=60.0)
+    s.set_particles(p0)
+    s.reconstruct()
+    u = s.get_velocity()
+    cells = s.trace(cells=True)
+    pg = s.get_particles()
+    m = O.Mesh((sd.grid.nx, sd.grid.ny, sd.grid.nz), sd.resolution, sd.aabb_min, inside=sd.inside)
+    pr, cr = O.particles(m, u, p0, resolution=sd.resolution, timestep=0.01, aabb_min=sd.aabb_min,
+                         aabb_max=sd.aabb_max, lifetime=60.0, seed=0, step=0, cells=True)
+    assert np.array_equal(cells, cr)
+    assert np.array_equal(pg.view(np.uint32), pr.view(np.uint32))
