@@ -36,14 +36,13 @@ sys.path.insert(0, ROOT)
 METRIC = "reduced steps/s + particle-steps/s at 64³,K=256; p99 latency; err vs FP64"
 HBM_PEAK = 8.0e12
 FP64_PEAK = 78.6e12
-# HBM bytes per k_trace launch at 64^3 with 2^20 particles, measured the same
-# way (14,100 KiB fetched x 2 plus 16,384 KiB written)
-TRACE_TRAFFIC_64_1M = 45_654_016
-# HBM bytes per k_reconstruct<float> launch at 64^3/K=256, measured with
+# HBM bytes per launch at 64^3/K=256 with 2^20 particles, measured with
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --counter-pass`
-# (399,390 KiB fetched x 2, the factor a 1 GiB hipMemcpy reads back, plus
-# 3,120.1 KiB written); counters cannot be read from inside this process.
-RECON_TRAFFIC_64_256 = 821_145_723
+# (fetched KiB x 2, the factor a 1 GiB hipMemcpy reads back, plus written
+# KiB); counters cannot be read from inside this process.
+RECON_TRAFFIC_64_256 = 821_147_648          # k_reconstruct<float, 1>: 399,392 KiB read, 3,120 KiB written
+WINDOW_TRAFFIC_64_256_W32 = 922_556_416     # k_reconstruct_window<float, 2, 32>: 400,547 / 99,840 KiB
+TRACE_TRAFFIC_64_1M = 46_614_938            # k_trace: 14,569 / 16,384 KiB
 
 
 def log(msg):
@@ -286,6 +285,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rate", type=float, default=200.0)
+    ap.add_argument("--serving-profile", type=int, default=0, metavar="N",
+                    help="only run the first N requests of the serving stream, all queued at once, through the "
+                         "batched server (a short run for a profiler); prints one line and exits")
     ap.add_argument("--counter-pass", action="store_true",
                     help="short run for hardware-counter collection: the frame loop, one persistent launch of "
                          "--reduced-steps steps, then three plain 1 GiB device copies that calibrate the byte "
@@ -298,6 +300,15 @@ def main():
 
     from parallelspectralfluids3d_amd import _lib as L
     from parallelspectralfluids3d_amd import hip
+
+    if args.serving_profile:
+        ctx = hip.Context(0)
+        bases = serving_bases()
+        reqs = serving_workload(args.serving_profile, args.serving_rate)
+        r = run_stream(ctx, bases, reqs, max_active=64, saturate=True)
+        ctx.close()
+        print(json.dumps({"serving_profile": args.serving_profile, **r}), flush=True)
+        return
 
     n, K, N = args.grid, args.modes, args.particles
     sc = build_scene(n, K, N)
@@ -364,7 +375,7 @@ def main():
                                             achieved=wflops / (ms_wrec * 1e-3) / 1e12,
                                             algorithmic_flops_per_launch=wflops,
                                             algorithmic_bytes_per_launch=F * K * 4 + W * (F * 4 + K * 8),
-                                            traffic=None)
+                                            traffic=WINDOW_TRAFFIC_64_256_W32 if (n, K, W) == (64, 256, 32) else None)
         cbytes = K * K * (K + 1) // 2 * 4 + 32 * K
         roof["k_persistent"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
                                     achieved=cbytes / (kern["k_persistent"] * 1e-3) / 1e9,
