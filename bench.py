@@ -113,6 +113,15 @@ def cpu_baseline(sc, budget_s=12.0):
     timed = frames - 1
     tot = t_red + t_rec + t_par
     N = p.shape[0]
+    # one thread, a few frames of the same workload
+    t1_frames, t1_tot = 3, 0.0
+    for f in range(t1_frames):
+        t0 = time.perf_counter()
+        w = O.reduced_step(w, sc["C"], sc["lam"], sc["g"], 0.01, 0.0, False, nthreads=1)
+        u = O.reconstruct(Phi, w, nthreads=1)
+        p = O.particles(m, u, p, resolution=sc["h"], timestep=0.01, aabb_min=(-.5, -.5, -.5),
+                        aabb_max=(.5, .5, .5), lifetime=60.0, seed=0, step=frames + f, nthreads=1)
+        t1_tot += time.perf_counter() - t0
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as fh:
@@ -127,7 +136,12 @@ def cpu_baseline(sc, budget_s=12.0):
                       f"after 1 warm-up frame (FP64 C and Phi as in psfsolver.cpp)",
             "ms_per_frame": 1e3 * tot / timed, "ms_reduced": 1e3 * t_red / timed,
             "ms_reconstruct": 1e3 * t_rec / timed, "ms_particles": 1e3 * t_par / timed,
-            "host_cpus": os.cpu_count(), "cpu_model": cpu_model}
+            # AbstractSolver's benchmark.csv averages (abstractsolver.cpp:292-294): t0 = psfsolver.cpp:32-67
+            # (step with u = Phi w), t1 = the particle loop (psfsolver.cpp:70-231)
+            "benchmark_csv_ms": {"t0": 1e3 * (t_red + t_rec) / timed, "t1": 1e3 * t_par / timed},
+            "value_1thread": N * t1_frames / t1_tot, "ms_per_frame_1thread": 1e3 * t1_tot / t1_frames,
+            "host_cpus": os.cpu_count(), "omp_threads_env": os.environ.get("OMP_NUM_THREADS"),
+            "cpu_model": cpu_model}
 
 
 SERVE_GRIDS = (16, 24, 32, 48)
@@ -171,13 +185,13 @@ def serving_bases():
     return out
 
 
-def run_stream(ctx, bases, reqs, max_active, window=32, saturate=False):
+def run_stream(ctx, bases, reqs, max_active, window=32, saturate=False, modes_per_wg=0):
     """Feeds the requests to a server at their arrival times (all at once when
     saturate) and runs rounds until every request has finished.  Returns the
     throughput over the makespan and the latency of each request from its
     arrival."""
     from parallelspectralfluids3d_amd import hip
-    sv = hip.Server(ctx, window=window, max_active=max_active)
+    sv = hip.Server(ctx, window=window, max_active=max_active, modes_per_wg=modes_per_wg)
     bid = {n: sv.add_basis(b["grid"], b["h"], b["Phi"], b["lam"], b["g"], b["C"]) for n, b in bases.items()}
     pending = sorted(range(len(reqs)), key=lambda j: reqs[j]["arrival"])
     outstanding = {}
@@ -223,13 +237,15 @@ def serving_bench(ctx, n_requests, rate):
     bases = serving_bases()
     reqs = serving_workload(n_requests, rate)
     log(f"serving: {len(bases)} bases (K0={SERVE_K0}) and {n_requests} requests in {time.time() - t:.1f}s")
-    batched = run_stream(ctx, bases, reqs, max_active=64)
+    # batched: four modes per workgroup packs more scenes into each persistent
+    # launch; one at a time: the per-request default (the fastest steps)
+    batched = run_stream(ctx, bases, reqs, max_active=64, modes_per_wg=4)
     log(f"serving batched: {batched['scene_frames_per_s']:.0f} scene-frames/s, "
         f"p50 {batched['latency_ms_p50']:.1f} ms, p99 {batched['latency_ms_p99']:.1f} ms")
     alone = run_stream(ctx, bases, reqs, max_active=1)
     log(f"serving one at a time: {alone['scene_frames_per_s']:.0f} scene-frames/s, "
         f"p50 {alone['latency_ms_p50']:.1f} ms, p99 {alone['latency_ms_p99']:.1f} ms")
-    sat = run_stream(ctx, bases, reqs, max_active=64, saturate=True)
+    sat = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=4)
     log(f"serving batched, all queued at once: {sat['scene_frames_per_s']:.0f} scene-frames/s")
     return {
         "workload": f"{n_requests} requests, Poisson arrivals at {rate:g}/s, grids {SERVE_GRIDS}^3, K in [16, 192], "
@@ -305,7 +321,7 @@ def main():
         ctx = hip.Context(0)
         bases = serving_bases()
         reqs = serving_workload(args.serving_profile, args.serving_rate)
-        r = run_stream(ctx, bases, reqs, max_active=64, saturate=True)
+        r = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=4)
         ctx.close()
         print(json.dumps({"serving_profile": args.serving_profile, **r}), flush=True)
         return

@@ -449,7 +449,12 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
 {
     const uint32_t id = blockIdx.x * 256u + threadIdx.x;
     if (id >= n) return;
-    parts[id] = trace_one(P, as_global(u), parts[id], as_global(pid), id, cells);
+    // particle records stream through once: nontemporal, so u stays in L2
+    f32x4* rec = reinterpret_cast<f32x4*>(parts) + id;
+    const f32x4 v = __builtin_nontemporal_load(rec);
+    const float4 p = trace_one(P, as_global(u), make_float4(v.x, v.y, v.z, v.w), as_global(pid), id, cells);
+    const f32x4 o = {p.x, p.y, p.z, p.w};
+    __builtin_nontemporal_store(o, rec);
 }
 
 // Several scenes, each through nframes frames of its window: particle
@@ -465,14 +470,14 @@ __global__ __launch_bounds__(256) void k_trace_window_multi(const TraceScene* __
     const gptr<f32x4> parts = as_global(reinterpret_cast<f32x4*>(ts.parts));
     const gptr<const float> uh = as_global(ts.u_hist);
     const gptr<const uint32_t> pid = as_global(ts.pid);
-    const f32x4 v = parts[id];
+    const f32x4 v = __builtin_nontemporal_load(&parts[id]);
     float4 p = make_float4(v.x, v.y, v.z, v.w);
     for (uint32_t t = 0; t < ts.nframes; ++t) {
         p = trace_one(P, uh + static_cast<size_t>(t) * ts.ldU, p, pid, id, nullptr);
         P.frame++;
     }
     const f32x4 o = {p.x, p.y, p.z, p.w};
-    parts[id] = o;
+    __builtin_nontemporal_store(o, &parts[id]);
 }
 
 __global__ void k_zero_int(int* p, int n)

@@ -88,6 +88,9 @@ struct psf_server {
     hipEvent_t ev_step[2] = {nullptr, nullptr};   // after the persistent launch of a round
     hipEvent_t ev_done[2] = {nullptr, nullptr};   // after the particle launch of a round
     hipStream_t second = nullptr;                 // reconstruction and particles
+    hipStream_t upload = nullptr;                 // admission uploads
+    hipEvent_t ev_upload = nullptr;               // after the last admission's uploads
+    bool upload_pending = false;
     int64_t launched = 0;                         // rounds launched
     int64_t retired = -1;                         // last round whose requests were handed out
     int wg_capacity = 0, wgs_in_use = 0;
@@ -174,8 +177,20 @@ void release(psf_server* sv, Request* r)
     sv->mem_in_use -= r->footprint;
 }
 
+// The scene calls used by admit() queue their copies and kernels on the
+// context's stream; during admission that is the server's upload stream, so
+// admission never waits for a running round.  The next persistent launch
+// waits for the uploads through ev_upload.
+struct UploadStream {
+    psf_ctx* ctx;
+    hipStream_t saved;
+    UploadStream(psf_server* sv) : ctx(sv->ctx), saved(sv->ctx->stream) { ctx->stream = sv->upload; }
+    ~UploadStream() { ctx->stream = saved; }
+};
+
 void admit_queued(psf_server* sv)
 {
+    UploadStream up(sv);
     while (!sv->queue.empty() && sv->active.size() < sv->cfg.max_active) {
         Request* r = sv->queue.front();
         if (sv->wgs_in_use + r->wgs > sv->wg_capacity) break;
@@ -186,7 +201,7 @@ void admit_queued(psf_server* sv)
         sv->mem_peak = std::max(sv->mem_peak, sv->mem_in_use);
         if (admit(sv, r) != PSF_OK) {
             r->error = sv->ctx->err;
-            (void)hipStreamSynchronize(sv->ctx->stream);   // uploads already queued for the scene
+            (void)hipStreamSynchronize(sv->upload);   // uploads already queued for the scene
             r->state = kFailed;
             r->t_done = Clock::now();
             release(sv, r);
@@ -195,7 +210,9 @@ void admit_queued(psf_server* sv)
         }
         r->state = kActive;
         sv->active.push_back(r);
+        sv->upload_pending = true;
     }
+    if (sv->upload_pending) (void)hipEventRecord(sv->ev_upload, sv->upload);
 }
 
 }  // namespace
@@ -220,6 +237,8 @@ int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** o
     sv->ctx = ctx;
     sv->cfg = c;
     hipError_t e = hipStreamCreateWithFlags(&sv->second, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->ev_upload, hipEventDisableTiming);
     for (int q = 0; q < 2 && e == hipSuccess; ++q) {
         sv->batch[q] = persistent_batch_create(ctx->device, static_cast<int>(kMaxActive), 8192);
         if (!sv->batch[q]) e = hipErrorOutOfMemory;
@@ -252,6 +271,7 @@ int psf_server_destroy(psf_server* sv)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (sv->second) (void)hipStreamSynchronize(sv->second);
+    if (sv->upload) (void)hipStreamSynchronize(sv->upload);
     for (auto& kv : sv->reqs)
         if (kv.second->scene) {
             scene_free(kv.second->scene, kv.second->hist2);
@@ -271,6 +291,8 @@ int psf_server_destroy(psf_server* sv)
         if (sv->ev_done[q]) (void)hipEventDestroy(sv->ev_done[q]);
     }
     if (sv->second) (void)hipStreamDestroy(sv->second);
+    if (sv->upload) (void)hipStreamDestroy(sv->upload);
+    if (sv->ev_upload) (void)hipEventDestroy(sv->ev_upload);
     ctx->servers.erase(sv);
     delete sv;
     return PSF_OK;
@@ -422,6 +444,10 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
         }
         // the round two back read this parity's history buffers
         if (sv->launched >= 2) SRV_HIPCHK(ctx, hipStreamWaitEvent(sa, sv->ev_done[q], 0));
+        if (sv->upload_pending) {   // newly admitted scenes' uploads
+            SRV_HIPCHK(ctx, hipStreamWaitEvent(sa, sv->ev_upload, 0));
+            sv->upload_pending = false;
+        }
         SRV_HIPCHK(ctx, launch_persistent_batch(sv->batch[q], sc.data(), nsteps.data(), hist.data(), ld.data(), n, sa));
         SRV_HIPCHK(ctx, hipEventRecord(sv->ev_step[q], sa));
         SRV_HIPCHK(ctx, hipStreamWaitEvent(sb, sv->ev_step[q], 0));
