@@ -258,6 +258,91 @@ def serving_bench(ctx, n_requests, rate):
     }
 
 
+def large_bench(args):
+    """The large single scene: 128^3 closed box, K=512 analytic modes (Phi
+    12.99 GB in FP32), power-law omega (|omega_k| ~ k^-5/6, random signs),
+    500 frames at dt=0.005, 2^24 particles.  The oracle cannot run this size,
+    so the checks are size-independent ones: omega against the FP64 plain
+    path after all frames, u on sampled faces against the host FP64 Phi, one
+    particle pass of a particle sample bit for bit against the oracle, and
+    unit energy."""
+    from parallelspectralfluids3d_amd import _lib as L
+    from parallelspectralfluids3d_amd import advection, basis, hip
+    from parallelspectralfluids3d_amd.grid import Grid, box_mask
+    n, K, N, frames, dt = 128, 512, 1 << 24, 500, 0.005
+    g = Grid(n, n, n)
+    h = 1.0 / n
+    t = time.time()
+    Phi, lam, tab = basis.analytic_box_modes(g, K, dtype=np.float64)
+    log(f"large: basis {g.nF} faces x {K} modes in {time.time() - t:.1f}s")
+    t = time.time()
+    V = (basis.curl_operator(g, h) @ Phi.T).T
+    log(f"large: curl of the basis in {time.time() - t:.1f}s")
+    t = time.time()
+    C = advection.build_advection_box_analytic(g, h, Phi, lam, tab, V, kchunk=32,
+                                               progress=lambda i, m: log(f"large: C slot {i}/{m}"))
+    del V
+    log(f"large: C ({K}^3) in {time.time() - t:.1f}s")
+    gvec = basis.project(Phi, basis.gravity_field(g, box_mask(g)))
+    omega = basis.power_law_omega(K, 0)
+    rng = np.random.default_rng(1)
+    parts = np.empty((N, 4), np.float32)
+    parts[:, :3] = rng.random((N, 3), dtype=np.float32) - np.float32(0.5)
+    parts[:, 3] = np.inf
+    sc = dict(grid=g, h=h, Phi=Phi, lam=lam, C=C, g=gvec, omega=omega, parts=parts)
+    ctx = hip.Context(0)
+    t = time.time()
+    s = make_hip_scene(ctx, sc, L.C_BF16X2, L.PHI_FP32, True, N)
+    s.set_params(dt, 0.0, False, lifetime=60.0)
+    s.set_frame_window(32)
+    log(f"large: scene uploaded in {time.time() - t:.1f}s")
+    s.time_phase(L.PHASE_FRAME, 32)            # warm-up window
+    s.set_omega(omega)
+    s.set_particles(parts)
+    ms = s.time_phase(L.PHASE_FRAME, frames)
+    log(f"large: {frames} frames in {ms:.1f} ms")
+    ms_tr = s.time_phase(L.PHASE_TRACE, 5) / 5
+    ms_w = s.time_phase(L.PHASE_WINDOW_RECON, 2) / 2
+    w_gpu = s.get_omega()
+    u_gpu = s.get_velocity()
+    # omega against the FP64 plain path over the same frames
+    ref = make_hip_scene(ctx, sc, L.C_FP64, L.PHI_FP32, False, 0)
+    ref.set_params(dt, 0.0, False, lifetime=60.0)
+    ref.step(frames)
+    w_ref = ref.get_omega()
+    ref.close()
+    rel_w = float(np.linalg.norm(w_gpu - w_ref) / np.linalg.norm(w_ref))
+    # u on sampled faces against the FP64 sum over the host's FP32-rounded Phi
+    fs = rng.choice(g.nF, 4096, replace=False)
+    u_host = (Phi[:, fs].astype(np.float32).astype(np.float64).T @ w_gpu).astype(np.float32)
+    u_err = float(np.max(np.abs(u_gpu[fs] - u_host)) / np.max(np.abs(u_host)))
+    # one particle pass of a sample against the oracle, with the GPU's u
+    p_now = s.get_particles()
+    sample = p_now[rng.choice(N, 4096, replace=False)].copy()
+    s.set_particles(sample)
+    s.trace()
+    p_gpu = s.get_particles()
+    s.close()
+    ctx.close()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    m = O.Mesh((n, n, n), h, (-0.5, -0.5, -0.5), inside=np.ones(g.nV, bool))
+    p_ora = O.particles(m, u_gpu, sample, resolution=h, timestep=dt, aabb_min=(-.5, -.5, -.5),
+                        aabb_max=(.5, .5, .5), lifetime=60.0, seed=0, step=frames + 32)
+    trace_exact = bool(np.array_equal(p_gpu.view(np.uint32), p_ora.view(np.uint32)))
+    out = {"large": {
+        "workload": f"{n}^3 closed box, K={K} analytic modes, power-law omega, {frames} frames at dt={dt}, "
+                    f"{N} particles, frame window 32, BF16 hi+lo C, FP32 Phi",
+        "phi_gb_fp32": g.nF * K * 4 / 1e9,
+        "ms_per_frame": ms / frames, "particle_steps_per_s": N * frames / (ms * 1e-3),
+        "trace_ms": ms_tr, "window_reconstruct_ms": ms_w,
+        "window_reconstruct_tflops": 2.0 * g.nF * K * 32 / (ms_w * 1e-3) / 1e12,
+        "omega_rel_err_vs_fp64_path": rel_w, "omega_norm": float(np.linalg.norm(w_gpu)),
+        "u_sample_max_rel_err": u_err, "trace_sample_bit_exact_vs_oracle": trace_exact}}
+    print(json.dumps(out), flush=True)
+
+
 def counter_pass_copies(frames, ms):
     """Three device-to-device copies of 1 GiB with the HIP runtime's own copy
     kernel: the reference for the FETCH_SIZE / WRITE_SIZE counters (each copy
@@ -304,6 +389,9 @@ def main():
     ap.add_argument("--serving-profile", type=int, default=0, metavar="N",
                     help="only run the first N requests of the serving stream, all queued at once, through the "
                          "batched server (a short run for a profiler); prints one line and exits")
+    ap.add_argument("--large", action="store_true",
+                    help="only the large single scene (128^3, K=512, 2^24 particles, 500 frames): timing and "
+                         "size-independent checks; prints one line and exits (several minutes, ~60 GB of host memory)")
     ap.add_argument("--counter-pass", action="store_true",
                     help="short run for hardware-counter collection: the frame loop, one persistent launch of "
                          "--reduced-steps steps, then three plain 1 GiB device copies that calibrate the byte "
@@ -316,6 +404,10 @@ def main():
 
     from parallelspectralfluids3d_amd import _lib as L
     from parallelspectralfluids3d_amd import hip
+
+    if args.large:
+        large_bench(args)
+        return
 
     if args.serving_profile:
         ctx = hip.Context(0)

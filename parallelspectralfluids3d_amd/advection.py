@@ -115,9 +115,11 @@ def _profile_table(n, freqs, kind, off):
     return out
 
 
-def build_advection_box_analytic(g: Grid, resolution: float, Phi_cm, lam, table, V_cm=None, kchunk=64):
+def build_advection_box_analytic(g: Grid, resolution: float, Phi_cm, lam, table, V_cm=None, kchunk=64,
+                                 progress=None):
     """C (K, K, K) for a closed box whose basis is analytic_box_modes(g, K)
-    (table = its mode table, amplitudes normalised)."""
+    (table = its mode table, amplitudes normalised).  progress(slot, nslots),
+    if given, is called after each of the 12 edge slots."""
     K = Phi_cm.shape[0]
     if V_cm is None:
         V_cm = (curl_operator(g, resolution) @ Phi_cm.T).T        # (K, E)
@@ -166,5 +168,7 @@ def build_advection_box_analytic(g: Grid, resolution: float, Phi_cm, lam, table,
             Tij = Tij * (ia[None, :, None] * ib[None, None, :])
             anti = Tij - np.transpose(Tij, (0, 2, 1))                          # T[i,j] - T[j,i]
             C[:, :, k0:k1] += np.transpose(anti, (1, 2, 0)) * (sh * c_s)
+        if progress is not None:
+            progress(s + 1, len(SLOTS))
     C /= np.asarray(lam)[:, None, None]
     return C

@@ -25,25 +25,37 @@ from .grid import Grid
 
 
 # ---------------------------------------------------------------------------
-def _box_mode_list(g: Grid):
-    """All (lambda, a, b, c, pol) of divergence-free box modes, sorted."""
+def _box_mode_list(g: Grid, K=None):
+    """(lambda, a, b, c, pol) of divergence-free box modes, sorted; with K,
+    at least the K lowest (all modes whose indices stay below a bound M that
+    is raised until every mode beyond it has a larger eigenvalue)."""
     na, nb, nc = g.nx, g.ny, g.nz
-    modes = []
     sa = np.sin(np.arange(na) * np.pi / (2 * na))
     sb = np.sin(np.arange(nb) * np.pi / (2 * nb))
     sc = np.sin(np.arange(nc) * np.pi / (2 * nc))
-    for a in range(na):
-        for b in range(nb):
-            for c in range(nc):
-                zeros = (a == 0) + (b == 0) + (c == 0)
-                if zeros >= 2:
-                    continue
-                lam = 4.0 * (sa[a] ** 2 + sb[b] ** 2 + sc[c] ** 2)
-                npol = 2 if zeros == 0 else 1
-                for pol in range(npol):
-                    modes.append((lam, a, b, c, pol))
-    modes.sort(key=lambda m: (round(m[0], 12), m[1], m[2], m[3], m[4]))
-    return modes
+    M = 8 if K is not None else max(na, nb, nc)
+    while True:
+        ma, mb, mc = min(M, na), min(M, nb), min(M, nc)
+        modes = []
+        for a in range(ma):
+            for b in range(mb):
+                for c in range(mc):
+                    zeros = (a == 0) + (b == 0) + (c == 0)
+                    if zeros >= 2:
+                        continue
+                    lam = 4.0 * (sa[a] ** 2 + sb[b] ** 2 + sc[c] ** 2)
+                    npol = 2 if zeros == 0 else 1
+                    for pol in range(npol):
+                        modes.append((lam, a, b, c, pol))
+        modes.sort(key=lambda m: (round(m[0], 12), m[1], m[2], m[3], m[4]))
+        complete = ma == na and mb == nb and mc == nc
+        if complete or K is None:
+            return modes
+        # a mode with some index >= M has lambda >= 4 sin^2(M pi / 2n) on that axis
+        bound = min(4.0 * np.sin(min(M, n) * np.pi / (2 * n)) ** 2 for n in (na, nb, nc) if M < n)
+        if len(modes) >= K and modes[K - 1][0] < bound:
+            return modes
+        M *= 2
 
 
 def _polarisation(kappa, pol, zero_axis):
@@ -62,7 +74,7 @@ def _polarisation(kappa, pol, zero_axis):
 
 def box_mode_table(g: Grid, K: int):
     """(K, 7) array: lambda, a, b, c, A, B, C (unnormalised polarisation)."""
-    modes = _box_mode_list(g)
+    modes = _box_mode_list(g, K)
     if K > len(modes):
         raise ValueError(f"box {g.nx}x{g.ny}x{g.nz} has only {len(modes)} divergence-free modes")
     out = np.zeros((K, 7))
