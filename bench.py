@@ -232,6 +232,33 @@ def run_stream(ctx, bases, reqs, max_active, window=32, saturate=False, modes_pe
                 mem_peak_gb=st["mem_peak"] / 1e9)
 
 
+def dropin_sequential(ctx, bases, reqs):
+    """The requests one after another through the single-scene drop-in: a
+    scene per request (upload of Phi's first K columns, C[:K,:K,:K], omega and
+    particles), integrate() frame by frame with no frame window, read back.
+    Returns scene-frames/s over the wall time of all of it."""
+    from parallelspectralfluids3d_amd import _lib as L
+    from parallelspectralfluids3d_amd import hip
+    frames = 0
+    t0 = time.perf_counter()
+    for r in reqs:
+        b = bases[r["grid"]]
+        K = r["K"]
+        d = hip.make_desc(b["grid"], b["h"], K, L.C_BF16X2, L.PHI_FP32, r["parts"].shape[0])
+        s = hip.Scene(ctx, d)
+        s.upload_basis(b["Phi"][:K], b["lam"][:K], b["g"][:K])
+        s.upload_advection(np.ascontiguousarray(b["C"][:K, :K, :K]))
+        s.set_omega(r["omega"])
+        s.set_params(0.01, 0.0, False, lifetime=60.0)
+        s.set_particles(r["parts"])
+        s.integrate(r["frames"])
+        s.get_omega()
+        s.get_particles()
+        s.close()
+        frames += r["frames"]
+    return frames / (time.perf_counter() - t0)
+
+
 def serving_bench(ctx, n_requests, rate):
     t = time.time()
     bases = serving_bases()
@@ -247,6 +274,10 @@ def serving_bench(ctx, n_requests, rate):
         f"p50 {alone['latency_ms_p50']:.1f} ms, p99 {alone['latency_ms_p99']:.1f} ms")
     sat = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=4)
     log(f"serving batched, all queued at once: {sat['scene_frames_per_s']:.0f} scene-frames/s")
+    nd = min(16, n_requests)
+    dropin = dropin_sequential(ctx, bases, reqs[:nd])
+    log(f"serving: first {nd} requests one after another through the drop-in, frame by frame: {dropin:.0f} "
+        f"scene-frames/s")
     return {
         "workload": f"{n_requests} requests, Poisson arrivals at {rate:g}/s, grids {SERVE_GRIDS}^3, K in [16, 192], "
                     f"frames in [100, 2000], 2^14 particles each; one K0={SERVE_K0} basis per grid",
@@ -255,6 +286,10 @@ def serving_bench(ctx, n_requests, rate):
         "batched_all_queued": sat,
         "throughput_vs_one_at_a_time": batched["scene_frames_per_s"] / alone["scene_frames_per_s"],
         "capacity_vs_one_at_a_time": sat["scene_frames_per_s"] / alone["scene_frames_per_s"],
+        "dropin_frame_by_frame": {"requests": nd, "scene_frames_per_s": dropin,
+                                  "note": "the first requests of the stream, each through psf_integrate with no "
+                                          "frame window, uploads included"},
+        "capacity_vs_dropin_frame_by_frame": sat["scene_frames_per_s"] / dropin,
     }
 
 

@@ -288,17 +288,27 @@ __device__ __forceinline__ float mixf(float x, float y, float a)
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// int(a / b) with a / b rounded to double first, as the CPU path computes it,
-// from rb = RN(1/b): q = RN(a * rb) is within 3 * 2^-53 |q| of RN(a / b), so
-// when q lies farther than 2^-40 max(|q|, 1) from every integer, no integer
-// separates the two and they truncate alike.  Otherwise (particles within
-// ~1e-12 cells of a voxel plane, or NaN) the IEEE division decides.
-__device__ __forceinline__ int trunc_div(double a, double b, double rb)
+// The three cell indices of one axis as the CPU path computes them
+// (psfsolver.cpp:80-100): O = int(a / h), M = int((a - h/2) / h),
+// P = int((a + h/2) / h), each quotient rounded to double before the
+// truncation.  From rb = RN(1/h), q = RN(a * rb) is within 3 * 2^-53 |q| of
+// a / h, and q -+ 1/2 are as close to the other two quotients; so when q lies
+// farther than 2^-40 max(|q|, 1/2) from every multiple of 1/2, no integer
+// separates any computed quotient from the CPU's and they truncate alike.
+// Otherwise (particles within ~1e-12 cells of a voxel or half-voxel plane,
+// or NaN) the three IEEE divisions decide.
+__device__ __forceinline__ void cell_indices(double a, double h, double hh, double rb, int& O, int& M, int& P)
 {
-    const double q = a * rb;
-    if (__builtin_fabs(q - __builtin_rint(q)) > 0x1p-40 * __builtin_fmax(__builtin_fabs(q), 1.0))
-        return static_cast<int>(q);
-    return static_cast<int>(a / b);
+    const double q = a * rb, q2 = q + q;
+    if (__builtin_fabs(q2 - __builtin_rint(q2)) > 0x1p-39 * __builtin_fmax(__builtin_fabs(q2), 1.0)) {
+        O = static_cast<int>(q);
+        M = static_cast<int>(q - 0.5);
+        P = static_cast<int>(q + 0.5);
+    } else {
+        O = static_cast<int>(a / h);
+        M = static_cast<int>((a - hh) / h);
+        P = static_cast<int>((a + hh) / h);
+    }
 }
 
 // two consecutive face fluxes u[i], u[i + 1] in one 4-byte-aligned load (i + 1 < F
@@ -329,15 +339,10 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
         const float sy = (P.aabb_min[1] + P.ext[1]) + p.y;
         const float sz = (P.aabb_min[2] + P.ext[2]) + p.z;
         const double res = P.res, hres = P.res / 2, rr = P.inv_res;
-        int yO = trunc_div(static_cast<double>(sy), res, rr);
-        int xO = trunc_div(static_cast<double>(sx), res, rr);
-        int zO = trunc_div(static_cast<double>(sz), res, rr);
-        int zM = trunc_div(static_cast<double>(sz) - hres, res, rr);
-        int zP = trunc_div(static_cast<double>(sz) + hres, res, rr);
-        int yM = trunc_div(static_cast<double>(sy) - hres, res, rr);
-        int yP = trunc_div(static_cast<double>(sy) + hres, res, rr);
-        int xM = trunc_div(static_cast<double>(sx) - hres, res, rr);
-        int xP = trunc_div(static_cast<double>(sx) + hres, res, rr);
+        int xO, xM, xP, yO, yM, yP, zO, zM, zP;
+        cell_indices(static_cast<double>(sx), res, hres, rr, xO, xM, xP);
+        cell_indices(static_cast<double>(sy), res, hres, rr, yO, yM, yP);
+        cell_indices(static_cast<double>(sz), res, hres, rr, zO, zM, zP);
         xO = clampi(xO, 0, nx - 1); yO = clampi(yO, 0, ny - 1); zO = clampi(zO, 0, nz - 1);
         xM = clampi(xM, 0, nx - 1); xP = clampi(xP, 0, nx - 1);
         yM = clampi(yM, 0, ny - 1); yP = clampi(yP, 0, ny - 1);
