@@ -353,14 +353,14 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
             c[9] = 0; c[10] = 0; c[11] = 0;
         }
         // face indices (decmesh3d.cpp:17-30): Z faces, then Y, then X
-        const uint32_t nxy = P.nx * P.ny;
-        auto zf = [&](int x, int y, int z) { return static_cast<uint32_t>(z) * nxy + static_cast<uint32_t>(y) * P.nx + x; };
-        auto yf = [&](int x, int y, int z) {
-            return static_cast<uint32_t>(z) * ((P.ny + 1) * P.nx) + static_cast<uint32_t>(y) * P.nx + x + P.nZF;
-        };
-        auto xf = [&](int x, int y, int z) {
-            return static_cast<uint32_t>(z) * (P.ny * (P.nx + 1)) + static_cast<uint32_t>(y) * (P.nx + 1) + x + P.nZF + P.nYF;
-        };
+        // Face indices (decmesh3d.cpp:17-30): Z faces, then Y, then X, each
+        // block z-major.  Strides stay below 2^24 and indices below 2^31, so
+        // the products run on the 24-bit multiplier; neighbouring faces are
+        // reached by adding strides (the M/P pairs differ by 0, 1 or 2 cells).
+        const uint32_t sZy = P.nx, sZz = P.nx * P.ny;
+        const uint32_t sYy = P.nx, sYz = (P.ny + 1) * P.nx;
+        const uint32_t sXy = P.nx + 1, sXz = P.ny * (P.nx + 1);
+        const uint32_t dy = static_cast<uint32_t>(yP - yM), dz = static_cast<uint32_t>(zP - zM);
         // Faces of neighbouring x are adjacent in every block of the layout, so
         // each (x, x+1) couple comes in one load: the x faces of voxel xO, and
         // the y and z faces of columns xM and xP (xP = xM + 1 but at clamps).
@@ -369,15 +369,18 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
         auto second = [&](const FluxPair& q, uint32_t i) { return dx == 1 ? q.y : (dx == 0 ? q.x : u[i + dx]); };
         // x component: voxels (xO, yM|yP, zM|zP), faces f5 (left) / f6 (right)
         const float s4 = face_sign(4, xO, 0, 0), s5 = face_sign(5, xO, 0, 0);
-        const FluxPair x1 = pair(xf(xO, yM, zM)), x2 = pair(xf(xO, yP, zM));
-        const FluxPair x3 = pair(xf(xO, yM, zP)), x4 = pair(xf(xO, yP, zP));
+        const uint32_t bx = __umul24(zM, sXz) + __umul24(yM, sXy) + xO + P.nZF + P.nYF;
+        const uint32_t dyx = __umul24(dy, sXy), dzx = __umul24(dz, sXz);
+        const FluxPair x1 = pair(bx), x2 = pair(bx + dyx);
+        const FluxPair x3 = pair(bx + dzx), x4 = pair(bx + dyx + dzx);
         const float vel1x = -s4 * x1.x, vel3x = s5 * x1.y;
         const float vel2x = -s4 * x2.x, vel4x = s5 * x2.y;
         const float vel5x = -s4 * x3.x, vel7x = s5 * x3.y;
         const float vel6x = -s4 * x4.x, vel8x = s5 * x4.y;
         // y component: voxels (xM|xP, yO, zM|zP), faces f3 (bottom) / f4 (top)
         const float s2 = face_sign(2, 0, yO, 0), s3 = face_sign(3, 0, yO, 0);
-        const uint32_t iy1 = yf(xM, yO, zM), iy2 = yf(xM, yO + 1, zM), iy3 = yf(xM, yO, zP), iy4 = yf(xM, yO + 1, zP);
+        const uint32_t iy1 = __umul24(zM, sYz) + __umul24(yO, sYy) + xM + P.nZF;
+        const uint32_t iy2 = iy1 + sYy, iy3 = iy1 + __umul24(dz, sYz), iy4 = iy3 + sYy;
         const FluxPair y1 = pair(iy1), y2 = pair(iy2), y3 = pair(iy3), y4 = pair(iy4);
         const float vel1y = -s2 * y1.x, vel2y = s3 * y2.x;
         const float vel3y = -s2 * second(y1, iy1), vel4y = s3 * second(y2, iy2);
@@ -385,7 +388,8 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
         const float vel7y = -s2 * second(y3, iy3), vel8y = s3 * second(y4, iy4);
         // z component: voxels (xM|xP, yM|yP, zO), faces f1 (front) / f2 (back)
         const float s0 = face_sign(0, 0, 0, zO), s1 = face_sign(1, 0, 0, zO);
-        const uint32_t iz1 = zf(xM, yM, zO), iz2 = zf(xM, yP, zO), iz3 = zf(xM, yM, zO + 1), iz4 = zf(xM, yP, zO + 1);
+        const uint32_t iz1 = __umul24(zO, sZz) + __umul24(yM, sZy) + xM;
+        const uint32_t iz2 = iz1 + __umul24(dy, sZy), iz3 = iz1 + sZz, iz4 = iz2 + sZz;
         const FluxPair z1 = pair(iz1), z2 = pair(iz2), z3 = pair(iz3), z4 = pair(iz4);
         const float vel1z = s0 * z1.x, vel2z = s0 * z2.x;
         const float vel3z = s0 * second(z1, iz1), vel4z = s0 * second(z2, iz2);
