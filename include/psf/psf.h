@@ -124,6 +124,26 @@ PSF_API int psf_scene_upload_advection_matrices(psf_scene* scene, const double* 
    checks the caller's table against that closed form and fails on mismatch. */
 PSF_API int psf_scene_check_signs(psf_scene* scene, const uint32_t* words, size_t nwords);
 
+/* ---- closed boxes: basis and C built on the device ---------------------------
+ * No counterpart in the reference, which eigensolves M (psfsolver.cpp:236-342)
+ * and sums C over voxels on the CPU (psfsolver.cpp:431-551).  For a closed box
+ * (every voxel inside) the discrete sin/cos modes form a divergence-free basis
+ * with closed forms, and C factorises into 1-D sums along x, y and z, so both
+ * are built in place on the device from the mode table: nothing of size F or E
+ * crosses PCIe and C takes O(K^3) work.
+ * freqs: K rows (a, b, c) of frequencies along x, y, z (0 <= a < nx, ...);
+ * amps: K rows of the x-, y- and z-face amplitudes of the unit-norm mode;
+ * lambda: K eigenvalues; gravity: K or NULL (as for psf_scene_upload_basis).
+ * Mode k's flux on an X face (x node, y cell, z cell) is
+ *   amps[3k] * sin(a pi x / nx) * cos(b pi (y+1/2) / ny) * cos(c pi (z+1/2) / nz),
+ * and likewise on Y and Z faces with the sine along the face's own axis. */
+PSF_API int psf_scene_generate_box_basis(psf_scene* scene, const int32_t* freqs, const double* amps,
+                                         const double* lambda, const double* gravity);
+/* C of the same modes in the scene's c_precision (not FP16, whose per-mode
+   scales need the host upload). */
+PSF_API int psf_scene_build_box_advection(psf_scene* scene, const int32_t* freqs, const double* amps,
+                                          const double* lambda);
+
 /* setTimestep / setViscosity / setGravityActive / setLifeTime
    (Solver/abstractsolver.cpp:226-250).  lifetime is the value stored in
    AbstractSolver::lifeTime (frames). */

@@ -77,6 +77,8 @@ def load():
         "psf_scene_upload_advection": (i32, [vp, vp]),
         "psf_scene_upload_advection_matrices": (i32, [vp, vp]),
         "psf_scene_check_signs": (i32, [vp, vp, ct.c_size_t]),
+        "psf_scene_generate_box_basis": (i32, [vp, vp, vp, vp, vp]),
+        "psf_scene_build_box_advection": (i32, [vp, vp, vp, vp]),
         "psf_set_params": (i32, [vp, dbl, dbl, i32, flt]),
         "psf_set_step_path": (i32, [vp, i32]),
         "psf_set_omega": (i32, [vp, vp]),

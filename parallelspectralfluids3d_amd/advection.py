@@ -172,3 +172,98 @@ def build_advection_box_analytic(g: Grid, resolution: float, Phi_cm, lam, table,
             progress(s + 1, len(SLOTS))
     C /= np.asarray(lam)[:, None, None]
     return C
+
+
+# ---------------------------------------------------------------------------
+# Closed boxes, fully separable: C from the mode table alone.
+#
+# V = curl Phi is separable too.  Around an edge of axis A the curl takes four
+# face fluxes with fixed offsets and signs (CURL_FACES below).  Edges on a
+# low wall plane (a node coordinate 0) are oriented the other way, which
+# flips their sign; edges on two low walls touch only wall faces, where every
+# mode vanishes.  With every factor a product of 1-D profiles, each slot's
+# voxel sum becomes, per curl term, a product over the three axes of 1-D
+# triple sums S[f_k, f_i, f_j] over the distinct frequencies of that axis.
+# Nothing of size F or E is formed; the cost is O(K^3) per (slot, term).
+
+# faces around an edge of each axis: (face axis, face coordinates - edge coordinates, sign)
+CURL_FACES = {"x": [("y", (0, 0, -1), -1.0), ("y", (0, 0, 0), 1.0), ("z", (0, -1, 0), 1.0), ("z", (0, 0, 0), -1.0)],
+              "y": [("x", (0, 0, -1), -1.0), ("x", (0, 0, 0), 1.0), ("z", (-1, 0, 0), 1.0), ("z", (0, 0, 0), -1.0)],
+              "z": [("x", (0, -1, 0), -1.0), ("x", (0, 0, 0), 1.0), ("y", (-1, 0, 0), 1.0), ("y", (0, 0, 0), -1.0)]}
+
+
+def _profile_at(n, f, kind, idx):
+    """Node sine at node idx (zero at and beyond the walls) or cell cosine at
+    cell idx (zero outside 0..n-1)."""
+    idx = np.asarray(idx)
+    if kind == "node":
+        v = np.sin(f * np.pi * idx / n)
+        return np.where((idx <= 0) | (idx >= n), 0.0, v)
+    v = np.cos(f * np.pi * (idx + 0.5) / n)
+    return np.where((idx < 0) | (idx > n - 1), 0.0, v)
+
+
+def box_separable_terms(g: Grid, table):
+    """The (slot, curl term) list of the separable form: for each, the slot
+    constant c_s, the curl sign, the face axes of the curl term and of faces
+    a and b, and per axis the table S[f_k, f_i, f_j] over that axis's
+    distinct frequencies (ascending).  Returns (terms, uniq) with uniq[axis]
+    the distinct frequencies."""
+    dims = {"x": g.nx, "y": g.ny, "z": g.nz}
+    col = {"x": 1, "y": 2, "z": 3}
+    uniq = {ax: np.unique(np.asarray(table)[:, col[ax]].astype(int)) for ax in "xyz"}
+    one = np.ones(3)
+    terms = []
+    for s, (fs, es, a, b) in enumerate(SLOTS):
+        fa, fb = _FACE_OF_SLOT[a], _FACE_OF_SLOT[b]
+        c_s = float(np.cross(_INTERIOR_NORMAL[fa[0]], _INTERIOR_NORMAL[fb[0]]) @ one)
+        if c_s == 0.0:
+            continue
+        ea, dx, dy, dz = _SLOT_EDGE[s]
+        off = {"x": dx, "y": dy, "z": dz}
+        for (ft, d, sg) in CURL_FACES[ea]:
+            dd = {"x": d[0], "y": d[1], "z": d[2]}
+            S = {}
+            for al in "xyz":
+                n = dims[al]
+                v = np.arange(n)
+                ec = v + off[al]
+                Q = np.stack([_profile_at(n, f, "node" if al == ft else "cell", ec + dd[al]) for f in uniq[al]])
+                if al != ea:
+                    Q = Q * np.where(ec == 0, -1.0, 1.0)[None, :]
+                Pa = np.stack([_profile_at(n, f, "node", v + fa[1]) if al == fa[0] else _profile_at(n, f, "cell", v)
+                               for f in uniq[al]])
+                Pb = np.stack([_profile_at(n, f, "node", v + fb[1]) if al == fb[0] else _profile_at(n, f, "cell", v)
+                               for f in uniq[al]])
+                S[al] = np.einsum("kv,iv,jv->kij", Q, Pa, Pb)
+            terms.append(dict(c=c_s, sign=sg, face=ft, fa=fa[0], fb=fb[0], S=S))
+    return terms, uniq
+
+
+def build_advection_box_separable(g: Grid, resolution: float, lam, table, kchunk=32):
+    """C (K, K, K) for a closed box whose basis is analytic_box_modes(g, K),
+    from its mode table alone (no Phi, no curl)."""
+    from .basis import hodge_primal
+    table = np.asarray(table)
+    K = table.shape[0]
+    Hf = float(hodge_primal(Grid(1, 1, 1), resolution)[0])
+    sh = scale_sh(resolution)
+    terms, uniq = box_separable_terms(g, table)
+    amp = {"x": table[:, 4], "y": table[:, 5], "z": table[:, 6]}
+    pos = {ax: np.searchsorted(uniq[ax], table[:, c].astype(int)) for ax, c in (("x", 1), ("y", 2), ("z", 3))}
+    C = np.zeros((K, K, K))
+    for k0 in range(0, K, kchunk):
+        k1 = min(K, k0 + kchunk)
+        kk = np.arange(k0, k1)
+        acc = np.zeros((K, K, k1 - k0))          # [i, j, k]
+        for t in terms:
+            prod = np.ones((k1 - k0, K, K))       # [k, i, j]
+            for al in "xyz":
+                p = pos[al]
+                prod *= t["S"][al][p[kk][:, None, None], p[None, :, None], p[None, None, :]]
+            Tk = (t["sign"] * Hf) * amp[t["face"]][kk][:, None, None] * amp[t["fa"]][None, :, None] * \
+                amp[t["fb"]][None, None, :] * prod
+            acc += (np.transpose(Tk, (1, 2, 0)) - np.transpose(Tk, (2, 1, 0))) * (sh * t["c"])
+        C[:, :, k0:k1] = acc
+    C /= np.asarray(lam)[:, None, None]
+    return C

@@ -96,6 +96,49 @@ def mode_profiles(n: int, freq: int):
     return s, c
 
 
+def box_mode_table_normalised(g: Grid, K: int):
+    """The mode table of analytic_box_modes (amplitudes of the unit-norm
+    modes) without forming Phi: a mode's squared norm is a sum over the three
+    face families of products of 1-D sums."""
+    tab = box_mode_table(g, K)
+    for m in range(K):
+        a, b, c = int(tab[m, 1]), int(tab[m, 2]), int(tab[m, 3])
+        sx, cx = mode_profiles(g.nx, a)
+        sy, cy = mode_profiles(g.ny, b)
+        sz, cz = mode_profiles(g.nz, c)
+        A, B, C = tab[m, 4:7]
+        n2 = (C * C * (sz @ sz) * (cy @ cy) * (cx @ cx) + B * B * (cz @ cz) * (sy @ sy) * (cx @ cx) +
+              A * A * (cz @ cz) * (cy @ cy) * (sx @ sx))
+        tab[m, 4:7] /= np.sqrt(n2)
+    return tab
+
+
+def box_mode_values(g: Grid, table, faces):
+    """Phi[:, faces] (K, len(faces)) of the modes of a normalised mode table,
+    evaluated at the given faces only."""
+    faces = np.asarray(faces, dtype=np.int64)
+    tab = np.asarray(table)
+    out = np.zeros((tab.shape[0], faces.size))
+    nzf, nyf = g.nZF, g.nYF
+    for m in range(tab.shape[0]):
+        a, b, c = int(tab[m, 1]), int(tab[m, 2]), int(tab[m, 3])
+        A, B, C = tab[m, 4:7]
+        sx, cx = mode_profiles(g.nx, a)
+        sy, cy = mode_profiles(g.ny, b)
+        sz, cz = mode_profiles(g.nz, c)
+        for j, f in enumerate(faces):
+            if f < nzf:
+                z, r = divmod(int(f), g.nx * g.ny); y, x = divmod(r, g.nx)
+                out[m, j] = C * ((sz[z] * cy[y]) * cx[x])
+            elif f < nzf + nyf:
+                z, r = divmod(int(f) - nzf, (g.ny + 1) * g.nx); y, x = divmod(r, g.nx)
+                out[m, j] = B * ((cz[z] * sy[y]) * cx[x])
+            else:
+                z, r = divmod(int(f) - nzf - nyf, g.ny * (g.nx + 1)); y, x = divmod(r, g.nx + 1)
+                out[m, j] = A * ((cz[z] * cy[y]) * sx[x])
+    return out
+
+
 def analytic_box_modes(g: Grid, K: int, dtype=np.float64):
     """Phi_cm (K, F), lambda (K,), mode table.  Each column has unit 2-norm."""
     tab = box_mode_table(g, K)

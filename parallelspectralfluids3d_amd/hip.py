@@ -133,6 +133,27 @@ class Scene:
         assert C.shape == (self.K, self.K, self.K)
         self._c(self.ctx.lib.psf_scene_upload_advection(self.h, _ptr(C)), "upload_advection")
 
+    def generate_box_basis(self, table, gravity=None):
+        """Phi, lambda (and gravity) of a closed box's analytic modes, written on
+        the device from the normalised mode table (columns lambda, a, b, c and
+        the x-, y-, z-face amplitudes, as basis.analytic_box_modes returns it)."""
+        tab = np.asarray(table, np.float64)
+        freqs = np.ascontiguousarray(tab[:, 1:4].astype(np.int32))
+        amps = np.ascontiguousarray(tab[:, 4:7])
+        lam = np.ascontiguousarray(tab[:, 0])
+        g = None if gravity is None else np.ascontiguousarray(gravity, np.float64)
+        self._c(self.ctx.lib.psf_scene_generate_box_basis(self.h, _ptr(freqs), _ptr(amps), _ptr(lam),
+                                                          None if g is None else _ptr(g)), "generate_box_basis")
+
+    def build_box_advection(self, table):
+        """C of the same modes, built on the device in the scene's c_precision."""
+        tab = np.asarray(table, np.float64)
+        freqs = np.ascontiguousarray(tab[:, 1:4].astype(np.int32))
+        amps = np.ascontiguousarray(tab[:, 4:7])
+        lam = np.ascontiguousarray(tab[:, 0])
+        self._c(self.ctx.lib.psf_scene_build_box_advection(self.h, _ptr(freqs), _ptr(amps), _ptr(lam)),
+                "build_box_advection")
+
     def check_signs(self, words):
         w = np.ascontiguousarray(words, dtype=np.uint32)
         self._c(self.ctx.lib.psf_scene_check_signs(self.h, _ptr(w), w.size), "check_signs")

@@ -58,3 +58,26 @@ def test_slot_table_is_the_voxel_edge_set():
         faces = g.voxel_faces[v]
         edges = set(g.face_edges[faces].reshape(-1))
         assert set(per_voxel[v]) == edges and len(set(per_voxel[v])) == 12
+
+
+@pytest.mark.parametrize("dims,K", [((8, 8, 8), 24), ((6, 5, 7), 20), ((12, 10, 9), 48)])
+def test_box_separable_equals_box_analytic(dims, K):
+    """C from the mode table alone (separable curl, 1-D triple sums) equals the
+    construction that forms V = curl Phi, to rounding."""
+    from parallelspectralfluids3d_amd.grid import Grid
+    g = Grid(*dims)
+    h = 1.0 / max(dims)
+    Phi, lam, tab = basis.analytic_box_modes(g, K)
+    Cr = advection.build_advection_box_analytic(g, h, Phi, lam, tab)
+    Cs = advection.build_advection_box_separable(g, h, lam, tab, kchunk=7)
+    assert np.abs(Cs - Cr).max() <= 1e-14 * np.abs(Cr).max()
+
+
+def test_normalised_mode_table_without_phi():
+    from parallelspectralfluids3d_amd.grid import Grid
+    for dims, K in (((8, 8, 8), 40), ((6, 5, 7), 30)):
+        g = Grid(*dims)
+        _, _, tab = basis.analytic_box_modes(g, K)
+        t2 = basis.box_mode_table_normalised(g, K)
+        assert np.array_equal(tab[:, :4], t2[:, :4])
+        assert np.abs(tab[:, 4:] - t2[:, 4:]).max() <= 1e-15

@@ -65,3 +65,15 @@ def test_bounded_mode_enumeration_equals_full():
     for dims, K in (((8, 8, 8), 64), ((6, 5, 7), 40), ((16, 16, 16), 200), ((12, 12, 12), 300), ((20, 9, 14), 150)):
         g = Grid(*dims)
         assert B._box_mode_list(g, K)[:K] == B._box_mode_list(g)[:K], dims
+
+
+@pytest.mark.parametrize("dims", [(6, 5, 7), (8, 8, 8)])
+def test_mode_table_without_phi(dims):
+    """The normalised mode table and sampled Phi columns, computed from 1-D
+    profile sums without forming Phi, equal the explicit unit-norm modes."""
+    g = Grid(*dims)
+    Phi, lam, tab = basis.analytic_box_modes(g, 20)
+    t2 = basis.box_mode_table_normalised(g, 20)
+    assert np.allclose(t2, tab, rtol=0, atol=1e-14)
+    faces = np.random.default_rng(0).choice(g.nF, 200, replace=False)
+    assert np.allclose(basis.box_mode_values(g, t2, faces), Phi[:, faces], rtol=0, atol=1e-14)
