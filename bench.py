@@ -417,6 +417,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-density", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rate", type=float, default=200.0)
     ap.add_argument("--serving-profile", type=int, default=0, metavar="N",
@@ -491,6 +492,23 @@ def main():
     if W <= 1:
         kern["k_reconstruct"] = ms_rec
         kern["k_persistent"] = ms_st
+    # particle density volume on the reference's 256^3 grid, counted inside
+    # the particle pass: a frame followed by psf_density, and each pass alone
+    dens = None
+    if not args.no_density:
+        dims = (256, 256, 256)
+        s.set_density_grid(dims, count_in_trace=True)
+        s.time_phase(L.PHASE_FRAME_DENSITY, args.warmup)
+        ms_fd = s.time_phase(L.PHASE_FRAME_DENSITY, args.steps) / args.steps
+        ms_vol = s.time_phase(L.PHASE_DENSITY_VOLUME, reps) / reps
+        ms_cnt = s.time_phase(L.PHASE_DENSITY_COUNT, reps) / reps
+        s.set_density_grid((0, 0, 0))
+        cells = dims[0] * dims[1] * dims[2]
+        dens = {"grid": list(dims), "ms_per_frame_with_density": ms_fd, "ms_per_frame_without": ms_frame,
+                "volume_pass_ms": ms_vol, "count_pass_ms": ms_cnt,
+                "count_pass_particles_per_s": N / (ms_cnt * 1e-3)}
+        log(f"density {dims}: frame + volume {1e3 * ms_fd:.1f} us, volume pass {1e3 * ms_vol:.1f} us, "
+            f"count pass {1e3 * ms_cnt:.1f} us")
     # the same frames one at a time
     s.set_frame_window(1)
     s.time_phase(L.PHASE_FRAME, args.warmup)
@@ -522,6 +540,16 @@ def main():
                                     achieved=cbytes / (kern["k_persistent"] * 1e-3) / 1e9,
                                     algorithmic_bytes_per_step=cbytes, traffic=None,
                                     note="C stays in registers across the steps of a launch: effective bandwidth")
+    if dens is not None:
+        vbytes = 10 * cells         # counters read and cleared (4 + 4 B), volume written (2 B)
+        roof["k_density_volume"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
+                                        achieved=vbytes / (ms_vol * 1e-3) / 1e9, algorithmic_bytes_per_launch=vbytes,
+                                        traffic=None)
+        cbytes_d = 16 * N
+        roof["k_density_count"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
+                                       achieved=cbytes_d / (ms_cnt * 1e-3) / 1e9,
+                                       algorithmic_bytes_per_launch=cbytes_d, traffic=None,
+                                       note="plus one counter increment per particle (L2 atomics)")
     for r in roof.values():
         r["frac"] = r["achieved"] / r["peak"]
     dominant = max(kern, key=kern.get)
@@ -601,6 +629,7 @@ def main():
         "phase_ms_one_at_a_time": {"reduced_step": ms_st, "reconstruct": ms_rec, "trace": ms_tr},
         "roofline": head,
         "roofline_by_kernel": roof,
+        "density": dens,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -187,13 +187,35 @@ PSF_API int psf_integrate(psf_scene* scene, int nframes);
    holds W * (4 * F' + 8 * K) + 8 * K more bytes, F' = F rounded up to 256. */
 PSF_API int psf_set_frame_window(psf_scene* scene, uint32_t frames);
 
+/* ---- particle density volume --------------------------------------------------
+ * The histogram -> volume step of the GPU solver's particle drawing
+ * (Solver/SolverImpl/psfsolvergpu.cpp:876-913, Resources/Volume/volume.comp:1-17;
+ * 256 x 256 x 256 there).  Counts of particles per cell of an rx x ry x rz grid
+ * spanning the scene's AABB, cell index (z * ry + y) * rx + x, and the volume
+ * half(float(count) / 1e6) as IEEE binary16 bits (round to nearest even).
+ * A particle's cell on each axis is trunc((p - aabb_min) / extent * r) in
+ * float, clamped to [0, r-1] (NaN: 0), so every particle is counted once; the
+ * reference's counting shader (history.comp) is absent from its sources, so
+ * this rule is the library's own.
+ * psf_set_density_grid allocates the grid (1..1024 cells per axis; 0,0,0
+ * removes it).  With count_in_trace, every particle pass also counts the
+ * particles' new positions, so psf_density after a frame reads no particle
+ * records.  psf_density computes the density of the stored particles and
+ * copies counts (rx*ry*rz uint32) and volume (rx*ry*rz uint16) to the host
+ * where the pointers are not null; the volume stays on the device either way. */
+PSF_API int psf_set_density_grid(psf_scene* scene, uint32_t rx, uint32_t ry, uint32_t rz, int count_in_trace);
+PSF_API int psf_density(psf_scene* scene, uint32_t* counts, uint16_t* volume);
+
 /* ---- measurement -----------------------------------------------------------
  * Runs `count` launches of one phase back to back on the context stream
  * between two HIP events and returns the elapsed milliseconds.
  * phase: 0 reduced step (one step per launch), 1 reconstruction, 2 trace,
  *        3 full integrate frame, 4 one launch of `count` reduced steps,
  *        5 the frame window's reconstruction (all W frames; window > 1,
- *          recomputes the u of the frames already computed ahead). */
+ *          recomputes the u of the frames already computed ahead),
+ *        6 a density count pass of the stored particles (adds to the
+ *          counters), 7 a density volume pass (clears the counters),
+ *        8 a full integrate frame followed by psf_density without copies. */
 PSF_API int psf_time_phase(psf_scene* scene, int phase, int count, double* ms);
 
 /* ---- scene server (continuous batching) ----------------------------------------

@@ -233,3 +233,35 @@ def particles(mesh: Mesh, u, parts, *, resolution, timestep, aabb_min, aabb_max,
 
 def respawn_draw(seed, step, particle, k):
     return int(lib().ora_respawn_draw(seed, step, particle, k))
+
+
+def density(parts, aabb_min, aabb_max, dims):
+    """Particle density of the GPU solver's particle drawing
+    (Solver/SolverImpl/psfsolvergpu.cpp:876-913): counts per cell of a
+    dims = (nx, ny, nz) grid over the AABB, shaped (nz, ny, nx), and the r16f
+    volume float(count) / 1e6 (Resources/Volume/volume.comp:14) as float16.
+
+    The counting shader (Res/Volume/history.comp) is absent from the
+    reference, so the cell rule is the library's: per axis
+    trunc((p - aabb_min) / extent * n) in float32, clamped to [0, n-1], NaN to
+    cell 0, with extent = aabb_max - aabb_min in float32 (AABB::getExtent)."""
+    p = np.asarray(parts, np.float32).reshape(-1, 4)
+    amin = np.asarray(aabb_min, np.float32)
+    ext = (np.asarray(aabb_max, np.float32) - amin).astype(np.float32)
+    cell = []
+    for a in range(3):
+        n = int(dims[a])
+        r = np.float32(n)
+        with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+            t = ((p[:, a] - amin[a]) / ext[a]) * r
+        c = np.zeros(p.shape[0], np.int64)
+        hi = t >= r
+        mid = (t > np.float32(0.0)) & ~hi
+        c[hi] = n - 1
+        c[mid] = np.trunc(t[mid]).astype(np.int64)
+        cell.append(c)
+    nx, ny, nz = (int(v) for v in dims)
+    lin = (cell[2] * ny + cell[1]) * nx + cell[0]
+    counts = np.bincount(lin, minlength=nx * ny * nz).astype(np.uint32).reshape(nz, ny, nx)
+    volume = (counts.astype(np.float32) / np.float32(1.0e6)).astype(np.float16)
+    return counts, volume

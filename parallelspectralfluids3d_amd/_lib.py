@@ -18,6 +18,7 @@ C_FP64, C_FP32, C_BF16, C_FP16, C_BF16X2 = 0, 1, 2, 3, 4
 PHI_FP64, PHI_FP32 = 0, 1
 PATH_PLAIN, PATH_PERSISTENT = 0, 1
 PHASE_STEP, PHASE_RECON, PHASE_TRACE, PHASE_FRAME, PHASE_NSTEP, PHASE_WINDOW_RECON = 0, 1, 2, 3, 4, 5
+PHASE_DENSITY_COUNT, PHASE_DENSITY_VOLUME, PHASE_FRAME_DENSITY = 6, 7, 8
 
 _STATUS = {0: "PSF_OK", -1: "PSF_ERR_ARG", -2: "PSF_ERR_HIP", -3: "PSF_ERR_NOMEM", -4: "PSF_ERR_STATE",
            -5: "PSF_ERR_UNSUPPORTED", -6: "PSF_ERR_NONFINITE"}
@@ -79,6 +80,8 @@ def load():
         "psf_scene_check_signs": (i32, [vp, vp, ct.c_size_t]),
         "psf_scene_generate_box_basis": (i32, [vp, vp, vp, vp, vp]),
         "psf_scene_build_box_advection": (i32, [vp, vp, vp, vp]),
+        "psf_set_density_grid": (i32, [vp, u32, u32, u32, i32]),
+        "psf_density": (i32, [vp, vp, vp]),
         "psf_set_params": (i32, [vp, dbl, dbl, i32, flt]),
         "psf_set_step_path": (i32, [vp, i32]),
         "psf_set_omega": (i32, [vp, vp]),

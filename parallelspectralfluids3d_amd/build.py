@@ -21,7 +21,8 @@ LIB = os.path.join(PKG, "libpsf_hip.so")
 SOLVER_CHECK = os.path.join(PKG, "solver_check")
 ARCH = os.environ.get("PSF_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["psf_kernels.hip", "psf_persistent.hip", "psf_api.hip", "psf_server.hip", "psf_box.hip"]
+SOURCES = ["psf_kernels.hip", "psf_persistent.hip", "psf_api.hip", "psf_server.hip", "psf_box.hip",
+           "psf_density.hip"]
 
 CXXFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -248,7 +248,9 @@ int enqueue_reconstruct(psf_scene* s)
 int enqueue_trace(psf_scene* s, int32_t* dcells)
 {
     TraceParams p = trace_params(s);
-    HIPCHK(s->ctx, launch_trace(p, current_u(s), s->dev.parts, s->dev.pid, s->n_particles, dcells, s->ctx->stream));
+    DensityGrid D;
+    if (int rc = density_for_trace(s, D)) return rc;
+    HIPCHK(s->ctx, launch_trace(p, current_u(s), s->dev.parts, s->dev.pid, s->n_particles, dcells, D, s->ctx->stream));
     s->frame++;
     return PSF_OK;
 }
@@ -547,6 +549,7 @@ void psf::scene_release(psf_scene* s)
     sfree(s, s->dev.u); sfree(s, s->dev.parts); sfree(s, s->dev.cells); sfree(s, s->dev.pid);
     sfree(s, s->dev.hist); sfree(s, s->dev.u_hist); sfree(s, s->dev.omega_view);
     sfree(s, s->dev.hist_b); sfree(s, s->dev.u_hist_b);
+    density_release(s);
     release_side(s);
     s->ctx->scenes.erase(s);
     delete host_of(s);
@@ -793,6 +796,7 @@ int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
         HIPCHK(s->ctx, hipMemcpy(s->dev.pid, pid.data(), size_t(n) * 4, hipMemcpyHostToDevice));
     }
     s->n_particles = n;
+    density_particles_changed(s);
     return PSF_OK;
 }
 
@@ -866,13 +870,14 @@ int psf_integrate(psf_scene* s, int nframes)
 
 int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
 {
-    if (!s || !ms || count <= 0 || phase < 0 || phase > 5) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    if (!s || !ms || count <= 0 || phase < 0 || phase > 8) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
     if (phase == 5 && s->window <= 1) return set_err(s->ctx, PSF_ERR_STATE, "phase 5 needs a frame window > 1");
-    if (phase != 2) {
+    if (phase >= 6 && !s->dev.dcounts) return set_err(s->ctx, PSF_ERR_STATE, "phases 6 to 8 need a density grid");
+    if (phase != 2 && phase != 6 && phase != 7) {
         int rc = ready_for_step(s);
         if (rc) return rc;
     }
-    if ((phase == 1 || phase == 3 || phase == 5) && !s->has_phi) return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
+    if ((phase == 1 || phase == 3 || phase == 5 || phase == 8) && !s->has_phi) return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
     psf_ctx* ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     if (phase == 0 || phase == 1 || phase == 4 || phase == 5) {
@@ -899,6 +904,14 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
                 if (le != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "window reconstruction: %s", hipGetErrorString(le));
             }
             break;
+        case 6: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = density_enqueue_count(s); break;
+        case 7: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = density_enqueue_volume(s); break;
+        case 8:
+            for (int i = 0; i < count && rc == PSF_OK; ++i) {
+                rc = enqueue_frame(s);
+                if (rc == PSF_OK) rc = density_enqueue(s, nullptr);
+            }
+            break;
         default: rc = enqueue_steps(s, count); break;
         }
     }
@@ -912,7 +925,7 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    if (rc == PSF_OK && phase != 1 && phase != 2 && phase != 5) rc = check_status(s);
+    if (rc == PSF_OK && (phase == 0 || phase == 3 || phase == 4 || phase == 8)) rc = check_status(s);
     return rc;
 }
 

@@ -25,5 +25,16 @@ TraceParams scene_trace_params(const psf_scene* s);
 int scene_check_status(psf_scene* s);
 int ctx_set_error(psf_ctx* ctx, int code, const std::string& msg);
 size_t pool_class_bytes(size_t bytes);   // bytes a pooled allocation of this size holds
+// the density grid a particle pass counts into (counts null: none); clears
+// the counters first when they hold anything
+int density_for_trace(psf_scene* s, DensityGrid& D);
+// the stored particles changed without being counted
+void density_particles_changed(psf_scene* s);
+void density_release(psf_scene* s);
+int density_enqueue_count(psf_scene* s);    // count pass of the stored particles
+int density_enqueue_volume(psf_scene* s);   // volume pass (clears the counters)
+// psf_density on the context stream; counts_host (optional) receives the
+// counts by an asynchronous copy
+int density_enqueue(psf_scene* s, uint32_t* counts_host);
 
 }  // namespace psf

@@ -453,8 +453,14 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
     return p;
 }
 
+__device__ __forceinline__ void density_add(const DensityGrid& D, float x, float y, float z)
+{
+    __hip_atomic_fetch_add(D.counts + density_cell(D, x, y, z), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __restrict__ u, float4* __restrict__ parts,
-                                               const uint32_t* __restrict__ pid, uint32_t n, int32_t* __restrict__ cells)
+                                               const uint32_t* __restrict__ pid, uint32_t n, int32_t* __restrict__ cells,
+                                               DensityGrid D)
 {
     const uint32_t id = blockIdx.x * 256u + threadIdx.x;
     if (id >= n) return;
@@ -464,6 +470,49 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
     const float4 p = trace_one(P, as_global(u), make_float4(v.x, v.y, v.z, v.w), as_global(pid), id, cells);
     const f32x4 o = {p.x, p.y, p.z, p.w};
     __builtin_nontemporal_store(o, rec);
+    if (D.counts) density_add(D, p.x, p.y, p.z);
+}
+
+// Density of the stored particles: one counter increment per particle
+// (the histogram pass of the GPU solver's particle drawing,
+// psfsolvergpu.cpp:893-899).
+__global__ __launch_bounds__(256) void k_density_count(DensityGrid D, const float4* __restrict__ parts, uint32_t n)
+{
+    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= n) return;
+    const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(parts) + id);
+    density_add(D, v.x, v.y, v.z);
+}
+
+// volume = half(float(count) / 1e6) (Resources/Volume/volume.comp:14, stored
+// to an r16f image), and the counters are cleared for the next count.  Each
+// thread converts 8 cells: two 16-byte loads, one 16-byte volume store, two
+// 16-byte zero stores.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t density_half(uint32_t c)
+{
+    const _Float16 h = static_cast<_Float16>(__fdiv_rn(static_cast<float>(c), 1000000.0f));
+    return __builtin_bit_cast(uint16_t, h);
+}
+
+__global__ __launch_bounds__(256) void k_density_volume(uint32_t* __restrict__ counts, uint16_t* __restrict__ vol,
+                                                        size_t n8)
+{
+    const size_t t = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= n8) return;
+    u32x4* c = reinterpret_cast<u32x4*>(counts) + 2 * t;
+    const u32x4 a = __builtin_nontemporal_load(c);
+    const u32x4 b = __builtin_nontemporal_load(c + 1);
+    u32x4 o;
+    o.x = density_half(a.x) | (density_half(a.y) << 16);
+    o.y = density_half(a.z) | (density_half(a.w) << 16);
+    o.z = density_half(b.x) | (density_half(b.y) << 16);
+    o.w = density_half(b.z) | (density_half(b.w) << 16);
+    __builtin_nontemporal_store(o, reinterpret_cast<u32x4*>(vol) + t);
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    c[0] = z;
+    c[1] = z;
 }
 
 // Several scenes, each through nframes frames of its window: particle
@@ -603,10 +652,24 @@ hipError_t launch_gather_column(const double* hist, int ld, int t, int K, double
 }
 
 hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, const uint32_t* pid, uint32_t n,
-                        int32_t* cells, hipStream_t s)
+                        int32_t* cells, const DensityGrid& D, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    k_trace<<<(n + 255) / 256, 256, 0, s>>>(p, u, parts, pid, n, cells);
+    k_trace<<<(n + 255) / 256, 256, 0, s>>>(p, u, parts, pid, n, cells, D);
+    return hipGetLastError();
+}
+
+hipError_t launch_density_count(const DensityGrid& D, const float4* parts, uint32_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_density_count<<<(n + 255) / 256, 256, 0, s>>>(D, parts, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_density_volume(uint32_t* counts, uint16_t* volume, size_t n8, hipStream_t s)
+{
+    if (n8 == 0) return hipSuccess;
+    k_density_volume<<<static_cast<unsigned>((n8 + 255) / 256), 256, 0, s>>>(counts, volume, n8);
     return hipGetLastError();
 }
 

@@ -19,6 +19,36 @@ struct TraceParams {
     uint32_t seed, frame;
 };
 
+// Particle density grid: counts of particles per cell of an n[0] x n[1] x n[2]
+// grid spanning the AABB, x fastest, then y, then z (a 3-D image's texel
+// order).  Counting is off when counts is null.
+struct DensityGrid {
+    float amin[3], ext[3];      // AABB min and AABB::getExtent (float)
+    float r[3];                 // cells per axis as float (exact: n < 2^24)
+    uint32_t n[3];              // cells per axis
+    uint32_t* counts;           // n[0]*n[1]*n[2] counters
+};
+
+// Cell of a coordinate on one axis: trunc((p - amin) / ext * r) in float,
+// clamped to [0, n-1]; NaN lands in cell 0.  Division and product are
+// IEEE-rounded (no contraction, correctly rounded division).
+__device__ __forceinline__ uint32_t density_axis(float p, float amin, float ext, float r, uint32_t n)
+{
+#pragma clang fp contract(off)
+    const float t = __fdiv_rn(p - amin, ext) * r;
+    if (t >= r) return n - 1u;
+    if (t > 0.0f) return static_cast<uint32_t>(t);
+    return 0u;
+}
+
+__device__ __forceinline__ uint32_t density_cell(const DensityGrid& D, float x, float y, float z)
+{
+    const uint32_t cx = density_axis(x, D.amin[0], D.ext[0], D.r[0], D.n[0]);
+    const uint32_t cy = density_axis(y, D.amin[1], D.ext[1], D.r[1], D.n[1]);
+    const uint32_t cz = density_axis(z, D.amin[2], D.ext[2], D.r[2], D.n[2]);
+    return (cz * D.n[1] + cy) * D.n[0] + cx;
+}
+
 // one scene's share of a batched window reconstruction (PHI_FP32)
 struct WindowScene {
     const void* Phi;
@@ -65,9 +95,14 @@ uint32_t window_blocks(int window, size_t ldF);
 hipError_t launch_trace_window_multi(const TraceScene* d_tab, int n, uint32_t blocks, hipStream_t s);
 // out[k] = hist[k * ld + t]
 hipError_t launch_gather_column(const double* hist, int ld, int t, int K, double* out, hipStream_t s);
-// one particle pass
+// one particle pass; with D.counts set, each particle's new position is also
+// counted into D's grid
 hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, const uint32_t* pid, uint32_t n,
-                        int32_t* cells, hipStream_t s);
+                        int32_t* cells, const DensityGrid& D, hipStream_t s);
+// counts of the n stored particles into D's grid (added to what it holds)
+hipError_t launch_density_count(const DensityGrid& D, const float4* parts, uint32_t n, hipStream_t s);
+// volume[v] = half(float(counts[v]) / 1e6) and counts[v] = 0, for v < n8 * 8
+hipError_t launch_density_volume(uint32_t* counts, uint16_t* volume, size_t n8, hipStream_t s);
 hipError_t launch_zero_int(int* p, int n, hipStream_t s);
 
 }  // namespace psf

@@ -214,6 +214,25 @@ class Scene:
         (1 = one frame at a time)."""
         self._c(self.ctx.lib.psf_set_frame_window(self.h, int(frames)), "set_frame_window")
 
+    def set_density_grid(self, dims, count_in_trace=True):
+        """Particle density grid of dims = (rx, ry, rz) cells over the AABB
+        ((0, 0, 0) removes it); count_in_trace makes every particle pass count
+        the particles' new positions."""
+        rx, ry, rz = (int(v) for v in dims)
+        self._c(self.ctx.lib.psf_set_density_grid(self.h, rx, ry, rz, 1 if count_in_trace else 0),
+                "set_density_grid")
+        self._dens = (rx, ry, rz)
+
+    def density(self, counts=True, volume=True):
+        """Density of the stored particles: (counts uint32, volume float16), each
+        shaped (rz, ry, rx), or None where not asked for."""
+        rx, ry, rz = self._dens
+        c = np.empty((rz, ry, rx), np.uint32) if counts else None
+        v = np.empty((rz, ry, rx), np.uint16) if volume else None
+        self._c(self.ctx.lib.psf_density(self.h, None if c is None else _ptr(c), None if v is None else _ptr(v)),
+                "density")
+        return c, (None if v is None else v.view(np.float16))
+
     def time_phase(self, phase, count):
         ms = ct.c_double()
         self._c(self.ctx.lib.psf_time_phase(self.h, int(phase), int(count), ct.byref(ms)), "time_phase")
