@@ -44,7 +44,7 @@ struct DeviceScene {
     double* hist_b = nullptr;
     float* u_hist_b = nullptr;
     // particle density (psf_set_density_grid): counters and the half-precision
-    // volume, one per cell, padded to a multiple of 8 cells
+    // volume, one per cell, padded to a multiple of 2048 cells
     uint32_t* dcounts = nullptr;
     uint16_t* dvol = nullptr;
 };
@@ -94,7 +94,7 @@ struct psf_scene {
     bool pooled = false;         // device memory comes from ctx->pool and goes back there
     bool phi_borrowed = false;   // dev.Phi belongs to someone else (a server's shared basis)
     // particle density grid: cells per axis (0: none), cell count rounded up
-    // to a multiple of 8, whether particle passes count their new positions,
+    // to a multiple of 2048, whether particle passes count their new positions,
     // and what the counters hold: 0 zeros, 1 the counts of the stored
     // particles, 2 anything else
     uint32_t dens_n[3] = {0, 0, 0};

@@ -97,7 +97,7 @@ int density_enqueue_count(psf_scene* s)
 
 int density_enqueue_volume(psf_scene* s)
 {
-    DENS_HIPCHK(s->ctx, launch_density_volume(s->dev.dcounts, s->dev.dvol, s->dens_cells / 8, s->ctx->stream));
+    DENS_HIPCHK(s->ctx, launch_density_volume(s->dev.dcounts, s->dev.dvol, s->dens_cells / 2048, s->ctx->stream));
     s->dens_state = 0;
     return PSF_OK;
 }
@@ -136,7 +136,7 @@ int psf_set_density_grid(psf_scene* s, uint32_t rx, uint32_t ry, uint32_t rz, in
     DENS_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));   // earlier passes may still count into the old grid
     density_release(s);
     if (off) return PSF_OK;
-    const size_t cells = (size_t(rx) * ry * rz + 7) / 8 * 8;
+    const size_t cells = (size_t(rx) * ry * rz + 2047) / 2048 * 2048;   // whole volume-pass blocks
     void* c = nullptr;
     void* v = nullptr;
     hipError_t e = scene_alloc(s, &c, cells * 4);

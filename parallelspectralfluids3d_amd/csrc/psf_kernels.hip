@@ -453,9 +453,30 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
     return p;
 }
 
+// One counter increment per particle, with equal cells of neighbouring lanes
+// merged: the first lane of a run of equal cells adds the run's length.  A
+// wave's atomic costs about one L2 request per distinct line plus one per
+// repeated address, so runs (particles stored by voxel, piled up at a wall)
+// become one request.  Callers run it with every lane of the wave that holds
+// a particle; the active lanes are a prefix of the wave (consecutive ids,
+// the tail cut off by the particle count).
+__device__ __forceinline__ void density_add_key(uint32_t* counts, uint32_t key)
+{
+    const int lane = static_cast<int>(threadIdx.x & (kWave - 1));
+    const uint64_t active = __ballot(1);
+    const uint32_t prev = __shfl_up(key, 1, kWave);
+    const bool head = lane == 0 || prev != key;
+    const uint64_t ends = __ballot(head) | ~active;   // lanes where a run starts, or past the last particle
+    if (!head) return;
+    const uint64_t above = lane == kWave - 1 ? 0ull : ends & (~0ull << (lane + 1));
+    const int next = above ? __builtin_ctzll(above) : kWave;
+    __hip_atomic_fetch_add(counts + key, static_cast<uint32_t>(next - lane), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void density_add(const DensityGrid& D, float x, float y, float z)
 {
-    __hip_atomic_fetch_add(D.counts + density_cell(D, x, y, z), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    density_add_key(D.counts, density_cell(D, x, y, z));
 }
 
 __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __restrict__ u, float4* __restrict__ parts,
@@ -473,22 +494,88 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
     if (D.counts) density_add(D, p.x, p.y, p.z);
 }
 
-// Density of the stored particles: one counter increment per particle
-// (the histogram pass of the GPU solver's particle drawing,
-// psfsolvergpu.cpp:893-899).
+// Density of the stored particles (the histogram pass of the GPU solver's
+// particle drawing, psfsolvergpu.cpp:893-899).  Particles often gather in
+// few cells (they respawn on a small emitter sheet and live at most
+// `lifetime` frames), and global atomics on one address serialise in its L2
+// channel.  So each block loads a chunk of kDensChunk particles, counts their
+// cells into an LDS hash table (open addressing, at most kDensChunk keys in
+// twice as many slots) and then:
+//   - where that takes fewer global atomics than merging runs of equal cells
+//     in neighbouring lanes (distinct cells < half the runs: a hash-order
+//     atomic touches its own line, storage-order ones share lines), adds
+//     each distinct cell's count with one global atomic;
+//   - otherwise adds per particle with run-merged atomics in storage order.
+constexpr int kDensPer = 8;                       // particles per thread
+constexpr int kDensChunk = 256 * kDensPer;
+constexpr int kDensSlots = 2 * kDensChunk;
+constexpr uint32_t kDensEmpty = 0xffffffffu;      // cell indexes are below 2^30
+
 __global__ __launch_bounds__(256) void k_density_count(DensityGrid D, const float4* __restrict__ parts, uint32_t n)
 {
-    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
-    if (id >= n) return;
-    const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(parts) + id);
-    density_add(D, v.x, v.y, v.z);
+    __shared__ uint32_t key[kDensSlots];
+    __shared__ uint32_t cnt[kDensSlots];
+    __shared__ uint32_t distinct, runs;
+    for (int s = threadIdx.x; s < kDensSlots; s += 256) {
+        key[s] = kDensEmpty;
+        cnt[s] = 0u;
+    }
+    if (threadIdx.x == 0) {
+        distinct = 0u;
+        runs = 0u;
+    }
+    const uint32_t base = blockIdx.x * static_cast<uint32_t>(kDensChunk) + threadIdx.x;
+    uint32_t k[kDensPer];
+#pragma unroll
+    for (int j = 0; j < kDensPer; ++j) {
+        const uint32_t id = base + static_cast<uint32_t>(j * 256);
+        f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (id < n) v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(parts) + id);
+        k[j] = id < n ? density_cell(D, v.x, v.y, v.z) : kDensEmpty;
+    }
+    __syncthreads();
+    uint32_t my_runs = 0;
+    const int lane = static_cast<int>(threadIdx.x & (kWave - 1));
+#pragma unroll
+    for (int j = 0; j < kDensPer; ++j) {
+        const uint32_t prev = __shfl_up(k[j], 1, kWave);
+        my_runs += static_cast<uint32_t>(__builtin_popcountll(__ballot(k[j] != kDensEmpty && (lane == 0 || prev != k[j]))));
+    }
+    if (lane == 0) atomicAdd(&runs, my_runs);
+#pragma unroll
+    for (int j = 0; j < kDensPer; ++j) {
+        if (k[j] == kDensEmpty) continue;
+        uint32_t h = (k[j] * 2654435761u) >> (32 - 12);
+        static_assert(kDensSlots == 1 << 12, "hash width");
+        for (int probe = 0; probe < kDensSlots; ++probe) {   // a free or matching slot exists
+            const uint32_t prev = atomicCAS(&key[h], kDensEmpty, k[j]);
+            if (prev == kDensEmpty) atomicAdd(&distinct, 1u);
+            if (prev == kDensEmpty || prev == k[j]) break;
+            h = (h + 1u) & static_cast<uint32_t>(kDensSlots - 1);
+        }
+        atomicAdd(&cnt[h], 1u);
+    }
+    __syncthreads();
+    if (2u * distinct < runs) {
+        for (int s = threadIdx.x; s < kDensSlots; s += 256) {
+            const uint32_t c = cnt[s];
+            if (c) __hip_atomic_fetch_add(D.counts + key[s], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kDensPer; ++j)
+            if (k[j] != kDensEmpty) density_add_key(D.counts, k[j]);   // valid lanes: a prefix of the wave
+    }
 }
 
 // volume = half(float(count) / 1e6) (Resources/Volume/volume.comp:14, stored
-// to an r16f image), and the counters are cleared for the next count.  Each
-// thread converts 8 cells: two 16-byte loads, one 16-byte volume store, two
-// 16-byte zero stores.
+// to an r16f image), and the counters are cleared for the next count.  A
+// block converts 2048 cells; each of its load and store instructions covers
+// one contiguous span (a thread takes cells 4t..4t+3 and 1024+4t..1024+4t+3
+// of the block), which measured 6.3 TB/s against 4.9 for 32 contiguous bytes
+// per thread.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t density_half(uint32_t c)
 {
@@ -497,22 +584,20 @@ __device__ __forceinline__ uint32_t density_half(uint32_t c)
 }
 
 __global__ __launch_bounds__(256) void k_density_volume(uint32_t* __restrict__ counts, uint16_t* __restrict__ vol,
-                                                        size_t n8)
+                                                        size_t n2048)
 {
-    const size_t t = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (t >= n8) return;
-    u32x4* c = reinterpret_cast<u32x4*>(counts) + 2 * t;
-    const u32x4 a = __builtin_nontemporal_load(c);
-    const u32x4 b = __builtin_nontemporal_load(c + 1);
-    u32x4 o;
-    o.x = density_half(a.x) | (density_half(a.y) << 16);
-    o.y = density_half(a.z) | (density_half(a.w) << 16);
-    o.z = density_half(b.x) | (density_half(b.y) << 16);
-    o.w = density_half(b.z) | (density_half(b.w) << 16);
-    __builtin_nontemporal_store(o, reinterpret_cast<u32x4*>(vol) + t);
+    if (blockIdx.x >= n2048) return;
+    u32x4* c = reinterpret_cast<u32x4*>(counts) + static_cast<size_t>(blockIdx.x) * 512;
+    u32x2* v = reinterpret_cast<u32x2*>(vol) + static_cast<size_t>(blockIdx.x) * 512;
+    const u32x4 a = __builtin_nontemporal_load(c + threadIdx.x);
+    const u32x4 b = __builtin_nontemporal_load(c + 256 + threadIdx.x);
+    const u32x2 oa = {density_half(a.x) | (density_half(a.y) << 16), density_half(a.z) | (density_half(a.w) << 16)};
+    const u32x2 ob = {density_half(b.x) | (density_half(b.y) << 16), density_half(b.z) | (density_half(b.w) << 16)};
+    __builtin_nontemporal_store(oa, v + threadIdx.x);
+    __builtin_nontemporal_store(ob, v + 256 + threadIdx.x);
     const u32x4 z = {0u, 0u, 0u, 0u};
-    c[0] = z;
-    c[1] = z;
+    c[threadIdx.x] = z;
+    c[256 + threadIdx.x] = z;
 }
 
 // Several scenes, each through nframes frames of its window: particle
@@ -662,14 +747,14 @@ hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, con
 hipError_t launch_density_count(const DensityGrid& D, const float4* parts, uint32_t n, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    k_density_count<<<(n + 255) / 256, 256, 0, s>>>(D, parts, n);
+    k_density_count<<<(n + kDensChunk - 1) / kDensChunk, 256, 0, s>>>(D, parts, n);
     return hipGetLastError();
 }
 
-hipError_t launch_density_volume(uint32_t* counts, uint16_t* volume, size_t n8, hipStream_t s)
+hipError_t launch_density_volume(uint32_t* counts, uint16_t* volume, size_t n2048, hipStream_t s)
 {
-    if (n8 == 0) return hipSuccess;
-    k_density_volume<<<static_cast<unsigned>((n8 + 255) / 256), 256, 0, s>>>(counts, volume, n8);
+    if (n2048 == 0) return hipSuccess;
+    k_density_volume<<<static_cast<unsigned>(n2048), 256, 0, s>>>(counts, volume, n2048);
     return hipGetLastError();
 }
 

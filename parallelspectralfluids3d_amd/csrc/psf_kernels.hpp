@@ -101,8 +101,8 @@ hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, con
                         int32_t* cells, const DensityGrid& D, hipStream_t s);
 // counts of the n stored particles into D's grid (added to what it holds)
 hipError_t launch_density_count(const DensityGrid& D, const float4* parts, uint32_t n, hipStream_t s);
-// volume[v] = half(float(counts[v]) / 1e6) and counts[v] = 0, for v < n8 * 8
-hipError_t launch_density_volume(uint32_t* counts, uint16_t* volume, size_t n8, hipStream_t s);
+// volume[v] = half(float(counts[v]) / 1e6) and counts[v] = 0, for v < n2048 * 2048
+hipError_t launch_density_volume(uint32_t* counts, uint16_t* volume, size_t n2048, hipStream_t s);
 hipError_t launch_zero_int(int* p, int n, hipStream_t s);
 
 }  // namespace psf
