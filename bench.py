@@ -418,6 +418,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-density", action="store_true")
+    ap.add_argument("--serving-only", action="store_true", help="only the serving section")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rate", type=float, default=200.0)
     ap.add_argument("--serving-profile", type=int, default=0, metavar="N",
@@ -441,6 +442,13 @@ def main():
 
     if args.large:
         large_bench(args)
+        return
+
+    if args.serving_only:
+        ctx = hip.Context(0)
+        r = serving_bench(ctx, args.serving_requests, args.serving_rate)
+        ctx.close()
+        print(json.dumps({"serving": r}), flush=True)
         return
 
     if args.serving_profile:

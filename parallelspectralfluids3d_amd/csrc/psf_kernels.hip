@@ -6,6 +6,8 @@
 //   k_reconstruct      u = float(Phi w)     psfsolver.cpp:60
 //   k_trace            particle pass        psfsolver.cpp:70-232
 // (paths relative to the reference repository root)
+#include <type_traits>
+
 #include "psf_kernels.hpp"
 
 namespace psf {
@@ -180,6 +182,17 @@ __global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, 
 // frames, and runs the same per-face fma chain over k = 0..K-1 as
 // k_reconstruct.  The w_t of one mode are uniform across the wave and come
 // through scalar loads.
+// Blocks of a launch are dispatched to the 8 XCDs round-robin (block b to XCD
+// b % 8), and each XCD has its own L2.  xcd_block maps launch block b to
+// logical block (b % 8) * per + b / 8, so XCD x runs the contiguous logical
+// range [x * per, (x + 1) * per): neighbouring particles (stored by voxel)
+// and a scene's blocks share one L2, which then holds only their part of u.
+// The launch has 8 * per blocks, per = ceil(logical blocks / 8); blocks past
+// the logical count exit.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t per) { return (b % kXcds) * per + b / kXcds; }
+inline uint32_t xcd_per(uint32_t blocks) { return (blocks + kXcds - 1) / kXcds; }
+
 // Scene of a block in a batched launch: tab[0..n) (n <= 64) by ascending
 // block0; one table load per lane and a ballot.
 template <typename Tab>
@@ -239,6 +252,164 @@ __device__ __forceinline__ void window_body(const T* Phi_, size_t ldF, uint32_t 
 #pragma unroll
         for (int j = 0; j < FPT; ++j)
             if (f0 + j < F) uh[static_cast<size_t>(t) * ldF + f0 + j] = static_cast<float>(acc[j][t]);
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction on the FP64 matrix cores: U (faces x frames) = Phi (faces x
+// modes) * H (modes x frames), H[k][t] = w_t[k] = hist[k * hist_ld + t] for
+// t < W (W = 1 is u = float(Phi w)).  v_mfma_f64_16x16x4f64 adds the four
+// products of its k-step to the accumulator as a sequential fma chain
+// (checked bit for bit against fma(a3, b3, fma(a2, b2, fma(a1, b1, fma(a0, b0,
+// c))))), so a chain of k-steps from 0.0 is the per-face fma chain over
+// k = 0..K-1 of k_reconstruct: the same bits, W frames per pass over Phi.
+// Modes past K enter as 0 * 0, which leaves every accumulator unchanged.
+// A wave owns 64 faces as 4 row tiles of 16, with rows interleaved (row i of
+// tile rt is face f0 + 4i + rt), so a lane loads its A operands for all four
+// tiles as one aligned 16-byte load of 4 consecutive faces, and stores 4
+// consecutive faces of one frame at once (accumulator slot r of lane l holds
+// row l/16 + 4r, column l%16).  NT column tiles of 16 frames: W <= 16 * NT.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T, int NT>
+__device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint32_t F, int K, const double* hist_,
+                                                 int hist_ld, int W, float* uh_, size_t ldU, uint32_t blk)
+{
+    const gptr<const T> Phi = as_global(Phi_);
+    const gptr<const double> hist = as_global(hist_);
+    const gptr<float> uh = as_global(uh_);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t f0 = (static_cast<size_t>(blk) * 4 + wv) * 64;
+    if (f0 >= ldF) return;
+    const int li = lane & 15, lk = lane >> 4;
+    const gptr<const T> arow = Phi + f0 + 4 * li;
+    f64x4 acc[4][NT];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f64x4{0.0, 0.0, 0.0, 0.0};
+    // frames of the lane's B column: t = 16 ct + li; columns t >= W load a
+    // valid element (t clamped) and multiply by 0 instead
+    bool tv[NT];
+    int tc[NT];
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) {
+        tv[ct] = 16 * ct + li < W;
+        tc[ct] = tv[ct] ? 16 * ct + li : W - 1;
+    }
+    // operands as loaded (FP32 Phi is widened at use, and B columns past W
+    // are zeroed at use, so no load is waited for before its step)
+    struct RawA {
+        typename std::conditional<sizeof(T) == 4, f32x4, f64x2>::type v[sizeof(T) == 4 ? 1 : 2];
+    };
+    auto load_a = [&](int k, RawA& r) {
+        if constexpr (sizeof(T) == 4) {
+            r.v[0] = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4>>(arow + static_cast<size_t>(k) * ldF));
+        } else {
+            const gptr<const f64x2> q = reinterpret_cast<gptr<const f64x2>>(arow + static_cast<size_t>(k) * ldF);
+            r.v[0] = __builtin_nontemporal_load(q);
+            r.v[1] = __builtin_nontemporal_load(q + 1);
+        }
+    };
+    auto load_b = [&](int k, double (&b)[NT]) {
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) b[ct] = hist[static_cast<size_t>(k) * hist_ld + tc[ct]];
+    };
+    auto step = [&](const RawA& r, const double (&braw)[NT]) {
+        double a[4];
+        if constexpr (sizeof(T) == 4) {
+            a[0] = r.v[0].x; a[1] = r.v[0].y; a[2] = r.v[0].z; a[3] = r.v[0].w;
+        } else {
+            a[0] = r.v[0].x; a[1] = r.v[0].y; a[2] = r.v[1].x; a[3] = r.v[1].y;
+        }
+        double b[NT];
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) b[ct] = tv[ct] ? braw[ct] : 0.0;
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct)
+                acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+    };
+    // full k-steps (every k < K), branch-free, loads kPipe steps ahead: the
+    // queue slot of step k4 is refilled with step k4 + kPipe (clamped to the
+    // last full step; the extra loads are in bounds and unused)
+    constexpr int kPipe = 4;
+    const int K4 = K >> 2;
+    if (K4 > 0) {
+        RawA qa[kPipe];
+        double qb[kPipe][NT];
+#pragma unroll
+        for (int st = 0; st < kPipe; ++st) {
+            const int k = 4 * (st < K4 ? st : K4 - 1) + lk;
+            load_a(k, qa[st]);
+            load_b(k, qb[st]);
+        }
+        int k4 = 0;
+        for (; k4 + kPipe <= K4; k4 += kPipe) {
+#pragma unroll
+            for (int st = 0; st < kPipe; ++st) {
+                step(qa[st], qb[st]);
+                const int nx = k4 + st + kPipe;
+                const int k = 4 * (nx < K4 ? nx : K4 - 1) + lk;
+                load_a(k, qa[st]);
+                load_b(k, qb[st]);
+                // keep the refill here: the scheduler would otherwise sink
+                // the loads to the loop's end and wait for them there
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+#pragma unroll
+        for (int st = 0; st < kPipe; ++st)
+            if (st < K4 - k4) step(qa[st], qb[st]);
+    }
+    // the last, partial k-step: modes past K enter as 0 * 0
+    if (K & 3) {
+        const int k = 4 * K4 + lk;
+        RawA ra{};
+        double rb[NT];
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) rb[ct] = 0.0;
+        if (k < K) {
+            load_a(k, ra);
+            load_b(k, rb);
+        }
+        step(ra, rb);
+    }
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) {
+        if (!tv[ct]) continue;
+        const gptr<float> ut = uh + static_cast<size_t>(16 * ct + li) * ldU;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const size_t f = f0 + 4 * (lk + 4 * r);
+            const f32x4 o = {static_cast<float>(acc[0][ct][r]), static_cast<float>(acc[1][ct][r]),
+                             static_cast<float>(acc[2][ct][r]), static_cast<float>(acc[3][ct][r])};
+            if (f + 3 < F) {
+                *reinterpret_cast<gptr<f32x4>>(ut + f) = o;
+            } else {
+                if (f < F) ut[f] = o.x;
+                if (f + 1 < F) ut[f + 1] = o.y;
+                if (f + 2 < F) ut[f + 2] = o.z;
+            }
+        }
+    }
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(256) void k_reconstruct_mfma(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
+                                                          const double* __restrict__ hist, int hist_ld, int W,
+                                                          float* __restrict__ uh, size_t ldU)
+{
+    mfma_window_body<T, NT>(Phi, ldF, F, K, hist, hist_ld, W, uh, ldU, blockIdx.x);
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_reconstruct_mfma_multi(const WindowScene* __restrict__ tab, int n, int W)
+{
+    const WindowScene& w = tab[scene_of_block(tab, n, blockIdx.x)];
+    mfma_window_body<float, NT>(static_cast<const float*>(w.Phi), w.ldF, w.F, w.K, w.hist, W, W, w.uh, w.ldF,
+                                blockIdx.x - w.block0);
 }
 
 template <typename T, int FPT, int TT>
@@ -481,9 +652,9 @@ __device__ __forceinline__ void density_add(const DensityGrid& D, float x, float
 
 __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __restrict__ u, float4* __restrict__ parts,
                                                const uint32_t* __restrict__ pid, uint32_t n, int32_t* __restrict__ cells,
-                                               DensityGrid D)
+                                               DensityGrid D, uint32_t per)
 {
-    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t id = xcd_block(blockIdx.x, per) * 256u + threadIdx.x;
     if (id >= n) return;
     // particle records stream through once: nontemporal, so u stays in L2
     f32x4* rec = reinterpret_cast<f32x4*>(parts) + id;
@@ -604,10 +775,13 @@ __global__ __launch_bounds__(256) void k_density_volume(uint32_t* __restrict__ c
 // records do not interact, so each thread carries its particle through the
 // frames with u_t = u_hist + t * ldU and respawn frame P.frame + t, and
 // stores it once.
-__global__ __launch_bounds__(256) void k_trace_window_multi(const TraceScene* __restrict__ tab, int n)
+__global__ __launch_bounds__(256) void k_trace_window_multi(const TraceScene* __restrict__ tab, int n, uint32_t per,
+                                                            uint32_t blocks)
 {
-    const TraceScene& ts = tab[scene_of_block(tab, n, blockIdx.x)];
-    const uint32_t id = (blockIdx.x - ts.block0) * 256u + threadIdx.x;
+    const uint32_t blk = xcd_block(blockIdx.x, per);
+    if (blk >= blocks) return;
+    const TraceScene& ts = tab[scene_of_block(tab, n, blk)];
+    const uint32_t id = (blk - ts.block0) * 256u + threadIdx.x;
     if (id >= ts.n) return;
     TraceParams P = ts.P;
     const gptr<f32x4> parts = as_global(reinterpret_cast<f32x4*>(ts.parts));
@@ -656,6 +830,8 @@ hipError_t launch_update(double* omega, const double* y, const double* visc, con
     return hipGetLastError();
 }
 
+// W = 1: the GEMV (HBM-bound; 0.83 of peak against 0.76 for one MFMA column
+// tile); the window kernels give the same bits
 hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F, int K, const double* omega,
                               float* u, hipStream_t s)
 {
@@ -670,35 +846,22 @@ hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F,
     return hipGetLastError();
 }
 
-namespace {
-template <typename T, int FPT, int TT>
-hipError_t window_launch(const void* Phi, size_t ldF, uint32_t F, int K, const double* hist, float* uh, hipStream_t s)
-{
-    const unsigned grid = static_cast<unsigned>((ldF + 256 * FPT - 1) / (256 * FPT));
-    k_reconstruct_window<T, FPT, TT><<<grid, 256, 0, s>>>(static_cast<const T*>(Phi), ldF, F, K, hist, uh);
-    return hipGetLastError();
-}
-template <typename T>
-hipError_t window_dispatch(int window, const void* Phi, size_t ldF, uint32_t F, int K, const double* hist, float* uh,
-                           hipStream_t s)
-{
-    switch (window) {
-    case 2: return window_launch<T, 4, 2>(Phi, ldF, F, K, hist, uh, s);
-    case 4: return window_launch<T, 4, 4>(Phi, ldF, F, K, hist, uh, s);
-    case 8: return window_launch<T, 4, 8>(Phi, ldF, F, K, hist, uh, s);
-    case 16: return window_launch<T, 4, 16>(Phi, ldF, F, K, hist, uh, s);
-    case 32: return window_launch<T, 2, 32>(Phi, ldF, F, K, hist, uh, s);
-    default: return hipErrorInvalidValue;
-    }
-}
-}  // namespace
-
 hipError_t launch_reconstruct_window(int prec, int window, const void* Phi, size_t ldF, uint32_t F, int K,
                                      const double* hist, float* uh, hipStream_t s)
 {
     if (ldF % 256 != 0) return hipErrorInvalidValue;
-    return prec == PSF_PHI_FP32 ? window_dispatch<float>(window, Phi, ldF, F, K, hist, uh, s)
-                                : window_dispatch<double>(window, Phi, ldF, F, K, hist, uh, s);
+    if (window != 2 && window != 4 && window != 8 && window != 16 && window != 32) return hipErrorInvalidValue;
+    const unsigned grid = static_cast<unsigned>(ldF / 256);
+    if (prec == PSF_PHI_FP32) {
+        const float* P = static_cast<const float*>(Phi);
+        if (window == 32) k_reconstruct_mfma<float, 2><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, window, window, uh, ldF);
+        else k_reconstruct_mfma<float, 1><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, window, window, uh, ldF);
+    } else {
+        const double* P = static_cast<const double*>(Phi);
+        if (window == 32) k_reconstruct_mfma<double, 2><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, window, window, uh, ldF);
+        else k_reconstruct_mfma<double, 1><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, window, window, uh, ldF);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_reconstruct_window_multi(int prec, int window, const WindowScene* d_tab, int n, uint32_t blocks,
@@ -706,27 +869,23 @@ hipError_t launch_reconstruct_window_multi(int prec, int window, const WindowSce
 {
     if (n <= 0 || blocks == 0) return hipSuccess;
     if (prec != PSF_PHI_FP32) return hipErrorInvalidValue;
-    switch (window) {
-    case 2: k_reconstruct_window_multi<float, 4, 2><<<blocks, 256, 0, s>>>(d_tab, n); break;
-    case 4: k_reconstruct_window_multi<float, 4, 4><<<blocks, 256, 0, s>>>(d_tab, n); break;
-    case 8: k_reconstruct_window_multi<float, 4, 8><<<blocks, 256, 0, s>>>(d_tab, n); break;
-    case 16: k_reconstruct_window_multi<float, 4, 16><<<blocks, 256, 0, s>>>(d_tab, n); break;
-    case 32: k_reconstruct_window_multi<float, 2, 32><<<blocks, 256, 0, s>>>(d_tab, n); break;
-    default: return hipErrorInvalidValue;
-    }
+    if (window != 2 && window != 4 && window != 8 && window != 16 && window != 32) return hipErrorInvalidValue;
+    if (window == 32) k_reconstruct_mfma_multi<2><<<blocks, 256, 0, s>>>(d_tab, n, window);
+    else k_reconstruct_mfma_multi<1><<<blocks, 256, 0, s>>>(d_tab, n, window);
     return hipGetLastError();
 }
 
 uint32_t window_blocks(int window, size_t ldF)
 {
-    const size_t per_block = 256 * (window == 32 ? 2 : 4);
-    return static_cast<uint32_t>((ldF + per_block - 1) / per_block);
+    (void)window;
+    return static_cast<uint32_t>((ldF + 255) / 256);   // 256 faces per block
 }
 
 hipError_t launch_trace_window_multi(const TraceScene* d_tab, int n, uint32_t blocks, hipStream_t s)
 {
     if (n <= 0 || blocks == 0) return hipSuccess;
-    k_trace_window_multi<<<blocks, 256, 0, s>>>(d_tab, n);
+    const uint32_t per = xcd_per(blocks);
+    k_trace_window_multi<<<per * kXcds, 256, 0, s>>>(d_tab, n, per, blocks);
     return hipGetLastError();
 }
 
@@ -740,7 +899,8 @@ hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, con
                         int32_t* cells, const DensityGrid& D, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    k_trace<<<(n + 255) / 256, 256, 0, s>>>(p, u, parts, pid, n, cells, D);
+    const uint32_t per = xcd_per((n + 255) / 256);
+    k_trace<<<per * kXcds, 256, 0, s>>>(p, u, parts, pid, n, cells, D, per);
     return hipGetLastError();
 }
 

@@ -1,19 +1,23 @@
 // Scene server: continuous batching of independent scenes (C ABI in psf.h).
 //
 // Every round advances all active scenes by up to `window` integrate() frames
-// (psfsolver.cpp:21-234) with three launches:
-//   1. on the context stream, one persistent launch holding every scene's
+// (psfsolver.cpp:21-234) with three launches on three streams:
+//   1. P, on the context stream: one persistent launch holding every scene's
 //      workgroups; each scene runs its own number of reduced steps and
 //      writes its omega history;
-//   2. on the server's second stream, after 1, one window reconstruction
-//      over all scenes' Phi (FP64 sums);
-//   3. on the second stream, one particle launch carrying every scene's
-//      particles through their frames of the round.
-// The reduced steps of round r+1 need only omega, so they run while round r
-// reconstructs and traces; each scene has two omega-history buffers, and a
-// round's launches wait for the round two back to release its buffer.  The
-// host keeps one round in flight: a call launches round r, then waits for
-// round r-1 and hands out the requests that finished in it.
+//   2. R, on the reconstruction stream, after 1: one window reconstruction
+//      over all scenes' Phi (FP64 sums) into their u history;
+//   3. T, on the particle stream, after 2: one particle launch carrying
+//      every scene's particles through their frames of the round.
+// Each scene has two omega-history and two u-history buffers, used by rounds
+// of alternate parity.  So round r+1's P and R run while round r's T does:
+// P(r) waits for R(r-2) to release its omega history, R(r) for T(r-2) to
+// release its u history, and T(r) follows T(r-1) on its stream.  P is
+// latency-bound (one device-wide exchange per step), R FP64-bound and T
+// gather-bound, so overlapping them fills the device better than running
+// them in sequence.  The host keeps one round in flight: a call launches
+// round r, then waits for round r-1 and hands out the requests that finished
+// in it.
 // Scenes never exchange data, and each scene's arithmetic depends only on its
 // own inputs and layout, so a request gives the same bits alone and in a mix.
 // Admission is FIFO at round start, bounded by active slots, the workgroups
@@ -56,6 +60,7 @@ struct Request {
     uint32_t basis = 0, K = 0, nframes = 0, sched = 0, n = 0;
     int64_t last_round = -1;     // round that runs its last frames
     double* hist2 = nullptr;     // second omega-history buffer (the first is the scene's)
+    float* u_hist2 = nullptr;    // second u-history buffer
     double dt = 0.0, nu = 0.0;
     int grav = 0;
     float lifetime = 0.0f;
@@ -86,8 +91,10 @@ struct psf_server {
     TraceScene* d_ttab[2] = {nullptr, nullptr};
     TraceScene* h_ttab[2] = {nullptr, nullptr};
     hipEvent_t ev_step[2] = {nullptr, nullptr};   // after the persistent launch of a round
+    hipEvent_t ev_rec[2] = {nullptr, nullptr};    // after the reconstruction of a round
     hipEvent_t ev_done[2] = {nullptr, nullptr};   // after the particle launch of a round
-    hipStream_t second = nullptr;                 // reconstruction and particles
+    hipStream_t second = nullptr;                 // particles
+    hipStream_t third = nullptr;                  // reconstruction
     hipStream_t upload = nullptr;                 // admission uploads
     hipEvent_t ev_upload = nullptr;               // after the last admission's uploads
     bool upload_pending = false;
@@ -128,6 +135,7 @@ uint64_t request_bytes(const psf_server* sv, const Basis& b, uint32_t K, uint32_
         cb, b.F * 4ull, n * 16ull, n * 4ull,                            // C u parts pid
         bpw * 4, size_t(G) * ((K + G - 1) / G) * 8, 2ull * 16 * K * 8,   // meta inv_scale xbuf
         K * Wn * 8, b.ldF * Wn * 4, K * 8ull, K * Wn * 8,               // hist u_hist omega_view hist2
+        b.ldF * Wn * 4,                                                  // u_hist2
     };
     uint64_t t = 0;
     for (size_t x : sizes) t += pool_class_bytes(x);
@@ -156,6 +164,7 @@ int admit(psf_server* sv, Request* r)
     if ((rc = psf_set_particles(s, r->parts.data(), r->n))) return rc;
     if ((rc = psf_set_frame_window(s, sv->cfg.window))) return rc;
     SRV_HIPCHK(sv->ctx, scene_alloc(s, reinterpret_cast<void**>(&r->hist2), sizeof(double) * r->K * sv->cfg.window));
+    SRV_HIPCHK(sv->ctx, scene_alloc(s, reinterpret_cast<void**>(&r->u_hist2), sizeof(float) * s->dev.ldF * sv->cfg.window));
     if ((rc = psf_scene_upload_basis(s, nullptr, b.lambda.data(), b.gravity.data()))) return rc;
     SRV_HIPCHK(sv->ctx, launch_pack_c(s, b.C, static_cast<int>(b.desc.K), sv->ctx->stream));
     s->has_C = true;
@@ -169,10 +178,12 @@ void release(psf_server* sv, Request* r)
 {
     if (r->scene) {
         scene_free(r->scene, r->hist2);
+        scene_free(r->scene, r->u_hist2);
         scene_release(r->scene);
     }
     r->scene = nullptr;
     r->hist2 = nullptr;
+    r->u_hist2 = nullptr;
     if (r->last_round < 0) sv->wgs_in_use -= r->wgs;   // otherwise freed when its last round launched
     sv->mem_in_use -= r->footprint;
 }
@@ -188,10 +199,17 @@ struct UploadStream {
     ~UploadStream() { ctx->stream = saved; }
 };
 
-void admit_queued(psf_server* sv)
+// Host time one round call may spend admitting while a round runs: longer
+// admission would leave the device idle once the round in flight ends.
+constexpr double kAdmitBudgetMs = 0.25;
+
+void admit_queued(psf_server* sv, bool device_idle)
 {
     UploadStream up(sv);
+    const Clock::time_point t0 = Clock::now();
     while (!sv->queue.empty() && sv->active.size() < sv->cfg.max_active) {
+        if (!device_idle && std::chrono::duration<double, std::milli>(Clock::now() - t0).count() > kAdmitBudgetMs)
+            break;
         Request* r = sv->queue.front();
         if (sv->wgs_in_use + r->wgs > sv->wg_capacity) break;
         if (sv->mem_in_use + r->footprint > sv->cfg.mem_budget) break;
@@ -237,6 +255,7 @@ int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** o
     sv->ctx = ctx;
     sv->cfg = c;
     hipError_t e = hipStreamCreateWithFlags(&sv->second, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->third, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->ev_upload, hipEventDisableTiming);
     for (int q = 0; q < 2 && e == hipSuccess; ++q) {
@@ -251,6 +270,7 @@ int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** o
             e = hipHostMalloc(reinterpret_cast<void**>(&sv->h_ttab[q]), sizeof(TraceScene) * kMaxActive,
                               hipHostMallocDefault);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->ev_step[q], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->ev_rec[q], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->ev_done[q], hipEventDisableTiming);
     }
     if (e != hipSuccess) {
@@ -271,10 +291,12 @@ int psf_server_destroy(psf_server* sv)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (sv->second) (void)hipStreamSynchronize(sv->second);
+    if (sv->third) (void)hipStreamSynchronize(sv->third);
     if (sv->upload) (void)hipStreamSynchronize(sv->upload);
     for (auto& kv : sv->reqs)
         if (kv.second->scene) {
             scene_free(kv.second->scene, kv.second->hist2);
+            scene_free(kv.second->scene, kv.second->u_hist2);
             scene_release(kv.second->scene);
         }
     for (Basis& b : sv->bases) {
@@ -288,9 +310,11 @@ int psf_server_destroy(psf_server* sv)
         if (sv->h_wtab[q]) (void)hipHostFree(sv->h_wtab[q]);
         if (sv->h_ttab[q]) (void)hipHostFree(sv->h_ttab[q]);
         if (sv->ev_step[q]) (void)hipEventDestroy(sv->ev_step[q]);
+        if (sv->ev_rec[q]) (void)hipEventDestroy(sv->ev_rec[q]);
         if (sv->ev_done[q]) (void)hipEventDestroy(sv->ev_done[q]);
     }
     if (sv->second) (void)hipStreamDestroy(sv->second);
+    if (sv->third) (void)hipStreamDestroy(sv->third);
     if (sv->upload) (void)hipStreamDestroy(sv->upload);
     if (sv->ev_upload) (void)hipEventDestroy(sv->ev_upload);
     ctx->servers.erase(sv);
@@ -402,7 +426,7 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
     SRV_HIPCHK(ctx, hipSetDevice(ctx->device));
     // with nothing on the device, admit first so that this call launches work
     if (sv->active.empty()) {
-        admit_queued(sv);
+        admit_queued(sv, true);
         sv->st.admit_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     }
 
@@ -415,10 +439,11 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
     if (launch) {
         const int W = static_cast<int>(sv->cfg.window);
         const int q = static_cast<int>(sv->launched & 1);
-        hipStream_t sa = ctx->stream, sb = sv->second;
+        hipStream_t sa = ctx->stream, sb = sv->second, sr = sv->third;
         std::vector<psf_scene*> sc(n);
         std::vector<int> nsteps(n), ld(n, W);
         std::vector<double*> hist(n);
+        std::vector<float*> uh(n);
         uint32_t wblocks = 0, tblocks = 0;
         for (int j = 0; j < n; ++j) {
             Request* r = run[j];
@@ -426,13 +451,13 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
             sc[j] = s;
             nsteps[j] = static_cast<int>(std::min<uint32_t>(static_cast<uint32_t>(W), r->nframes - r->sched));
             hist[j] = q == 0 ? s->dev.hist : r->hist2;
+            uh[j] = q == 0 ? s->dev.u_hist : r->u_hist2;
             const Basis& b = sv->bases[r->basis];
-            sv->h_wtab[q][j] = WindowScene{b.Phi, s->dev.ldF, s->F, static_cast<int>(s->d.K), hist[j], s->dev.u_hist,
-                                           wblocks};
+            sv->h_wtab[q][j] = WindowScene{b.Phi, s->dev.ldF, s->F, static_cast<int>(s->d.K), hist[j], uh[j], wblocks};
             wblocks += window_blocks(W, s->dev.ldF);
             TraceScene t{};
             t.P = scene_trace_params(s);
-            t.u_hist = s->dev.u_hist;
+            t.u_hist = uh[j];
             t.ldU = s->dev.ldF;
             t.parts = s->dev.parts;
             t.pid = s->dev.pid;
@@ -442,17 +467,23 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
             sv->h_ttab[q][j] = t;
             tblocks += (s->n_particles + 255) / 256;
         }
-        // the round two back read this parity's history buffers
-        if (sv->launched >= 2) SRV_HIPCHK(ctx, hipStreamWaitEvent(sa, sv->ev_done[q], 0));
+        // the round two back used this parity's history buffers: P waits for
+        // its reconstruction (omega history), R for its particles (u history)
+        if (sv->launched >= 2) {
+            SRV_HIPCHK(ctx, hipStreamWaitEvent(sa, sv->ev_rec[q], 0));
+            SRV_HIPCHK(ctx, hipStreamWaitEvent(sr, sv->ev_done[q], 0));
+        }
         if (sv->upload_pending) {   // newly admitted scenes' uploads
             SRV_HIPCHK(ctx, hipStreamWaitEvent(sa, sv->ev_upload, 0));
             sv->upload_pending = false;
         }
         SRV_HIPCHK(ctx, launch_persistent_batch(sv->batch[q], sc.data(), nsteps.data(), hist.data(), ld.data(), n, sa));
         SRV_HIPCHK(ctx, hipEventRecord(sv->ev_step[q], sa));
-        SRV_HIPCHK(ctx, hipStreamWaitEvent(sb, sv->ev_step[q], 0));
-        SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_wtab[q], sv->h_wtab[q], sizeof(WindowScene) * n, hipMemcpyHostToDevice, sb));
-        SRV_HIPCHK(ctx, launch_reconstruct_window_multi(PSF_PHI_FP32, W, sv->d_wtab[q], n, wblocks, sb));
+        SRV_HIPCHK(ctx, hipStreamWaitEvent(sr, sv->ev_step[q], 0));
+        SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_wtab[q], sv->h_wtab[q], sizeof(WindowScene) * n, hipMemcpyHostToDevice, sr));
+        SRV_HIPCHK(ctx, launch_reconstruct_window_multi(PSF_PHI_FP32, W, sv->d_wtab[q], n, wblocks, sr));
+        SRV_HIPCHK(ctx, hipEventRecord(sv->ev_rec[q], sr));
+        SRV_HIPCHK(ctx, hipStreamWaitEvent(sb, sv->ev_rec[q], 0));
         SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_ttab[q], sv->h_ttab[q], sizeof(TraceScene) * n, hipMemcpyHostToDevice, sb));
         SRV_HIPCHK(ctx, launch_trace_window_multi(sv->d_ttab[q], n, tblocks, sb));
         SRV_HIPCHK(ctx, hipEventRecord(sv->ev_done[q], sb));
@@ -473,7 +504,7 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
     // ---- admit while the device runs the round; admitted scenes join the next ------
     {
         const Clock::time_point ta = Clock::now();
-        admit_queued(sv);
+        admit_queued(sv, !launch);
         sv->st.admit_ms += std::chrono::duration<double, std::milli>(Clock::now() - ta).count();
     }
 
