@@ -187,6 +187,25 @@ PSF_API int psf_integrate(psf_scene* scene, int nframes);
    holds W * (4 * F' + 8 * K) + 8 * K more bytes, F' = F rounded up to 256. */
 PSF_API int psf_set_frame_window(psf_scene* scene, uint32_t frames);
 
+/* ---- scene cache -----------------------------------------------------------------
+ * A versioned file of a scene's precomputed inputs: Phi (optional, FP64, F x K
+ * column-major), lambda, g (optional) and C (FP64, K matrices K x K), i.e. what
+ * PSFSolver::buildLaplace / buildEigenFunctions / buildAdvection produce
+ * (Solver/SolverImpl/psfsolver.cpp:236-551), with the grid, resolution, AABB and
+ * K.  No counterpart in the reference, which rebuilds them on every start.
+ * The header carries a format version and an FNV-1a checksum of the payload;
+ * readers refuse other versions (PSF_ERR_UNSUPPORTED), truncated or corrupt
+ * files (PSF_ERR_ARG).  Writes go to path.tmp, then rename.
+ * psf_cache_write / psf_cache_read_header / psf_cache_read touch no GPU.
+ * psf_scene_load_cache uploads a cache into a scene created with the same
+ * grid, resolution and K, through psf_scene_upload_basis and
+ * psf_scene_upload_advection, in the scene's precisions. */
+PSF_API int psf_cache_write(const char* path, const psf_scene_desc* desc, const double* Phi, const double* lambda,
+                            const double* gravity, const double* C);
+PSF_API int psf_cache_read_header(const char* path, psf_scene_desc* desc, uint32_t* has_phi, uint32_t* has_gravity);
+PSF_API int psf_cache_read(const char* path, double* Phi, double* lambda, double* gravity, double* C);
+PSF_API int psf_scene_load_cache(psf_scene* scene, const char* path);
+
 /* ---- particle density volume --------------------------------------------------
  * The histogram -> volume step of the GPU solver's particle drawing
  * (Solver/SolverImpl/psfsolvergpu.cpp:876-913, Resources/Volume/volume.comp:1-17;

@@ -81,6 +81,10 @@ def load():
         "psf_scene_generate_box_basis": (i32, [vp, vp, vp, vp, vp]),
         "psf_scene_build_box_advection": (i32, [vp, vp, vp, vp]),
         "psf_set_density_grid": (i32, [vp, u32, u32, u32, i32]),
+        "psf_cache_write": (i32, [ct.c_char_p, vp, vp, vp, vp, vp]),
+        "psf_cache_read_header": (i32, [ct.c_char_p, vp, vp, vp]),
+        "psf_cache_read": (i32, [ct.c_char_p, vp, vp, vp, vp]),
+        "psf_scene_load_cache": (i32, [vp, ct.c_char_p]),
         "psf_density": (i32, [vp, vp, vp]),
         "psf_set_params": (i32, [vp, dbl, dbl, i32, flt]),
         "psf_set_step_path": (i32, [vp, i32]),

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import atexit
 import ctypes as ct
+import os
 import weakref
 
 import numpy as np
@@ -85,6 +86,54 @@ def footprint(desc) -> int:
     return int(out.value)
 
 
+def write_cache(path, grid: Grid, resolution, lam, C, Phi_cm=None, gravity=None, aabb_min=(-0.5, -0.5, -0.5),
+                aabb_max=(0.5, 0.5, 0.5)):
+    """Scene cache file (psf_cache_write): lambda (K,), C (K, K, K) with
+    C[i, b, a] = C_i(a, b), and optionally Phi (K, F) and g (K,), all FP64."""
+    lib = L.load()
+    lam = np.ascontiguousarray(lam, np.float64)
+    K = lam.size
+    d = make_desc(grid, resolution, K, aabb_min=aabb_min, aabb_max=aabb_max)
+    C = np.ascontiguousarray(C, np.float64)
+    assert C.shape == (K, K, K)
+    P = None if Phi_cm is None else np.ascontiguousarray(Phi_cm, np.float64)
+    if P is not None:
+        assert P.shape == (K, grid.nF)
+    g = None if gravity is None else np.ascontiguousarray(gravity, np.float64)
+    rc = lib.psf_cache_write(os.fsencode(path), ct.byref(d), None if P is None else _ptr(P), _ptr(lam),
+                             None if g is None else _ptr(g), _ptr(C))
+    if rc != 0:
+        raise L.PsfError(rc, f"psf_cache_write({path})")
+
+
+def read_cache_header(path):
+    """(desc, has_phi, has_gravity) of a scene cache file."""
+    lib = L.load()
+    d = L.SceneDesc()
+    hp, hg = ct.c_uint32(), ct.c_uint32()
+    rc = lib.psf_cache_read_header(os.fsencode(path), ct.byref(d), ct.byref(hp), ct.byref(hg))
+    if rc != 0:
+        raise L.PsfError(rc, f"psf_cache_read_header({path})")
+    return d, bool(hp.value), bool(hg.value)
+
+
+def read_cache(path):
+    """dict(lam, C, Phi or None, gravity or None, desc) of a scene cache file,
+    checked against its checksum."""
+    d, has_phi, has_g = read_cache_header(path)
+    K = int(d.K)
+    nF = Grid(d.nx, d.ny, d.nz).nF
+    lam = np.empty(K)
+    C = np.empty((K, K, K))
+    P = np.empty((K, nF)) if has_phi else None
+    g = np.empty(K) if has_g else None
+    rc = L.load().psf_cache_read(os.fsencode(path), None if P is None else _ptr(P), _ptr(lam),
+                                 None if g is None else _ptr(g), _ptr(C))
+    if rc != 0:
+        raise L.PsfError(rc, f"psf_cache_read({path})")
+    return dict(desc=d, lam=lam, C=C, Phi=P, gravity=g)
+
+
 class Scene:
     """One scene resident on the device (C, Phi, omega, u, particles)."""
 
@@ -153,6 +202,10 @@ class Scene:
         lam = np.ascontiguousarray(tab[:, 0])
         self._c(self.ctx.lib.psf_scene_build_box_advection(self.h, _ptr(freqs), _ptr(amps), _ptr(lam)),
                 "build_box_advection")
+
+    def load_cache(self, path):
+        """Phi, lambda, g and C from a scene cache file (psf_scene_load_cache)."""
+        self._c(self.ctx.lib.psf_scene_load_cache(self.h, os.fsencode(path)), f"load_cache({path})")
 
     def check_signs(self, words):
         w = np.ascontiguousarray(words, dtype=np.uint32)
