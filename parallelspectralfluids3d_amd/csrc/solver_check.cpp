@@ -3,10 +3,13 @@
 // state (here read from files), buildAdvection() uploads it, integrate() runs
 // once per frame, getters read the results back.
 //
-//   solver_check <dir>
+//   solver_check <dir> [extras]
 // <dir>/meta.txt: nx ny nz resolution K N frames dt viscosity gravity lifetime c_precision phi_precision
 // <dir>/{phi,lambda,gravity,C,omega}.bin (float64), parts.bin (float32 N x 4)
 // writes <dir>/{omega_out.bin (float64), u_out.bin, parts_out.bin (float32)}
+// With "extras" it then also writes the 16^3 density volume of the final
+// particles (<dir>/volume_out.bin, uint16 half bits) and runs the reference's
+// benchmark (startBenchmark, 4 recorded frames) into <dir>/benchmark.csv.
 #include <cstdio>
 #include <fstream>
 #include <string>
@@ -52,6 +55,12 @@ public:
         buildAdvection();
     }
 
+    void benchmarkSetup(const std::string& path, unsigned frames)
+    {
+        benchmarkPath = path;
+        maxFramesBenchmark = frames;
+    }
+
 protected:
     void buildLaplace() override
     {
@@ -79,10 +88,11 @@ private:
 
 int main(int argc, char** argv)
 {
-    if (argc != 2) {
-        std::fprintf(stderr, "usage: %s <dir>\n", argv[0]);
+    if (argc != 2 && argc != 3) {
+        std::fprintf(stderr, "usage: %s <dir> [extras]\n", argv[0]);
         return 2;
     }
+    const bool extras = argc == 3 && std::string(argv[2]) == "extras";
     const std::string dir = argv[1];
     std::ifstream meta(dir + "/meta.txt");
     uint32_t nx, ny, nz;
@@ -105,6 +115,14 @@ int main(int argc, char** argv)
         write_bin(dir + "/omega_out.bin", solver.getBasisCoefficients());
         write_bin(dir + "/u_out.bin", solver.getVelocityField());
         write_bin(dir + "/parts_out.bin", solver.getParticles());
+        if (extras) {
+            solver.enableDensity(16, 16, 16);
+            write_bin(dir + "/volume_out.bin", solver.getDensityVolume());
+            solver.benchmarkSetup(dir + "/benchmark.csv", 3);
+            solver.startBenchmark(true);
+            for (int f = 0; f < 5; ++f) solver.integrate();
+            if (solver.benchmarking()) throw std::runtime_error("benchmark did not end");
+        }
     } catch (const psf::Error& e) {
         std::fprintf(stderr, "psf error %d: %s\n", e.status(), e.what());
         return 1;

@@ -46,3 +46,34 @@ def test_hip_solver_frames(tmp_path, cprec, phiprec, tol_w):
     assert np.linalg.norm(w_gpu - w) / np.linalg.norm(w) <= tol_w
     assert np.abs(u_gpu - u).max() <= 1e-5 * np.abs(u).max()
     assert np.abs(p_gpu[:, :3] - p[:, :3]).max() <= 1e-5
+
+
+def test_hip_solver_density_and_benchmark(tmp_path):
+    """The density volume of HipSolver's particles equals the oracle's; the
+    reference's benchmark.csv is written with per-frame GPU times."""
+    print("[gpu-case] solver_check extras", flush=True)
+    sd = make_scene(n=8, K=16)
+    N, frames, dt = 1024, 10, 0.01
+    p0 = particles_uniform(sd, N, seed=12)
+    d = str(tmp_path)
+    with open(os.path.join(d, "meta.txt"), "w") as fh:
+        fh.write(f"8 8 8 {sd.resolution!r} {sd.K} {N} {frames} {dt!r} 0.0 0 60.0 4 1\n")
+    np.ascontiguousarray(sd.Phi, np.float64).tofile(os.path.join(d, "phi.bin"))
+    sd.lam.astype(np.float64).tofile(os.path.join(d, "lambda.bin"))
+    sd.g.astype(np.float64).tofile(os.path.join(d, "gravity.bin"))
+    np.ascontiguousarray(sd.C, np.float64).tofile(os.path.join(d, "C.bin"))
+    sd.omega.astype(np.float64).tofile(os.path.join(d, "omega.bin"))
+    p0.astype(np.float32).tofile(os.path.join(d, "parts.bin"))
+    r = subprocess.run([EXE, d, "extras"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    p_gpu = np.fromfile(os.path.join(d, "parts_out.bin"), np.float32).reshape(N, 4)
+    vol = np.fromfile(os.path.join(d, "volume_out.bin"), np.uint16).reshape(16, 16, 16)
+    _, ref = O.density(p_gpu, sd.aabb_min, sd.aabb_max, (16, 16, 16))
+    assert np.array_equal(vol, ref.view(np.uint16))
+    rows = open(os.path.join(d, "benchmark.csv")).read().split("\n")
+    assert rows[0] == str(N) and rows[1] == str(sd.K) and rows[2] == "8,8,8"
+    assert len(rows[3].split(",")) == 3
+    times = [r for r in rows[5:9]]
+    assert all(len(t.split(",")) == 2 and min(float(x) for x in t.split(",")) > 0.0 for t in times)
+    avg = [float(x) for x in rows[-1].split(",")]
+    assert avg[0] > 0.0 and avg[1] > 0.0
