@@ -376,6 +376,98 @@ def large_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def config2_bench(args):
+    """The plain drop-in form at full size: 32^3 box with a voxelised central
+    sphere (cells whose centre lies within radius 0.25 are solid), the K=128
+    lowest eigenpairs of the pruned DEC 1-form Laplacian (host eigensolve),
+    an upward force on modes 0-7, FP32 C on the plain path and FP32 Phi, one
+    integrate() per frame (no frame window), 2000 frames at dt=0.005, 2^18
+    particles in fluid cells.  Checks: omega after all frames against the
+    FP64 oracle, and the particles against the oracle's particle loop fed
+    with the oracle's own u (FP64 Phi), frame by frame."""
+    from parallelspectralfluids3d_amd import _lib as L
+    from parallelspectralfluids3d_amd import advection, basis, hip
+    from parallelspectralfluids3d_amd.grid import Grid, sphere_obstacle_mask
+    n, K, N, frames, dt = 32, 128, 1 << 18, 2000, 0.005
+    g = Grid(n, n, n)
+    h = 1.0 / n
+    amin, amax = (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)
+    inside = sphere_obstacle_mask(g, h, amin)
+    cache = args.config2_cache
+    t_eig = t_c = None
+    if cache and os.path.exists(cache):
+        sc = hip.read_cache(cache)
+        assert (sc["desc"].nx, sc["desc"].K) == (n, K)
+        Phi, lam, C, force = sc["Phi"], sc["lam"], sc["C"], sc["gravity"]
+        log(f"config2: basis, C and forcing read from the scene cache {cache}")
+    else:
+        t = time.time()
+        Phi, lam = basis.laplacian_eigenbasis(g, h, inside, K)
+        t_eig = time.time() - t
+        log(f"config2: eigenbasis K={K} in {t_eig:.1f}s")
+        t = time.time()
+        V = (basis.curl_operator(g, h) @ Phi.T).T
+        C = advection.build_advection(g, h, inside, Phi, lam, V)
+        del V
+        t_c = time.time() - t
+        log(f"config2: C in {t_c:.1f}s")
+        force = -basis.project(Phi, basis.gravity_field(g, inside))
+        force[8:] = 0.0
+        if cache:
+            hip.write_cache(cache, g, h, lam, C, Phi_cm=Phi, gravity=force, aabb_min=amin, aabb_max=amax)
+            log(f"config2: scene cache written to {cache}")
+        if args.config2_build_only:
+            return
+    omega = basis.unit_energy_omega(K, 0)
+    rng = np.random.default_rng(21)
+    x, y, z = g.voxel_coords()
+    vox = np.flatnonzero(inside)
+    pick = vox[rng.integers(0, vox.size, N)]
+    cell = np.stack([x[pick], y[pick], z[pick]], 1).astype(np.float64)
+    parts = np.zeros((N, 4), np.float32)
+    parts[:, :3] = np.asarray(amin) + (cell + rng.random((N, 3))) * h
+    parts[:, 3] = 60.0
+    ctx = hip.Context(0)
+    s = hip.Scene(ctx, hip.make_desc(g, h, K, L.C_FP32, L.PHI_FP32, N, amin, amax))
+    s.upload_basis(Phi, lam, force)
+    s.upload_advection(C)
+    s.set_omega(omega)
+    s.set_params(dt, 0.0, True, lifetime=60.0)
+    s.set_particles(parts)
+    ms = s.time_phase(L.PHASE_FRAME, frames)
+    w_gpu, p_gpu = s.get_omega(), s.get_particles()
+    s.close()
+    ctx.close()
+    log(f"config2: {frames} frames in {ms:.1f} ms on the GPU")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    nthreads = min(16, os.cpu_count() or 1)
+    m = O.Mesh((n, n, n), h, amin, inside=inside)
+    w, p = omega.copy(), parts.copy()
+    t = time.time()
+    for f in range(frames):
+        w = O.reduced_step(w, C, lam, force, dt, 0.0, True, nthreads=nthreads)
+        u = O.reconstruct(Phi, w, nthreads=nthreads)
+        p = O.particles(m, u, p, resolution=h, timestep=dt, aabb_min=amin, aabb_max=amax, lifetime=60.0,
+                        seed=0, step=f, nthreads=nthreads)
+        if f % 500 == 499:
+            log(f"config2: oracle frame {f + 1}")
+    t_ora = time.time() - t
+    err = np.abs(p_gpu[:, :3] - p[:, :3]).max(axis=1)
+    out = {"config2": {
+        "workload": f"{n}^3 box, central sphere obstacle r=0.25, K={K} eigenbasis of the pruned M, upward force on "
+                    f"modes 0-7, FP32 C (plain path), FP32 Phi, {frames} frames at dt={dt}, one integrate() per "
+                    f"frame, {N} particles in fluid cells",
+        "host_eigensolve_s": t_eig, "host_C_build_s": t_c,
+        "ms_per_frame": ms / frames, "particle_steps_per_s": N * frames / (ms * 1e-3),
+        "oracle_ms_per_frame": 1e3 * t_ora / frames, "oracle_threads": nthreads,
+        "omega_rel_err_vs_fp64_oracle": float(np.linalg.norm(w_gpu - w) / np.linalg.norm(w)),
+        "particle_err_max": float(err.max()), "particle_err_p99": float(np.percentile(err, 99)),
+        "lifetimes_equal": bool(np.array_equal(p_gpu[:, 3], p[:, 3]))}}
+    print(json.dumps(out), flush=True)
+
+
 def counter_pass_copies(frames, ms):
     """Three device-to-device copies of 1 GiB with the HIP runtime's own copy
     kernel: the reference for the FETCH_SIZE / WRITE_SIZE counters (each copy
@@ -418,6 +510,11 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-density", action="store_true")
+    ap.add_argument("--config2", action="store_true",
+                    help="the 32^3 sphere-obstacle K=128 drop-in configuration at full size, against the oracle")
+    ap.add_argument("--config2-cache", default=None,
+                    help="scene cache file of the configuration: read if present, else built and written")
+    ap.add_argument("--config2-build-only", action="store_true", help="build and write the cache, no GPU work")
     ap.add_argument("--serving-only", action="store_true", help="only the serving section")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rate", type=float, default=200.0)
@@ -442,6 +539,10 @@ def main():
 
     if args.large:
         large_bench(args)
+        return
+
+    if args.config2:
+        config2_bench(args)
         return
 
     if args.serving_only:
