@@ -43,6 +43,7 @@ FP64_PEAK = 78.6e12
 RECON_TRAFFIC_64_256 = 821_147_648          # k_reconstruct<float, 1>: 399,392 KiB read, 3,120 KiB written
 WINDOW_TRAFFIC_64_256_W32 = 922_556_416     # k_reconstruct_window<float, 2, 32>: 400,547 / 99,840 KiB
 TRACE_TRAFFIC_64_1M = 46_614_938            # k_trace: 14,569 / 16,384 KiB
+TRACE_FRAMES_TRAFFIC_64_1M = None           # k_trace_frames (8 frames per launch): not yet counted
 
 
 def log(msg):
@@ -526,8 +527,8 @@ def main():
                          "size-independent checks; prints one line and exits (several minutes, ~60 GB of host memory)")
     ap.add_argument("--counter-pass", action="store_true",
                     help="short run for hardware-counter collection: the frame loop, one persistent launch of "
-                         "--reduced-steps steps, then three plain 1 GiB device copies that calibrate the byte "
-                         "counters; prints one line and exits")
+                         "--reduced-steps steps, 20 single-frame particle passes and reconstructions, then three "
+                         "plain 1 GiB device copies that calibrate the byte counters; prints one line and exits")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU); "
@@ -580,6 +581,8 @@ def main():
     log(f"{args.steps} frames: {ms:.2f} ms (events), {1e3 * wall:.2f} ms (wall)")
     if args.counter_pass:
         s.step(args.reduced_steps)     # one persistent launch of that many steps
+        s.time_phase(L.PHASE_TRACE, 20)        # single-frame kernels of the one-at-a-time path
+        s.time_phase(L.PHASE_RECON, 20)
         ctx.sync()
         s.close()
         ctx.close()
@@ -591,6 +594,11 @@ def main():
     reps = 100
     ms_tr = s.time_phase(L.PHASE_TRACE, reps) / reps
     kern = {"k_trace": ms_tr}
+    ms_trf = None
+    if W >= L.TRACE_FUSE:
+        # the frame loop's particle launches: TRACE_FUSE frames each
+        ms_trf = s.time_phase(L.PHASE_TRACE_FUSED, reps // 4) / (reps // 4)
+        kern = {"k_trace_frames": ms_trf / L.TRACE_FUSE}
     if W > 1:
         ms_wrec = s.time_phase(L.PHASE_WINDOW_RECON, 10) / 10
         ms_wstep = s.time_phase(L.PHASE_NSTEP, W)
@@ -633,6 +641,12 @@ def main():
     roof["k_trace"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9, achieved=trace_bytes / (ms_tr * 1e-3) / 1e9,
                            algorithmic_bytes_per_launch=trace_bytes, traffic=TRACE_TRAFFIC_64_1M if
                            (n, N) == (64, 1 << 20) else None)
+    if ms_trf is not None:
+        fb = N * 32 + L.TRACE_FUSE * F * 4       # records once, each frame's u once
+        roof["k_trace_frames"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
+                                      achieved=fb / (ms_trf * 1e-3) / 1e9, algorithmic_bytes_per_launch=fb,
+                                      frames_per_launch=L.TRACE_FUSE,
+                                      traffic=TRACE_FRAMES_TRAFFIC_64_1M if (n, N) == (64, 1 << 20) else None)
     rec_bytes = F * K * 4 + F * 4 + K * 8
     roof["k_reconstruct"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
                                  achieved=rec_bytes / (ms_rec * 1e-3) / 1e9, algorithmic_bytes_per_launch=rec_bytes,
@@ -701,8 +715,13 @@ def main():
             errors[name] = e
             r.close()
         log("omega rel err at 1000 steps: " + ", ".join(f"{k}={v[-1]:.2e}" for k, v in errors.items()))
-    serving = None if args.no_serving else serving_bench(ctx, args.serving_requests, args.serving_rate)
     ctx.close()
+    serving = None
+    if not args.no_serving:
+        # a context of its own: no memory pool, streams or scenes left by the sections above
+        sctx = hip.Context(0)
+        serving = serving_bench(sctx, args.serving_requests, args.serving_rate)
+        sctx.close()
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
     if cpu:

@@ -173,7 +173,11 @@ PSF_API int psf_get_velocity(psf_scene* scene, float* u);
    xOfs,yOfs,zOfs,xOfs-,xOfs+,yOfs-,yOfs+,zOfs-,zOfs+ of advected particles
    (-1 for respawned ones) */
 PSF_API int psf_trace(psf_scene* scene, int32_t* cells);
-/* nframes x AbstractSolver::integrate(): step, reconstruct, trace */
+/* nframes x AbstractSolver::integrate(): step, reconstruct, trace.  With a
+   frame window the particle passes of consecutive frames are deferred and
+   run up to 8 frames per launch (the same bits as one pass per frame); every
+   call that reads or replaces the particles, psf_density and psf_ctx_sync
+   launch the deferred passes first. */
 PSF_API int psf_integrate(psf_scene* scene, int nframes);
 /* Frame window (no counterpart in the reference; a scheduling choice inside
    integrate()).  With frames = W > 1, when integrate() finds no frame
@@ -234,7 +238,9 @@ PSF_API int psf_density(psf_scene* scene, uint32_t* counts, uint16_t* volume);
  *          recomputes the u of the frames already computed ahead),
  *        6 a density count pass of the stored particles (adds to the
  *          counters), 7 a density volume pass (clears the counters),
- *        8 a full integrate frame followed by psf_density without copies. */
+ *        8 a full integrate frame followed by psf_density without copies,
+ *        9 one launch of the particle passes the windowed frame loop fuses
+ *          (8 frames; needs a frame window of at least 8). */
 PSF_API int psf_time_phase(psf_scene* scene, int phase, int count, double* ms);
 
 /* ---- scene server (continuous batching) ----------------------------------------

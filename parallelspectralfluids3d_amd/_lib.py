@@ -18,7 +18,8 @@ C_FP64, C_FP32, C_BF16, C_FP16, C_BF16X2 = 0, 1, 2, 3, 4
 PHI_FP64, PHI_FP32 = 0, 1
 PATH_PLAIN, PATH_PERSISTENT = 0, 1
 PHASE_STEP, PHASE_RECON, PHASE_TRACE, PHASE_FRAME, PHASE_NSTEP, PHASE_WINDOW_RECON = 0, 1, 2, 3, 4, 5
-PHASE_DENSITY_COUNT, PHASE_DENSITY_VOLUME, PHASE_FRAME_DENSITY = 6, 7, 8
+PHASE_DENSITY_COUNT, PHASE_DENSITY_VOLUME, PHASE_FRAME_DENSITY, PHASE_TRACE_FUSED = 6, 7, 8, 9
+TRACE_FUSE = 8   # frames per fused particle launch of the windowed frame loop
 
 _STATUS = {0: "PSF_OK", -1: "PSF_ERR_ARG", -2: "PSF_ERR_HIP", -3: "PSF_ERR_NOMEM", -4: "PSF_ERR_STATE",
            -5: "PSF_ERR_UNSUPPORTED", -6: "PSF_ERR_NONFINITE"}

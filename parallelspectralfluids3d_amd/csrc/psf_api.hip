@@ -90,6 +90,9 @@ static void pool_release(psf_ctx* c)
 
 namespace {
 
+// frames whose particle passes are deferred and launched together
+constexpr uint32_t kTraceFuse = 8;
+
 int set_err(psf_ctx* ctx, int code, const char* fmt, ...)
 {
     if (ctx) {
@@ -245,8 +248,25 @@ int enqueue_reconstruct(psf_scene* s)
     return PSF_OK;
 }
 
+// Particle passes of deferred frames, one launch (the per-frame bits; the
+// density grid, if it counts in the particle pass, counts the last frame).
+int flush_traces(psf_scene* s)
+{
+    if (s->trace_pending == 0) return PSF_OK;
+    TraceParams p = trace_params(s);
+    DensityGrid D;
+    if (int rc = density_for_trace(s, D)) return rc;
+    HIPCHK(s->ctx, launch_trace_frames(p, win_u(s, s->win_buf) + size_t(s->trace_c0) * s->dev.ldF, s->dev.ldF,
+                                       s->trace_pending, s->dev.parts, s->dev.pid, s->n_particles, D,
+                                       s->ctx->stream));
+    s->frame += s->trace_pending;
+    s->trace_pending = 0;
+    return PSF_OK;
+}
+
 int enqueue_trace(psf_scene* s, int32_t* dcells)
 {
+    if (int rc = flush_traces(s)) return rc;
     TraceParams p = trace_params(s);
     DensityGrid D;
     if (int rc = density_for_trace(s, D)) return rc;
@@ -272,6 +292,7 @@ int enqueue_window_fill(psf_scene* s, uint32_t b, hipStream_t st)
 // changes the scene's state or runs a single phase calls this first.
 int window_flush(psf_scene* s)
 {
+    if (int rc = flush_traces(s)) return rc;
     hipStream_t m = s->ctx->stream;
     if (s->win_ahead) {   // the side stream's window also moved dev.omega: wait for it
         HIPCHK(s->ctx, hipStreamWaitEvent(m, s->ev_ready[s->win_buf ^ 1u], 0));
@@ -328,7 +349,12 @@ int enqueue_frame(psf_scene* s)
         }
     }
     s->win_cursor++;
-    return enqueue_trace(s, nullptr);
+    // defer this frame's particle pass; kTraceFuse frames (or the rest of
+    // the window, whose u rows the next refill overwrites) go as one launch
+    if (s->trace_pending == 0) s->trace_c0 = s->win_cursor - 1;
+    s->trace_pending++;
+    if (s->trace_pending == kTraceFuse || s->win_cursor == s->win_filled) return flush_traces(s);
+    return PSF_OK;
 }
 
 int refresh_visc(psf_scene* s, const std::vector<double>& lam)
@@ -373,6 +399,7 @@ int scene_read_results(psf_scene* s, double* omega, float* xyzw)
 }
 
 TraceParams scene_trace_params(const psf_scene* s) { return trace_params(s); }
+int scene_flush_traces(psf_scene* s) { return flush_traces(s); }
 int scene_check_status(psf_scene* s) { return check_status(s); }
 int ctx_set_error(psf_ctx* ctx, int code, const std::string& msg)
 {
@@ -422,6 +449,9 @@ const char* psf_last_error(const psf_ctx* ctx) { return ctx ? ctx->err.c_str() :
 int psf_ctx_sync(psf_ctx* ctx)
 {
     if (!ctx) return PSF_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    for (psf_scene* s : ctx->scenes)
+        if (int rc = flush_traces(s)) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return PSF_OK;
 }
@@ -764,6 +794,7 @@ int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
     if (!s || (!xyzw && n)) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
     if (n > s->d.max_particles) return set_err(s->ctx, PSF_ERR_ARG, "%u particles exceed max_particles %u", n, s->d.max_particles);
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if (int rc = flush_traces(s)) return rc;   // deferred passes belong to the particles being replaced
     std::vector<uint32_t>& pid = host_of(s)->pid;
     pid.resize(n);
     std::iota(pid.begin(), pid.end(), 0u);
@@ -805,6 +836,7 @@ int psf_get_particles(psf_scene* s, float* xyzw, uint32_t n)
     if (!s || (!xyzw && n)) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
     if (n != s->n_particles) return set_err(s->ctx, PSF_ERR_ARG, "asked for %u particles, scene holds %u", n, s->n_particles);
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if (int rc = flush_traces(s)) return rc;
     std::vector<float> sorted(size_t(n) * 4);
     HIPCHK(s->ctx, hipMemcpyAsync(sorted.data(), s->dev.parts, size_t(n) * 16, hipMemcpyDeviceToHost, s->ctx->stream));
     HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
@@ -870,7 +902,9 @@ int psf_integrate(psf_scene* s, int nframes)
 
 int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
 {
-    if (!s || !ms || count <= 0 || phase < 0 || phase > 8) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    if (!s || !ms || count <= 0 || phase < 0 || phase > 9) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    if (phase == 9 && (s->window < kTraceFuse || !s->has_phi))
+        return set_err(s->ctx, PSF_ERR_STATE, "phase 9 needs Phi and a frame window of at least %u", kTraceFuse);
     if (phase == 5 && s->window <= 1) return set_err(s->ctx, PSF_ERR_STATE, "phase 5 needs a frame window > 1");
     if (phase >= 6 && !s->dev.dcounts) return set_err(s->ctx, PSF_ERR_STATE, "phases 6 to 8 need a density grid");
     if (phase != 2 && phase != 6 && phase != 7) {
@@ -884,6 +918,7 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
         int rc0 = window_flush(s);
         if (rc0) return rc0;
     }
+    if (int rc0 = flush_traces(s)) return rc0;   // deferred passes belong before the timed region
     hipEvent_t a, b;
     HIPCHK(ctx, hipEventCreate(&a));
     HIPCHK(ctx, hipEventCreate(&b));
@@ -912,9 +947,19 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
                 if (rc == PSF_OK) rc = density_enqueue(s, nullptr);
             }
             break;
+        case 9:
+            // the frame loop's fused particle launch: kTraceFuse frames over
+            // rows 0.. of the current window buffer
+            for (int i = 0; i < count && rc == PSF_OK; ++i) {
+                s->trace_c0 = 0;
+                s->trace_pending = kTraceFuse;
+                rc = flush_traces(s);
+            }
+            break;
         default: rc = enqueue_steps(s, count); break;
         }
     }
+    if (rc == PSF_OK) rc = flush_traces(s);      // the timed frames' deferred passes
     if (rc == PSF_OK) {
         e = hipEventRecord(b, ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(b);

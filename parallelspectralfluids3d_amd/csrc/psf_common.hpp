@@ -85,6 +85,10 @@ struct psf_scene {
     uint32_t win_buf = 0;        // buffer of the frames being consumed (0: hist/u_hist, 1: hist_b/u_hist_b)
     bool win_pipeline = true;    // compute the next window on the side stream during this one
     bool win_ahead = false;      // the next window is queued on the side stream into the other buffer
+    // particle passes of frames already advanced but not yet launched: rows
+    // trace_c0 .. trace_c0 + trace_pending - 1 of the current window buffer
+    // (they run as one launch; every read of the particles launches them)
+    uint32_t trace_pending = 0, trace_c0 = 0;
     hipStream_t side = nullptr;
     hipEvent_t ev_ready[2] = {nullptr, nullptr};   // side stream: buffer filled
     hipEvent_t ev_fork = nullptr;                  // context stream position the side stream starts from

@@ -106,6 +106,7 @@ int density_enqueue_volume(psf_scene* s)
 // the last particle pass counted them, then the volume pass
 int density_enqueue(psf_scene* s, uint32_t* counts_host)
 {
+    if (int rc = scene_flush_traces(s)) return rc;   // the particles of every frame advanced so far
     if (s->dens_state != 1) {
         if (s->dens_state != 0)
             if (int rc = zero_counts(s)) return rc;
@@ -133,6 +134,7 @@ int psf_set_density_grid(psf_scene* s, uint32_t rx, uint32_t ry, uint32_t rz, in
                  rz > kMaxCellsPerAxis))
         return fail(ctx, PSF_ERR_ARG, "density grid: 1..1024 cells per axis, or 0,0,0 to remove it");
     DENS_HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (int rc = scene_flush_traces(s)) return rc;         // deferred passes count into the old grid
     DENS_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));   // earlier passes may still count into the old grid
     density_release(s);
     if (off) return PSF_OK;
@@ -164,6 +166,7 @@ int psf_density(psf_scene* s, uint32_t* counts, uint16_t* volume)
     psf_ctx* ctx = s->ctx;
     if (!s->dev.dcounts) return fail(ctx, PSF_ERR_STATE, "no density grid (psf_set_density_grid)");
     DENS_HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (int rc = scene_flush_traces(s)) return rc;
     if (int rc = density_enqueue(s, counts)) return rc;
     if (volume)
         DENS_HIPCHK(ctx, hipMemcpyAsync(volume, s->dev.dvol, cells_of(s) * 2, hipMemcpyDeviceToHost, ctx->stream));

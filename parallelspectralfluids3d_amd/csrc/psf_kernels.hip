@@ -665,6 +665,32 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
     if (D.counts) density_add(D, p.x, p.y, p.z);
 }
 
+// Several frames' particle passes in one launch: each thread carries its
+// particle through frames t = 0..nframes-1 with u_t = u0 + t * ldU and the
+// respawn frame P.frame + t, and stores it once (the same trace_one per
+// frame, so the bits of nframes single passes).  With D.counts set the
+// positions after the last frame are counted.
+__global__ __launch_bounds__(256) void k_trace_frames(TraceParams P, const float* __restrict__ u0, size_t ldU,
+                                                      uint32_t nframes, float4* __restrict__ parts,
+                                                      const uint32_t* __restrict__ pid, uint32_t n, DensityGrid D,
+                                                      uint32_t per)
+{
+    const uint32_t id = xcd_block(blockIdx.x, per) * 256u + threadIdx.x;
+    if (id >= n) return;
+    f32x4* rec = reinterpret_cast<f32x4*>(parts) + id;
+    const f32x4 v = __builtin_nontemporal_load(rec);
+    float4 p = make_float4(v.x, v.y, v.z, v.w);
+    const gptr<const float> u = as_global(u0);
+    const gptr<const uint32_t> gp = as_global(pid);
+    for (uint32_t t = 0; t < nframes; ++t) {
+        p = trace_one(P, u + static_cast<size_t>(t) * ldU, p, gp, id, nullptr);
+        P.frame++;
+    }
+    const f32x4 o = {p.x, p.y, p.z, p.w};
+    __builtin_nontemporal_store(o, rec);
+    if (D.counts) density_add(D, p.x, p.y, p.z);
+}
+
 // Density of the stored particles (the histogram pass of the GPU solver's
 // particle drawing, psfsolvergpu.cpp:893-899).  Particles often gather in
 // few cells (they respawn on a small emitter sheet and live at most
@@ -901,6 +927,15 @@ hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, con
     if (n == 0) return hipSuccess;
     const uint32_t per = xcd_per((n + 255) / 256);
     k_trace<<<per * kXcds, 256, 0, s>>>(p, u, parts, pid, n, cells, D, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_frames(const TraceParams& p, const float* u0, size_t ldU, uint32_t nframes, float4* parts,
+                               const uint32_t* pid, uint32_t n, const DensityGrid& D, hipStream_t s)
+{
+    if (n == 0 || nframes == 0) return hipSuccess;
+    const uint32_t per = xcd_per((n + 255) / 256);
+    k_trace_frames<<<per * kXcds, 256, 0, s>>>(p, u0, ldU, nframes, parts, pid, n, D, per);
     return hipGetLastError();
 }
 

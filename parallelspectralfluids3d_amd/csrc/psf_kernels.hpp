@@ -99,6 +99,10 @@ hipError_t launch_gather_column(const double* hist, int ld, int t, int K, double
 // counted into D's grid
 hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, const uint32_t* pid, uint32_t n,
                         int32_t* cells, const DensityGrid& D, hipStream_t s);
+// nframes particle passes in one launch, frame t with u0 + t * ldU and respawn
+// frame p.frame + t; D counts the positions after the last
+hipError_t launch_trace_frames(const TraceParams& p, const float* u0, size_t ldU, uint32_t nframes, float4* parts,
+                               const uint32_t* pid, uint32_t n, const DensityGrid& D, hipStream_t s);
 // counts of the n stored particles into D's grid (added to what it holds)
 hipError_t launch_density_count(const DensityGrid& D, const float4* parts, uint32_t n, hipStream_t s);
 // volume[v] = half(float(counts[v]) / 1e6) and counts[v] = 0, for v < n2048 * 2048

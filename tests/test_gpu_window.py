@@ -93,6 +93,34 @@ def test_window_state_changes_mid_window(hip_ctx):
     _same(one, win)
 
 
+def test_deferred_particle_passes(hip_ctx):
+    """Particle passes of windowed frames run several frames per launch; a
+    new particle set, a density read, a context sync and short lifetimes
+    (respawn draws follow the frame number) see every frame advanced so far."""
+    _say("deferred particle passes")
+    sd = make_scene(n=8, K=16)
+    one, win = _pair(hip_ctx, sd, C_FP64, PHI_FP32, 700, 16, lifetime=3.0)
+    fresh = particles_uniform(sd, 700, seed=9, lifetime=2.0)
+    for s in (one, win):
+        s.integrate(3)
+        s.set_particles(fresh)          # the 3 deferred passes ran on the old set first
+        s.integrate(5)
+    _same(one, win)
+    for s in (one, win):
+        s.set_density_grid((4, 4, 4), count_in_trace=True)
+        s.integrate(6)
+    c1, v1 = one.density()
+    c2, v2 = win.density()
+    assert np.array_equal(c1, c2) and np.array_equal(v1.view(np.uint16), v2.view(np.uint16))
+    for s in (one, win):
+        s.integrate(7)
+    hip_ctx.sync()
+    _same(one, win)
+    for s in (one, win):
+        s.integrate(21)
+    _same(one, win)
+
+
 def test_window_frames_match_oracle(hip_ctx):
     """40 frames with a window of 16 against the oracle run frame by frame."""
     _say("window vs oracle")
