@@ -906,8 +906,9 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
     if (phase == 9 && (s->window < kTraceFuse || !s->has_phi))
         return set_err(s->ctx, PSF_ERR_STATE, "phase 9 needs Phi and a frame window of at least %u", kTraceFuse);
     if (phase == 5 && s->window <= 1) return set_err(s->ctx, PSF_ERR_STATE, "phase 5 needs a frame window > 1");
-    if (phase >= 6 && !s->dev.dcounts) return set_err(s->ctx, PSF_ERR_STATE, "phases 6 to 8 need a density grid");
-    if (phase != 2 && phase != 6 && phase != 7) {
+    if (phase >= 6 && phase <= 8 && !s->dev.dcounts)
+        return set_err(s->ctx, PSF_ERR_STATE, "phases 6 to 8 need a density grid");
+    if (phase != 2 && phase != 6 && phase != 7 && phase != 9) {
         int rc = ready_for_step(s);
         if (rc) return rc;
     }
