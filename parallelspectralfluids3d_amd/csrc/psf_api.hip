@@ -88,6 +88,9 @@ static void pool_release(psf_ctx* c)
 
 }  // namespace psf
 
+// host storage of a scene's last viscosity factors (defined with psf_scene_host)
+std::vector<double>& visc_of(psf_scene* s);
+
 namespace {
 
 // frames whose particle passes are deferred and launched together
@@ -359,11 +362,13 @@ int enqueue_frame(psf_scene* s)
 
 int refresh_visc(psf_scene* s, const std::vector<double>& lam)
 {
-    // omega_k *= exp(viscosity * eigenValues(k) * timeStep)  (psfsolver.cpp:51-55)
-    std::vector<double> v(s->d.K);
+    // omega_k *= exp(viscosity * eigenValues(k) * timeStep)  (psfsolver.cpp:51-55);
+    // the factors stay in the scene's host storage, the copy's source
+    std::vector<double>& v = visc_of(s);
+    v.resize(s->d.K);
     for (uint32_t k = 0; k < s->d.K; ++k) v[k] = std::exp(s->nu * lam[k] * s->dt);
     HIPCHK(s->ctx, hipMemcpyAsync(s->dev.visc, v.data(), sizeof(double) * s->d.K, hipMemcpyHostToDevice, s->ctx->stream));
-    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    if (!s->async_uploads) HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
     return PSF_OK;
 }
 
@@ -374,8 +379,10 @@ int refresh_visc(psf_scene* s, const std::vector<double>& lam)
 struct psf_scene_host {
     std::vector<double> lambda;
     std::vector<uint32_t> pid;
+    std::vector<double> visc;    // source of the last viscosity upload
 };
 static psf_scene_host* host_of(psf_scene* s) { return reinterpret_cast<psf_scene_host*>(s->host); }
+std::vector<double>& visc_of(psf_scene* s) { return host_of(s)->visc; }
 
 namespace psf {
 // omega (dev.omega) and the particles in the caller's order, read with
@@ -501,6 +508,7 @@ int psf::scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOp
     s->pooled = opt.pooled;
     s->modes_per_wg = opt.modes_per_wg;
     s->win_pipeline = opt.frame_pipeline;
+    s->async_uploads = opt.async_uploads;
     counts(*desc, s->V, s->F, s->nZF, s->nYF, s->nXF);
     s->path = is_plain_precision(desc->c_precision) ? PSF_PATH_PLAIN : PSF_PATH_PERSISTENT;
     const size_t K = desc->K;
@@ -539,11 +547,12 @@ int psf::scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOp
     // vectorised reconstruction) and the status word
     if (!s->phi_borrowed) chk(hipMemsetAsync(s->dev.Phi, 0, phi_bytes, ctx->stream));
     chk(launch_zero_int(s->dev.status, 64, ctx->stream));
-    std::vector<double> ones(K, 1.0), zeros(K, 0.0);
-    chk(hipMemcpyAsync(s->dev.visc, ones.data(), K * 8, hipMemcpyHostToDevice, ctx->stream));
-    chk(hipMemcpyAsync(s->dev.gravity, zeros.data(), K * 8, hipMemcpyHostToDevice, ctx->stream));
-    chk(hipMemcpyAsync(s->dev.lambda, zeros.data(), K * 8, hipMemcpyHostToDevice, ctx->stream));
-    chk(hipStreamSynchronize(ctx->stream));
+    // constants by fill kernels: host-to-device copies from pageable memory
+    // measured 0.5 ms per scene in a long-running process (scene admission)
+    chk(launch_fill_f64(s->dev.visc, static_cast<int>(K), 1.0, ctx->stream));
+    chk(launch_fill_f64(s->dev.gravity, static_cast<int>(K), 0.0, ctx->stream));
+    chk(launch_fill_f64(s->dev.lambda, static_cast<int>(K), 0.0, ctx->stream));
+    if (!s->async_uploads) chk(hipStreamSynchronize(ctx->stream));
     if (e != hipSuccess) {
         psf_scene_destroy(s);
         return set_err(ctx, PSF_ERR_HIP, "scene init: %s", hipGetErrorString(e));
@@ -625,7 +634,7 @@ int psf_scene_upload_basis(psf_scene* s, const double* Phi, const double* lambda
     if (gravity) {
         HIPCHK(ctx, hipMemcpyAsync(s->dev.gravity, gravity, K * 8, hipMemcpyHostToDevice, ctx->stream));
     }
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!s->async_uploads) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     s->has_basis = true;
     return refresh_visc(s, host_of(s)->lambda);
 }
@@ -724,8 +733,10 @@ int psf_set_frame_window(psf_scene* s, uint32_t frames)
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
     if (int rc0 = window_flush(s)) return rc0;
     if (frames == s->window) return PSF_OK;
-    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
-    if (s->side) HIPCHK(s->ctx, hipStreamSynchronize(s->side));
+    if (!s->async_uploads) {   // (a server's scene sets its window once, before any launch)
+        HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+        if (s->side) HIPCHK(s->ctx, hipStreamSynchronize(s->side));
+    }
     auto drop = [&]() {
         sfree(s, s->dev.hist); sfree(s, s->dev.u_hist); sfree(s, s->dev.omega_view);
         sfree(s, s->dev.hist_b); sfree(s, s->dev.u_hist_b);
@@ -766,7 +777,7 @@ int psf_set_omega(psf_scene* s, const double* omega)
     if (int rc0 = window_flush(s)) return rc0;
     HIPCHK(s->ctx, hipMemcpyAsync(s->dev.omega, omega, s->d.K * 8, hipMemcpyHostToDevice, s->ctx->stream));
     HIPCHK(s->ctx, launch_zero_int(s->dev.status, 64, s->ctx->stream));
-    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    if (!s->async_uploads) HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
     s->has_omega = true;
     return PSF_OK;
 }

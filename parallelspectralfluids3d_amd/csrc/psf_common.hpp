@@ -97,6 +97,10 @@ struct psf_scene {
     void* persistent = nullptr;  // persistent-path state (PersistentState)
     bool pooled = false;         // device memory comes from ctx->pool and goes back there
     bool phi_borrowed = false;   // dev.Phi belongs to someone else (a server's shared basis)
+    // uploads return without waiting for the stream (a server's scenes: their
+    // host sources outlive the copies, and the first launch waits on the
+    // upload stream through an event)
+    bool async_uploads = false;
     // particle density grid: cells per axis (0: none), cell count rounded up
     // to a multiple of 2048, whether particle passes count their new positions,
     // and what the counters hold: 0 zeros, 1 the counts of the stored

@@ -14,6 +14,7 @@ struct SceneOptions {
     bool cells = true;             // allocate the cell-lookup debug buffer
     bool pooled = false;           // take device memory from the context's pool
     bool frame_pipeline = true;    // frame windows keep a second buffer pair and a side stream
+    bool async_uploads = false;    // set-up calls do not wait for their copies (host sources must outlive them)
 };
 
 int scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOptions& opt, psf_scene** out);

@@ -953,6 +953,19 @@ hipError_t launch_density_volume(uint32_t* counts, uint16_t* volume, size_t n204
     return hipGetLastError();
 }
 
+__global__ void k_fill_f64(double* p, int n, double v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_f64(double* p, int n, double v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_fill_f64<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero_int(int* p, int n, hipStream_t s)
 {
     k_zero_int<<<(n + 255) / 256, 256, 0, s>>>(p, n);

@@ -108,5 +108,6 @@ hipError_t launch_density_count(const DensityGrid& D, const float4* parts, uint3
 // volume[v] = half(float(counts[v]) / 1e6) and counts[v] = 0, for v < n2048 * 2048
 hipError_t launch_density_volume(uint32_t* counts, uint16_t* volume, size_t n2048, hipStream_t s);
 hipError_t launch_zero_int(int* p, int n, hipStream_t s);
+hipError_t launch_fill_f64(double* p, int n, double v, hipStream_t s);
 
 }  // namespace psf

@@ -156,6 +156,7 @@ int admit(psf_server* sv, Request* r)
     o.cells = false;
     o.pooled = true;
     o.frame_pipeline = false;   // the server pipelines rounds itself
+    o.async_uploads = true;     // sources below live in the request and the basis; P waits on ev_upload
     psf_scene* s = nullptr;
     int rc = scene_create_ex(sv->ctx, &d, o, &s);
     if (rc) return rc;
