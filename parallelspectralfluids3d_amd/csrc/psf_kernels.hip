@@ -537,7 +537,9 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
         // the y and z faces of columns xM and xP (xP = xM + 1 but at clamps).
         auto pair = [&](uint32_t i) { return *reinterpret_cast<gptr<const FluxPair>>(u + i); };
         const int dx = xP - xM;
-        auto second = [&](const FluxPair& q, uint32_t i) { return dx == 1 ? q.y : (dx == 0 ? q.x : u[i + dx]); };
+        // the x+ neighbour from the pair: dx is 1 (or 0 at a clamp); a dx of
+        // any other value is patched once, below, with its own loads
+        auto second = [&](const FluxPair& q, uint32_t) { return dx == 1 ? q.y : q.x; };
         // x component: voxels (xO, yM|yP, zM|zP), faces f5 (left) / f6 (right)
         const float s4 = face_sign(4, xO, 0, 0), s5 = face_sign(5, xO, 0, 0);
         const uint32_t bx = __umul24(zM, sXz) + __umul24(yM, sXy) + xO + P.nZF + P.nYF;
@@ -554,18 +556,24 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
         const uint32_t iy2 = iy1 + sYy, iy3 = iy1 + __umul24(dz, sYz), iy4 = iy3 + sYy;
         const FluxPair y1 = pair(iy1), y2 = pair(iy2), y3 = pair(iy3), y4 = pair(iy4);
         const float vel1y = -s2 * y1.x, vel2y = s3 * y2.x;
-        const float vel3y = -s2 * second(y1, iy1), vel4y = s3 * second(y2, iy2);
+        float vel3y = -s2 * second(y1, iy1), vel4y = s3 * second(y2, iy2);
         const float vel5y = -s2 * y3.x, vel6y = s3 * y4.x;
-        const float vel7y = -s2 * second(y3, iy3), vel8y = s3 * second(y4, iy4);
+        float vel7y = -s2 * second(y3, iy3), vel8y = s3 * second(y4, iy4);
         // z component: voxels (xM|xP, yM|yP, zO), faces f1 (front) / f2 (back)
         const float s0 = face_sign(0, 0, 0, zO), s1 = face_sign(1, 0, 0, zO);
         const uint32_t iz1 = __umul24(zO, sZz) + __umul24(yM, sZy) + xM;
         const uint32_t iz2 = iz1 + __umul24(dy, sZy), iz3 = iz1 + sZz, iz4 = iz2 + sZz;
         const FluxPair z1 = pair(iz1), z2 = pair(iz2), z3 = pair(iz3), z4 = pair(iz4);
         const float vel1z = s0 * z1.x, vel2z = s0 * z2.x;
-        const float vel3z = s0 * second(z1, iz1), vel4z = s0 * second(z2, iz2);
+        float vel3z = s0 * second(z1, iz1), vel4z = s0 * second(z2, iz2);
         const float vel5z = -s1 * z3.x, vel6z = -s1 * z4.x;
-        const float vel7z = -s1 * second(z3, iz3), vel8z = -s1 * second(z4, iz4);
+        float vel7z = -s1 * second(z3, iz3), vel8z = -s1 * second(z4, iz4);
+        if (__builtin_expect(dx != 0 && dx != 1, 0)) {
+            vel3y = -s2 * u[iy1 + dx]; vel4y = s3 * u[iy2 + dx];
+            vel7y = -s2 * u[iy3 + dx]; vel8y = s3 * u[iy4 + dx];
+            vel3z = s0 * u[iz1 + dx]; vel4z = s0 * u[iz2 + dx];
+            vel7z = -s1 * u[iz3 + dx]; vel8z = -s1 * u[iz4 + dx];
+        }
 
         const float hf = P.hf;
         float cx0 = (P.aabb_min[0] + hf * static_cast<float>(xO)) + 0.5f * 0.0f;
