@@ -673,14 +673,14 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
     if (D.counts) density_add(D, p.x, p.y, p.z);
 }
 
-// 8 waves per SIMD (64 VGPRs, a few spilled outside the frame loop): alone
-// it is slightly slower than at 70 VGPRs, but the windowed frame loop, where
-// the next window's reconstruction and steps share the CUs, runs 5% faster.
 // Several frames' particle passes in one launch: each thread carries its
 // particle through frames t = 0..nframes-1 with u_t = u0 + t * ldU and the
 // respawn frame P.frame + t, and stores it once (the same trace_one per
 // frame, so the bits of nframes single passes).  With D.counts set the
 // positions after the last frame are counted.
+// 8 waves per SIMD (64 VGPRs, a few spilled outside the frame loop): alone
+// it is slightly slower than at 70 VGPRs, but the windowed frame loop, where
+// the next window's reconstruction and steps share the CUs, runs 5% faster.
 __global__ __launch_bounds__(256, 8) void k_trace_frames(TraceParams P, const float* __restrict__ u0, size_t ldU,
                                                       uint32_t nframes, float4* __restrict__ parts,
                                                       const uint32_t* __restrict__ pid, uint32_t n, DensityGrid D,
