@@ -678,10 +678,10 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
 // respawn frame P.frame + t, and stores it once (the same trace_one per
 // frame, so the bits of nframes single passes).  With D.counts set the
 // positions after the last frame are counted.
-// 8 waves per SIMD (64 VGPRs; three loop-invariant doubles are spilled to
-// scratch and reloaded each frame): alone it is slightly slower than at 70
-// VGPRs, but the windowed frame loop, where the next window's reconstruction
-// and steps share the CUs, runs 5% faster.
+// 8 waves per SIMD (64 VGPRs; three 64-bit values are spilled to scratch and
+// reloaded only in the respawn branch and before the final store): alone it
+// is slightly slower than at 70 VGPRs, but the windowed frame loop, where
+// the next window's reconstruction and steps share the CUs, runs 5% faster.
 __global__ __launch_bounds__(256, 8) void k_trace_frames(TraceParams P, const float* __restrict__ u0, size_t ldU,
                                                       uint32_t nframes, float4* __restrict__ parts,
                                                       const uint32_t* __restrict__ pid, uint32_t n, DensityGrid D,
