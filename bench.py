@@ -147,6 +147,10 @@ def cpu_baseline(sc, budget_s=12.0):
 
 SERVE_GRIDS = (16, 24, 32, 48)
 SERVE_K0 = 192
+# modes per workgroup of the persistent layout, each mode at its best of
+# {default, 2, 4, 8} measured on the 512-request stream (0 = per-request default)
+SERVE_MPW_BATCHED = 0
+SERVE_MPW_ALONE = 2
 
 
 def serving_workload(n_requests, rate, seed=0, particles=1 << 14):
@@ -265,15 +269,17 @@ def serving_bench(ctx, n_requests, rate):
     bases = serving_bases()
     reqs = serving_workload(n_requests, rate)
     log(f"serving: {len(bases)} bases (K0={SERVE_K0}) and {n_requests} requests in {time.time() - t:.1f}s")
-    # batched: four modes per workgroup packs more scenes into each persistent
-    # launch; one at a time: the per-request default (the fastest steps)
-    batched = run_stream(ctx, bases, reqs, max_active=64, modes_per_wg=4)
+    # Each mode at its best measured modes-per-workgroup setting: batched, the
+    # per-request default (the most modes whose C blocks fit in registers:
+    # fewer workgroups per scene, more scenes per persistent launch); one at a
+    # time, two (more workgroups share a lone scene's contraction).
+    batched = run_stream(ctx, bases, reqs, max_active=64, modes_per_wg=SERVE_MPW_BATCHED)
     log(f"serving batched: {batched['scene_frames_per_s']:.0f} scene-frames/s, "
         f"p50 {batched['latency_ms_p50']:.1f} ms, p99 {batched['latency_ms_p99']:.1f} ms")
-    alone = run_stream(ctx, bases, reqs, max_active=1)
+    alone = run_stream(ctx, bases, reqs, max_active=1, modes_per_wg=SERVE_MPW_ALONE)
     log(f"serving one at a time: {alone['scene_frames_per_s']:.0f} scene-frames/s, "
         f"p50 {alone['latency_ms_p50']:.1f} ms, p99 {alone['latency_ms_p99']:.1f} ms")
-    sat = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=4)
+    sat = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED)
     log(f"serving batched, all queued at once: {sat['scene_frames_per_s']:.0f} scene-frames/s")
     nd = min(16, n_requests)
     dropin = dropin_sequential(ctx, bases, reqs[:nd])
@@ -282,6 +288,7 @@ def serving_bench(ctx, n_requests, rate):
     return {
         "workload": f"{n_requests} requests, Poisson arrivals at {rate:g}/s, grids {SERVE_GRIDS}^3, K in [16, 192], "
                     f"frames in [100, 2000], 2^14 particles each; one K0={SERVE_K0} basis per grid",
+        "modes_per_wg": {"batched": SERVE_MPW_BATCHED or "per-request default", "one_at_a_time": SERVE_MPW_ALONE},
         "batched": batched,
         "one_at_a_time": alone,
         "batched_all_queued": sat,
@@ -557,7 +564,7 @@ def main():
         ctx = hip.Context(0)
         bases = serving_bases()
         reqs = serving_workload(args.serving_profile, args.serving_rate)
-        r = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=4)
+        r = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED)
         ctx.close()
         print(json.dumps({"serving_profile": args.serving_profile, **r}), flush=True)
         return
