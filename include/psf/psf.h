@@ -76,7 +76,8 @@ typedef struct psf_scene psf_scene;
 typedef struct psf_server psf_server;
 
 /* Grid and basis description.  Counts follow DECMesh3D (DEC/decmesh3d.cpp:64-70):
-   F = (nz+1)nx ny + (ny+1)nx nz + (nx+1)ny nz faces, V = nx ny nz voxels. */
+   F = (nz+1)nx ny + (ny+1)nx nz + (nx+1)ny nz faces, V = nx ny nz voxels.
+   Limits: 1 <= nx, ny, nz <= 2048 and F < 2^30. */
 typedef struct psf_scene_desc {
     uint32_t nx, ny, nz;       /* voxel counts (DECMesh3D::getDimensions) */
     double resolution;         /* AbstractSolver::resolution, the voxel size h */

@@ -163,7 +163,8 @@ int validate_desc(psf_ctx* ctx, const psf_scene_desc* d)
         return set_err(ctx, PSF_ERR_ARG, "grid dims out of range");
     const double nf = (double)(d->nx + 1) * d->ny * d->nz + (double)(d->ny + 1) * d->nx * d->nz +
                       (double)(d->nz + 1) * d->nx * d->ny;
-    if (nf > 2.0e9) return set_err(ctx, PSF_ERR_ARG, "too many faces");
+    // the particle pass addresses the face field by 32-bit byte offsets
+    if (nf >= 1073741824.0) return set_err(ctx, PSF_ERR_ARG, "too many faces (at most 2^30 - 1)");
     if (d->K == 0 || d->K > 4096) return set_err(ctx, PSF_ERR_ARG, "K must be in [1, 4096]");
     if (!(d->resolution > 0.0) || !std::isfinite(d->resolution)) return set_err(ctx, PSF_ERR_ARG, "resolution must be > 0");
     if (d->c_precision < PSF_C_FP64 || d->c_precision > PSF_C_BF16X2)
@@ -204,6 +205,8 @@ TraceParams trace_params(const psf_scene* s)
     p.center_z = s->d.aabb_min[2] + 0.5f * (s->d.aabb_max[2] - s->d.aabb_min[2]);
     p.seed = s->seed;
     p.frame = s->frame;
+    int e = 0;
+    p.pow2 = std::frexp(s->d.resolution, &e) == 0.5 && static_cast<double>(p.inv_h) == 1.0 / s->d.resolution;
     return p;
 }
 

@@ -457,7 +457,9 @@ __device__ __forceinline__ float mixf(float x, float y, float a)
     return x * (1.0f - a) + y * a;   // glm::mix
 }
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// v < lo ? lo : (v > hi ? hi : v) for lo <= hi, written as the median of
+// (v, lo, hi) in the form the backend selects as one v_med3_i32
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return max(min(v, lo), min(max(v, lo), hi)); }
 
 // The three cell indices of one axis as the CPU path computes them
 // (psfsolver.cpp:80-100): O = int(a / h), M = int((a - h/2) / h),
@@ -482,6 +484,25 @@ __device__ __forceinline__ void cell_indices(double a, double h, double hh, doub
     }
 }
 
+// The same three indices when h = 2^-k, in float; equal to cell_indices'
+// once clamped to [0, n - 1] (n <= 2048), which every caller does.
+// q = a / h = a * 2^k is exact in float and in double, and the CPU path's
+// (a -+ h/2) / h is q -+ 1/2 (exactly, or where a is too small or too large
+// for that, with the same clamped index).  For 1/2 <= q < 2^23, q - 1/2 is
+// exact in float (a multiple of q's ulp <= 1/2, below q), and q + 1/2 either
+// is exact or rounds within [2^j, 2^j + 1/2] after crossing a power of two
+// 2^j >= 1, so truncation agrees.  Below 1/2 every index clamps to 0 (q + 1/2
+// alone could round up to 1 in float, hence the select); from 2^23 up all
+// three clamp to n - 1; NaN converts to 0 in both.
+__device__ __forceinline__ void cell_indices_pow2(float a, float rb, int& O, int& M, int& P)
+{
+#pragma clang fp contract(off)
+    const float q = a * rb;
+    O = static_cast<int>(q);
+    M = static_cast<int>(q - 0.5f);
+    P = q < 0.5f ? 0 : static_cast<int>(q + 0.5f);
+}
+
 // two consecutive face fluxes u[i], u[i + 1] in one 4-byte-aligned load (i + 1 < F
 // always holds for the faces paired below)
 typedef float FluxPair __attribute__((ext_vector_type(2))) __attribute__((aligned(4)));
@@ -499,7 +520,9 @@ __device__ __forceinline__ uint32_t respawn_draw(uint32_t seed, uint32_t step, u
 }
 
 // one particle record through one frame; id is its storage slot (cells,
-// caller index)
+// caller index).  Pow2: the voxel size is a power of two (cell_indices_pow2).
+// u is addressed by 32-bit byte offsets from a uniform base (4 F < 2^32).
+template <bool Pow2>
 __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<const float> u, float4 p,
                                             const gptr<const uint32_t> pid, uint32_t id, int32_t* __restrict__ cells)
 {
@@ -509,11 +532,17 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
         const float sx = (P.aabb_min[0] + P.ext[0]) + p.x;
         const float sy = (P.aabb_min[1] + P.ext[1]) + p.y;
         const float sz = (P.aabb_min[2] + P.ext[2]) + p.z;
-        const double res = P.res, hres = P.res / 2, rr = P.inv_res;
         int xO, xM, xP, yO, yM, yP, zO, zM, zP;
-        cell_indices(static_cast<double>(sx), res, hres, rr, xO, xM, xP);
-        cell_indices(static_cast<double>(sy), res, hres, rr, yO, yM, yP);
-        cell_indices(static_cast<double>(sz), res, hres, rr, zO, zM, zP);
+        if constexpr (Pow2) {
+            cell_indices_pow2(sx, P.inv_h, xO, xM, xP);
+            cell_indices_pow2(sy, P.inv_h, yO, yM, yP);
+            cell_indices_pow2(sz, P.inv_h, zO, zM, zP);
+        } else {
+            const double res = P.res, hres = P.res / 2, rr = P.inv_res;
+            cell_indices(static_cast<double>(sx), res, hres, rr, xO, xM, xP);
+            cell_indices(static_cast<double>(sy), res, hres, rr, yO, yM, yP);
+            cell_indices(static_cast<double>(sz), res, hres, rr, zO, zM, zP);
+        }
         xO = clampi(xO, 0, nx - 1); yO = clampi(yO, 0, ny - 1); zO = clampi(zO, 0, nz - 1);
         xM = clampi(xM, 0, nx - 1); xP = clampi(xP, 0, nx - 1);
         yM = clampi(yM, 0, ny - 1); yP = clampi(yP, 0, ny - 1);
@@ -523,11 +552,11 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
             c[0] = xO; c[1] = yO; c[2] = zO; c[3] = xM; c[4] = xP; c[5] = yM; c[6] = yP; c[7] = zM; c[8] = zP;
             c[9] = 0; c[10] = 0; c[11] = 0;
         }
-        // face indices (decmesh3d.cpp:17-30): Z faces, then Y, then X
         // Face indices (decmesh3d.cpp:17-30): Z faces, then Y, then X, each
-        // block z-major.  Strides stay below 2^24 and indices below 2^31, so
-        // the products run on the 24-bit multiplier; neighbouring faces are
-        // reached by adding strides (the M/P pairs differ by 0, 1 or 2 cells).
+        // block z-major.  Strides stay below 2^24 and indices below 2^30, so
+        // the products run on the 24-bit multiplier and byte offsets fit 32
+        // bits; neighbouring faces are reached by adding strides (the M/P
+        // pairs differ by 0, 1 or 2 cells).
         const uint32_t sZy = P.nx, sZz = P.nx * P.ny;
         const uint32_t sYy = P.nx, sYz = (P.ny + 1) * P.nx;
         const uint32_t sXy = P.nx + 1, sXz = P.ny * (P.nx + 1);
@@ -535,74 +564,85 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
         // Faces of neighbouring x are adjacent in every block of the layout, so
         // each (x, x+1) couple comes in one load: the x faces of voxel xO, and
         // the y and z faces of columns xM and xP (xP = xM + 1 but at clamps).
-        auto pair = [&](uint32_t i) { return *reinterpret_cast<gptr<const FluxPair>>(u + i); };
+        const gptr<const char> ub = reinterpret_cast<gptr<const char>>(u);
+        auto pair = [&](uint32_t b) { return *reinterpret_cast<gptr<const FluxPair>>(ub + b); };
+        auto one = [&](uint32_t b) { return *reinterpret_cast<gptr<const float>>(ub + b); };
         const int dx = xP - xM;
         // the x+ neighbour from the pair: dx is 1 (or 0 at a clamp); a dx of
         // any other value is patched once, below, with its own loads
         auto second = [&](const FluxPair& q, uint32_t) { return dx == 1 ? q.y : q.x; };
         // x component: voxels (xO, yM|yP, zM|zP), faces f5 (left) / f6 (right)
         const float s4 = face_sign(4, xO, 0, 0), s5 = face_sign(5, xO, 0, 0);
-        const uint32_t bx = __umul24(zM, sXz) + __umul24(yM, sXy) + xO + P.nZF + P.nYF;
-        const uint32_t dyx = __umul24(dy, sXy), dzx = __umul24(dz, sXz);
+        const uint32_t bx = (__umul24(zM, sXz) + __umul24(yM, sXy) + xO + P.nZF + P.nYF) << 2;
+        const uint32_t dyx = __umul24(dy, sXy) << 2, dzx = __umul24(dz, sXz) << 2;
         const FluxPair x1 = pair(bx), x2 = pair(bx + dyx);
         const FluxPair x3 = pair(bx + dzx), x4 = pair(bx + dyx + dzx);
-        const float vel1x = -s4 * x1.x, vel3x = s5 * x1.y;
-        const float vel2x = -s4 * x2.x, vel4x = s5 * x2.y;
-        const float vel5x = -s4 * x3.x, vel7x = s5 * x3.y;
-        const float vel6x = -s4 * x4.x, vel8x = s5 * x4.y;
         // y component: voxels (xM|xP, yO, zM|zP), faces f3 (bottom) / f4 (top)
         const float s2 = face_sign(2, 0, yO, 0), s3 = face_sign(3, 0, yO, 0);
-        const uint32_t iy1 = __umul24(zM, sYz) + __umul24(yO, sYy) + xM + P.nZF;
-        const uint32_t iy2 = iy1 + sYy, iy3 = iy1 + __umul24(dz, sYz), iy4 = iy3 + sYy;
+        const uint32_t iy1 = (__umul24(zM, sYz) + __umul24(yO, sYy) + xM + P.nZF) << 2;
+        const uint32_t iy2 = iy1 + (sYy << 2), iy3 = iy1 + (__umul24(dz, sYz) << 2), iy4 = iy3 + (sYy << 2);
         const FluxPair y1 = pair(iy1), y2 = pair(iy2), y3 = pair(iy3), y4 = pair(iy4);
-        const float vel1y = -s2 * y1.x, vel2y = s3 * y2.x;
-        float vel3y = -s2 * second(y1, iy1), vel4y = s3 * second(y2, iy2);
-        const float vel5y = -s2 * y3.x, vel6y = s3 * y4.x;
-        float vel7y = -s2 * second(y3, iy3), vel8y = s3 * second(y4, iy4);
+        float r3y = second(y1, iy1), r4y = second(y2, iy2);
+        float r7y = second(y3, iy3), r8y = second(y4, iy4);
         // z component: voxels (xM|xP, yM|yP, zO), faces f1 (front) / f2 (back)
         const float s0 = face_sign(0, 0, 0, zO), s1 = face_sign(1, 0, 0, zO);
-        const uint32_t iz1 = __umul24(zO, sZz) + __umul24(yM, sZy) + xM;
-        const uint32_t iz2 = iz1 + __umul24(dy, sZy), iz3 = iz1 + sZz, iz4 = iz2 + sZz;
+        const uint32_t iz1 = (__umul24(zO, sZz) + __umul24(yM, sZy) + xM) << 2;
+        const uint32_t iz2 = iz1 + (__umul24(dy, sZy) << 2), iz3 = iz1 + (sZz << 2), iz4 = iz2 + (sZz << 2);
         const FluxPair z1 = pair(iz1), z2 = pair(iz2), z3 = pair(iz3), z4 = pair(iz4);
-        const float vel1z = s0 * z1.x, vel2z = s0 * z2.x;
-        float vel3z = s0 * second(z1, iz1), vel4z = s0 * second(z2, iz2);
-        const float vel5z = -s1 * z3.x, vel6z = -s1 * z4.x;
-        float vel7z = -s1 * second(z3, iz3), vel8z = -s1 * second(z4, iz4);
+        float r3z = second(z1, iz1), r4z = second(z2, iz2);
+        float r7z = second(z3, iz3), r8z = second(z4, iz4);
         if (__builtin_expect(dx != 0 && dx != 1, 0)) {
-            vel3y = -s2 * u[iy1 + dx]; vel4y = s3 * u[iy2 + dx];
-            vel7y = -s2 * u[iy3 + dx]; vel8y = s3 * u[iy4 + dx];
-            vel3z = s0 * u[iz1 + dx]; vel4z = s0 * u[iz2 + dx];
-            vel7z = -s1 * u[iz3 + dx]; vel8z = -s1 * u[iz4 + dx];
+            const uint32_t db = static_cast<uint32_t>(dx) << 2;
+            r3y = one(iy1 + db); r4y = one(iy2 + db);
+            r7y = one(iy3 + db); r8y = one(iy4 + db);
+            r3z = one(iz1 + db); r4z = one(iz2 + db);
+            r7z = one(iz3 + db); r8z = one(iz4 + db);
         }
 
+        // Face centres: min + h * i, plus h/2 on the axes the face spans.  On
+        // the face's own axis the CPU adds 0.5f * 0.0f; min + h * i is never
+        // -0 (h > 0, i >= 0, and -0 + +0 = +0) and finite for a valid AABB,
+        // so adding +0 leaves it unchanged and is skipped.
         const float hf = P.hf;
-        float cx0 = (P.aabb_min[0] + hf * static_cast<float>(xO)) + 0.5f * 0.0f;
+        float cx0 = P.aabb_min[0] + hf * static_cast<float>(xO);
         float cx1 = (P.aabb_min[1] + hf * static_cast<float>(yM)) + 0.5f * hf;
         float cx2 = (P.aabb_min[2] + hf * static_cast<float>(zM)) + 0.5f * hf;
         float cy0 = (P.aabb_min[0] + hf * static_cast<float>(xM)) + 0.5f * hf;
-        float cy1 = (P.aabb_min[1] + hf * static_cast<float>(yO)) + 0.5f * 0.0f;
+        float cy1 = P.aabb_min[1] + hf * static_cast<float>(yO);
         float cy2 = (P.aabb_min[2] + hf * static_cast<float>(zM)) + 0.5f * hf;
         float cz0 = (P.aabb_min[0] + hf * static_cast<float>(xM)) + 0.5f * hf;
         float cz1 = (P.aabb_min[1] + hf * static_cast<float>(yM)) + 0.5f * hf;
-        float cz2 = (P.aabb_min[2] + hf * static_cast<float>(zO)) + 0.5f * 0.0f;
+        float cz2 = P.aabb_min[2] + hf * static_cast<float>(zO);
         const float inv = P.inv_h;
         const float nx0 = inv * (p.x - cx0), nx1 = inv * (p.y - cx1), nx2 = inv * (p.z - cx2);
         const float ny0 = inv * (p.x - cy0), ny1 = inv * (p.y - cy1), ny2 = inv * (p.z - cy2);
         const float nz0 = inv * (p.x - cz0), nz1 = inv * (p.y - cz1), nz2 = inv * (p.z - cz2);
+        // The CPU path mixes the signed face velocities vel_k = s * u_k
+        // (psfsolver.cpp:160-206): first pairwise along one axis, then the
+        // pairs, then the halves.  In the first level glm::mix(s a, s' b, t)
+        // = (s a)(1 - t) + (s' b) t, and (s a) w = a (s w) exactly for s = +-1
+        // (round to nearest commutes with negation), so the signs ride on the
+        // four weights instead of the eight fluxes.
         float vx, vy, vz;
         {
-            const float a0 = mixf(vel1x, vel2x, nx1), a1 = mixf(vel3x, vel4x, nx1);
-            const float a2 = mixf(vel5x, vel6x, nx1), a3 = mixf(vel7x, vel8x, nx1);
+            // vel1,2,5,6 = -s4 * (left faces), vel3,4,7,8 = s5 * (right faces)
+            const float w = 1.0f - nx1, la = -s4 * w, lb = -s4 * nx1, ra = s5 * w, rb = s5 * nx1;
+            const float a0 = x1.x * la + x2.x * lb, a1 = x1.y * ra + x2.y * rb;
+            const float a2 = x3.x * la + x4.x * lb, a3 = x3.y * ra + x4.y * rb;
             vx = mixf(mixf(a0, a1, nx0), mixf(a2, a3, nx0), nx2);
         }
         {
-            const float a0 = mixf(vel1y, vel2y, ny1), a1 = mixf(vel3y, vel4y, ny1);
-            const float a2 = mixf(vel5y, vel6y, ny1), a3 = mixf(vel7y, vel8y, ny1);
+            // odd vel = -s2 * (bottom faces), even vel = s3 * (top faces)
+            const float wa = -s2 * (1.0f - ny1), wb = s3 * ny1;
+            const float a0 = y1.x * wa + y2.x * wb, a1 = r3y * wa + r4y * wb;
+            const float a2 = y3.x * wa + y4.x * wb, a3 = r7y * wa + r8y * wb;
             vy = mixf(mixf(a0, a1, ny0), mixf(a2, a3, ny0), ny2);
         }
         {
-            const float a0 = mixf(vel1z, vel2z, nz1), a1 = mixf(vel3z, vel4z, nz1);
-            const float a2 = mixf(vel5z, vel6z, nz1), a3 = mixf(vel7z, vel8z, nz1);
+            // vel1..4 = s0 * (front faces), vel5..8 = -s1 * (back faces)
+            const float w = 1.0f - nz1, fa = s0 * w, fb = s0 * nz1, ba = -s1 * w, bb = -s1 * nz1;
+            const float a0 = z1.x * fa + z2.x * fb, a1 = r3z * fa + r4z * fb;
+            const float a2 = z3.x * ba + z4.x * bb, a3 = r7z * ba + r8z * bb;
             vz = mixf(mixf(a0, a1, nz0), mixf(a2, a3, nz0), nz2);
         }
         // glm::clamp(v, lo, hi) = min(max(v, lo), hi)
@@ -658,6 +698,7 @@ __device__ __forceinline__ void density_add(const DensityGrid& D, float x, float
     density_add_key(D.counts, density_cell(D, x, y, z));
 }
 
+template <bool Pow2>
 __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __restrict__ u, float4* __restrict__ parts,
                                                const uint32_t* __restrict__ pid, uint32_t n, int32_t* __restrict__ cells,
                                                DensityGrid D, uint32_t per)
@@ -667,7 +708,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
     // particle records stream through once: nontemporal, so u stays in L2
     f32x4* rec = reinterpret_cast<f32x4*>(parts) + id;
     const f32x4 v = __builtin_nontemporal_load(rec);
-    const float4 p = trace_one(P, as_global(u), make_float4(v.x, v.y, v.z, v.w), as_global(pid), id, cells);
+    const float4 p = trace_one<Pow2>(P, as_global(u), make_float4(v.x, v.y, v.z, v.w), as_global(pid), id, cells);
     const f32x4 o = {p.x, p.y, p.z, p.w};
     __builtin_nontemporal_store(o, rec);
     if (D.counts) density_add(D, p.x, p.y, p.z);
@@ -682,6 +723,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
 // reloaded only in the respawn branch and before the final store): alone it
 // is slightly slower than at 70 VGPRs, but the windowed frame loop, where
 // the next window's reconstruction and steps share the CUs, runs 5% faster.
+template <bool Pow2>
 __global__ __launch_bounds__(256, 8) void k_trace_frames(TraceParams P, const float* __restrict__ u0, size_t ldU,
                                                       uint32_t nframes, float4* __restrict__ parts,
                                                       const uint32_t* __restrict__ pid, uint32_t n, DensityGrid D,
@@ -695,7 +737,7 @@ __global__ __launch_bounds__(256, 8) void k_trace_frames(TraceParams P, const fl
     const gptr<const float> u = as_global(u0);
     const gptr<const uint32_t> gp = as_global(pid);
     for (uint32_t t = 0; t < nframes; ++t) {
-        p = trace_one(P, u + static_cast<size_t>(t) * ldU, p, gp, id, nullptr);
+        p = trace_one<Pow2>(P, u + static_cast<size_t>(t) * ldU, p, gp, id, nullptr);
         P.frame++;
     }
     const f32x4 o = {p.x, p.y, p.z, p.w};
@@ -827,9 +869,16 @@ __global__ __launch_bounds__(256) void k_trace_window_multi(const TraceScene* __
     const gptr<const uint32_t> pid = as_global(ts.pid);
     const f32x4 v = __builtin_nontemporal_load(&parts[id]);
     float4 p = make_float4(v.x, v.y, v.z, v.w);
-    for (uint32_t t = 0; t < ts.nframes; ++t) {
-        p = trace_one(P, uh + static_cast<size_t>(t) * ts.ldU, p, pid, id, nullptr);
-        P.frame++;
+    if (P.pow2) {
+        for (uint32_t t = 0; t < ts.nframes; ++t) {
+            p = trace_one<true>(P, uh + static_cast<size_t>(t) * ts.ldU, p, pid, id, nullptr);
+            P.frame++;
+        }
+    } else {
+        for (uint32_t t = 0; t < ts.nframes; ++t) {
+            p = trace_one<false>(P, uh + static_cast<size_t>(t) * ts.ldU, p, pid, id, nullptr);
+            P.frame++;
+        }
     }
     const f32x4 o = {p.x, p.y, p.z, p.w};
     __builtin_nontemporal_store(o, &parts[id]);
@@ -938,7 +987,10 @@ hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, con
 {
     if (n == 0) return hipSuccess;
     const uint32_t per = xcd_per((n + 255) / 256);
-    k_trace<<<per * kXcds, 256, 0, s>>>(p, u, parts, pid, n, cells, D, per);
+    if (p.pow2)
+        k_trace<true><<<per * kXcds, 256, 0, s>>>(p, u, parts, pid, n, cells, D, per);
+    else
+        k_trace<false><<<per * kXcds, 256, 0, s>>>(p, u, parts, pid, n, cells, D, per);
     return hipGetLastError();
 }
 
@@ -947,7 +999,10 @@ hipError_t launch_trace_frames(const TraceParams& p, const float* u0, size_t ldU
 {
     if (n == 0 || nframes == 0) return hipSuccess;
     const uint32_t per = xcd_per((n + 255) / 256);
-    k_trace_frames<<<per * kXcds, 256, 0, s>>>(p, u0, ldU, nframes, parts, pid, n, D, per);
+    if (p.pow2)
+        k_trace_frames<true><<<per * kXcds, 256, 0, s>>>(p, u0, ldU, nframes, parts, pid, n, D, per);
+    else
+        k_trace_frames<false><<<per * kXcds, 256, 0, s>>>(p, u0, ldU, nframes, parts, pid, n, D, per);
     return hipGetLastError();
 }
 

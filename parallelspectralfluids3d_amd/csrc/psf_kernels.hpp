@@ -17,6 +17,7 @@ struct TraceParams {
     float lifetime;
     float center_z;             // AABB::getCenter().z (float)
     uint32_t seed, frame;
+    uint32_t pow2;              // resolution is 2^-k (so inv_h = 2^k exactly)
 };
 
 // Particle density grid: counts of particles per cell of an n[0] x n[1] x n[2]
