@@ -800,9 +800,12 @@ int psf_get_omega(psf_scene* s, double* omega)
     return check_status(s);
 }
 
-// Particles are stored sorted by the voxel that contains them (stable in the
-// caller's order), so the lanes of a wave gather neighbouring face fluxes;
-// every interface below speaks the caller's order.
+// Particles are stored sorted by the cell of the half-voxel-shifted grid
+// that contains them (stable in the caller's order; z-major like the voxels).
+// A particle's interpolation reads faces of the voxels around that cell, so
+// for the x component the lanes of a wave share the y and z voxel rows (and
+// likewise for y and z up to one row), and each gather touches few cache
+// lines.  Every interface below speaks the caller's order.
 int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
 {
     if (!s || (!xyzw && n)) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
@@ -819,14 +822,14 @@ int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
         const uint32_t dims[3] = {s->d.nx, s->d.ny, s->d.nz};
         for (int a = 0; a < 3; ++a) {
             const float sa = (s->d.aabb_min[a] + (s->d.aabb_max[a] - s->d.aabb_min[a])) + xyzw[4 * size_t(i) + a];
-            const double q = static_cast<double>(sa) / res;
+            const double q = static_cast<double>(sa) / res - 0.5;
             long v = std::isfinite(q) ? static_cast<long>(std::max(-1.0, std::min(q, 1.0e9))) : 0;
             c[a] = static_cast<uint32_t>(std::max(0L, std::min(v, static_cast<long>(dims[a]) - 1)));
         }
         key[i] = (c[2] * s->d.ny + c[1]) * s->d.nx + c[0];
     }
     if (size_t(s->V) <= std::max<size_t>(size_t(4) * n, size_t(1) << 16)) {
-        // stable counting sort by voxel
+        // stable counting sort by cell
         std::vector<uint32_t> start(size_t(s->V) + 1, 0u);
         for (uint32_t i = 0; i < n; ++i) start[key[i] + 1]++;
         for (size_t v = 0; v < s->V; ++v) start[v + 1] += start[v];
