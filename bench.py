@@ -387,7 +387,8 @@ def large_bench(args):
 def config2_bench(args):
     """The plain drop-in form at full size: 32^3 box with a voxelised central
     sphere (cells whose centre lies within radius 0.25 are solid), the K=128
-    lowest eigenpairs of the pruned DEC 1-form Laplacian (host eigensolve),
+    lowest eigenpairs of the pruned DEC 1-form Laplacian (device eigensolve,
+    or scipy's shift-invert with --config2-host-eig),
     an upward force on modes 0-7, FP32 C on the plain path and FP32 Phi, one
     integrate() per frame (no frame window), 2000 frames at dt=0.005, 2^18
     particles in fluid cells.  Checks: omega after all frames against the
@@ -403,6 +404,8 @@ def config2_bench(args):
     inside = sphere_obstacle_mask(g, h, amin)
     cache = args.config2_cache
     t_eig = t_c = None
+    eig_info = None
+    ctx = None if (args.config2_build_only and args.config2_host_eig) else hip.Context(0)
     if cache and os.path.exists(cache):
         sc = hip.read_cache(cache)
         assert (sc["desc"].nx, sc["desc"].K) == (n, K)
@@ -410,9 +413,12 @@ def config2_bench(args):
         log(f"config2: basis, C and forcing read from the scene cache {cache}")
     else:
         t = time.time()
-        Phi, lam = basis.laplacian_eigenbasis(g, h, inside, K)
+        if args.config2_host_eig:
+            Phi, lam = basis.laplacian_eigenbasis(g, h, inside, K)
+        else:
+            Phi, lam, eig_info = basis.laplacian_eigenbasis_device(g, h, inside, K, ctx)
         t_eig = time.time() - t
-        log(f"config2: eigenbasis K={K} in {t_eig:.1f}s")
+        log(f"config2: eigenbasis K={K} in {t_eig:.1f}s ({'host' if args.config2_host_eig else 'device'})")
         t = time.time()
         V = (basis.curl_operator(g, h) @ Phi.T).T
         C = advection.build_advection(g, h, inside, Phi, lam, V)
@@ -425,6 +431,8 @@ def config2_bench(args):
             hip.write_cache(cache, g, h, lam, C, Phi_cm=Phi, gravity=force, aabb_min=amin, aabb_max=amax)
             log(f"config2: scene cache written to {cache}")
         if args.config2_build_only:
+            if ctx is not None:
+                ctx.close()
             return
     omega = basis.unit_energy_omega(K, 0)
     rng = np.random.default_rng(21)
@@ -435,7 +443,6 @@ def config2_bench(args):
     parts = np.zeros((N, 4), np.float32)
     parts[:, :3] = np.asarray(amin) + (cell + rng.random((N, 3))) * h
     parts[:, 3] = 60.0
-    ctx = hip.Context(0)
     s = hip.Scene(ctx, hip.make_desc(g, h, K, L.C_FP32, L.PHI_FP32, N, amin, amax))
     s.upload_basis(Phi, lam, force)
     s.upload_advection(C)
@@ -467,7 +474,8 @@ def config2_bench(args):
         "workload": f"{n}^3 box, central sphere obstacle r=0.25, K={K} eigenbasis of the pruned M, upward force on "
                     f"modes 0-7, FP32 C (plain path), FP32 Phi, {frames} frames at dt={dt}, one integrate() per "
                     f"frame, {N} particles in fluid cells",
-        "host_eigensolve_s": t_eig, "host_C_build_s": t_c,
+        "eigensolve_s": t_eig, "eigensolve": ("host shift-invert" if args.config2_host_eig else eig_info),
+        "host_C_build_s": t_c,
         "ms_per_frame": ms / frames, "particle_steps_per_s": N * frames / (ms * 1e-3),
         "oracle_ms_per_frame": 1e3 * t_ora / frames, "oracle_threads": nthreads,
         "omega_rel_err_vs_fp64_oracle": float(np.linalg.norm(w_gpu - w) / np.linalg.norm(w)),
@@ -522,7 +530,9 @@ def main():
                     help="the 32^3 sphere-obstacle K=128 drop-in configuration at full size, against the oracle")
     ap.add_argument("--config2-cache", default=None,
                     help="scene cache file of the configuration: read if present, else built and written")
-    ap.add_argument("--config2-build-only", action="store_true", help="build and write the cache, no GPU work")
+    ap.add_argument("--config2-build-only", action="store_true", help="build the basis, C and forcing (and write the cache), then stop; with --config2-host-eig no GPU work")
+    ap.add_argument("--config2-host-eig", action="store_true",
+                    help="config 2: scipy shift-invert on the host instead of the device eigensolver")
     ap.add_argument("--serving-only", action="store_true", help="only the serving section")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rate", type=float, default=200.0)

@@ -211,6 +211,26 @@ PSF_API int psf_cache_read_header(const char* path, psf_scene_desc* desc, uint32
 PSF_API int psf_cache_read(const char* path, double* Phi, double* lambda, double* gravity, double* C);
 PSF_API int psf_scene_load_cache(psf_scene* scene, const char* path);
 
+/* ---- eigensolve for the basis ---------------------------------------------------
+ * Replaces the shift-invert eigensolve of the pruned Laplacian in
+ * PSFSolver::buildLaplace (psfsolver.cpp:267-341, Spectra SymEigsShiftSolver
+ * with one SparseLU per shift, eigenvalues above 1e-10 accepted).
+ * The K smallest eigenpairs of a symmetric n x n A in CSR (row_ptr[n+1],
+ * col_idx[nnz], values[nnz], host arrays), by Chebyshev-filtered subspace
+ * iteration on the context's GPU: a block of K + guard Gaussian vectors,
+ * filters of the given polynomial degree, Rayleigh-Ritz until every one of
+ * the K residuals ||A x - lambda x|| is at most tol * (Gershgorin bound of
+ * A's largest eigenvalue).  A null space is returned like any eigenspace;
+ * lift an unwanted one with a penalty term first.
+ * Out: lambda[K] ascending, X (n x K, column-major, orthonormal columns; the
+ * sign and the basis inside a repeated eigenvalue are arbitrary), and the
+ * number of products with A used.  Limits: K <= 256, K + guard <= 1024 and
+ * < n.  PSF_ERR_STATE if max_products is reached first.  Synchronous. */
+PSF_API int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                               const double* values, uint32_t K, uint32_t guard, uint32_t degree, double tol,
+                               uint32_t max_products, uint64_t seed, double* lambda, double* X,
+                               uint32_t* products);
+
 /* ---- particle density volume --------------------------------------------------
  * The histogram -> volume step of the GPU solver's particle drawing
  * (Solver/SolverImpl/psfsolvergpu.cpp:876-913, Resources/Volume/volume.comp:1-17;

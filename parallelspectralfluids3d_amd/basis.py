@@ -221,17 +221,9 @@ def laplacian_eigenbasis(g: Grid, resolution: float, inside, K: int, penalty=Non
     import scipy.sparse as sp
     import scipy.sparse.linalg as sla
     inside = np.asarray(inside, dtype=bool).reshape(-1)
-    M = laplacian(g, resolution, inside)
-    cnt = g.face_inside_count(inside)
-    act = np.flatnonzero(cnt == 2)
-    MII = M[act][:, act]
-    MII = ((MII + MII.T) * 0.5).tocsc()
-    vf_all = g.voxel_faces
-    deep = inside & (cnt[vf_all] == 2).all(1)
-    vf = vf_all[deep]
-    vs = g.voxel_face_signs[deep].astype(np.float64)
-    D = sp.csr_matrix((vs.reshape(-1), (np.repeat(np.arange(vf.shape[0]), 6), vf.reshape(-1))),
-                      shape=(vf.shape[0], g.nF))[:, act]
+    MII, act = pruned_laplacian(g, resolution, inside)
+    MII = MII.tocsc()
+    D = deep_divergence(g, inside, act)
     if penalty is None:
         penalty = 4.0 * max(1.0, float(np.abs(MII).sum(1).max()))
     A = (MII + penalty * (D.T @ D)).tocsc()
@@ -246,12 +238,83 @@ def laplacian_eigenbasis(g: Grid, resolution: float, inside, K: int, penalty=Non
     res = np.linalg.norm(MII @ vecs - vecs * vals, axis=0)
     if np.any(res > 1e-8 * max(1.0, float(vals.max()))):
         raise RuntimeError(f"eigenpair residual {res.max():.3e}: the penalty left a null direction in range")
+    return _basis_from_active(g, act, vecs, vals)
+
+
+def pruned_laplacian(g: Grid, resolution: float, inside):
+    """(M_II, act): M (psfsolver.cpp:247) restricted to the faces between two
+    inside voxels (the rows psfsolver.cpp:251-265 keep), symmetrised, as CSR,
+    and the indices of those faces."""
+    inside = np.asarray(inside, dtype=bool).reshape(-1)
+    M = laplacian(g, resolution, inside)
+    act = np.flatnonzero(g.face_inside_count(inside) == 2)
+    MII = M[act][:, act]
+    return ((MII + MII.T) * 0.5).tocsr(), act
+
+
+def deep_divergence(g: Grid, inside, act):
+    """D: the signed face sums of every inside voxel whose six faces all lie
+    between inside voxels, on the active faces act.  M_II D^T = 0: its rows'
+    transposes (voxel gradients) span M_II's null space apart from harmonic
+    fields, and every eigenvector of nonzero eigenvalue has D v = 0."""
+    import scipy.sparse as sp
+    inside = np.asarray(inside, dtype=bool).reshape(-1)
+    cnt = g.face_inside_count(inside)
+    vf_all = g.voxel_faces
+    deep = inside & (cnt[vf_all] == 2).all(1)
+    vf = vf_all[deep]
+    vs = g.voxel_face_signs[deep].astype(np.float64)
+    return sp.csr_matrix((vs.reshape(-1), (np.repeat(np.arange(vf.shape[0]), 6), vf.reshape(-1))),
+                         shape=(vf.shape[0], g.nF))[:, act]
+
+
+def _basis_from_active(g: Grid, act, vecs, vals):
+    """Phi (K x F) from eigenvectors on the active faces, zero elsewhere, each
+    mode signed so that its largest-magnitude entry is positive."""
+    K = vals.size
     Phi = np.zeros((K, g.nF))
     Phi[:, act] = vecs.T
-    # deterministic sign: the largest-magnitude entry of each mode is positive
     idx = np.argmax(np.abs(Phi), axis=1)
     Phi *= np.sign(Phi[np.arange(K), idx])[:, None]
     return Phi, vals
+
+
+def laplacian_eigenbasis_device(g: Grid, resolution: float, inside, K: int, ctx, tol=1e-11, degree=40,
+                                guard=0, seed=0, penalty=16.0):
+    """laplacian_eigenbasis on the GPU: hip.eigensolve_csr on
+    A = M_II + penalty * D^T D, which lifts M_II's gradient null space and
+    leaves every eigenpair of nonzero eigenvalue (D v = 0) unchanged.  A
+    modest penalty keeps A's spectrum short (the filter's cost grows with its
+    square root).  A returned mode with D v != 0 is a lifted gradient: then the
+    penalty is raised fourfold and the solve repeated.  Same acceptance
+    (eigenvalues above 1e-10, psfsolver.cpp:289-290), residual check against
+    M_II and sign convention as the host path.
+    Returns (Phi, lam, dict(products=..., penalty=..., attempts=...))."""
+    from . import hip
+    MII, act = pruned_laplacian(g, resolution, inside)
+    D = deep_divergence(g, inside, act)
+    DtD = (D.T @ D).tocsr()
+    Kx = min(K + 4, 256)
+    products = 0
+    for attempt in range(4):
+        A = (MII + penalty * DtD).tocsr()
+        vals, vecs, used = hip.eigensolve_csr(ctx, A, Kx, guard=guard, degree=degree, tol=tol, seed=seed)
+        products += used
+        keep = vals > 1e-10
+        vals, vecs = vals[keep], vecs[:, keep]
+        if vals.size < K:
+            raise RuntimeError("eigensolve returned fewer than K accepted eigenpairs")
+        vals, vecs = vals[:K], vecs[:, :K]
+        if np.abs(D @ vecs).max() <= 1e-6:
+            break
+        penalty *= 4.0
+    else:
+        raise RuntimeError("a lifted gradient stayed among the K smallest eigenpairs")
+    res = np.linalg.norm(MII @ vecs - vecs * vals, axis=0)
+    if np.any(res > 1e-8 * max(1.0, float(vals.max()))):
+        raise RuntimeError(f"eigenpair residual {res.max():.3e}")
+    Phi, lam = _basis_from_active(g, act, vecs, vals)
+    return Phi, lam, dict(products=products, penalty=penalty, attempts=attempt + 1)
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,311 @@
+// Device eigensolver for the basis of obstacle scenes (C ABI in psf.h).
+//
+// The reference takes its basis from the K smallest accepted eigenpairs of
+// the pruned DEC 1-form Laplacian (psfsolver.cpp:267-341: Spectra
+// shift-invert, one sparse LU per shift, eigenvalues above 1e-10 kept).
+// Here the K smallest eigenpairs of a symmetric sparse matrix A (the
+// caller's CSR) come from Chebyshev-filtered subspace iteration on a block
+// of m = K + guard vectors, all on the GPU:
+//   start  X = orth(R), R Gaussian;
+//   repeat Rayleigh-Ritz (H = X^T A X, H = V diag(w) V^T, X <- X V), stop
+//          when the K smallest residuals ||A x - w x|| <= tol * lambda_max;
+//          else X <- orth(p(A) X), p the Chebyshev polynomial of the given
+//          degree that is at most 1 on [w_m, lambda_max] and grows below it.
+// A null space is not special: its vectors are the smallest eigenpairs.  The
+// caller lifts one it does not want (the DEC Laplacian's gradients) with a
+// penalty term, as the host shift-invert path does.
+// Products with A: one fused CSR kernel per polynomial degree.  Gram matrices
+// and rotations: rocBLAS DGEMM; orthonormalisation: Cholesky QR twice
+// (rocSOLVER POTRF, rocBLAS TRSM in place); H's eigendecomposition: rocSOLVER
+// SYEVD.  The block is stored row-major (n x m), so each nonzero of a CSR row
+// reads one contiguous row of the block; for the BLAS calls that is the
+// column-major m x n matrix X^T.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <random>
+#include <string>
+#include <vector>
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include "psf_internal.hpp"
+
+namespace psf {
+namespace {
+
+constexpr uint32_t kMaxBlock = 1024;
+constexpr uint32_t kMaxK = 256;   // the residual kernel gives one thread per wanted mode
+
+int set_err(psf_ctx* ctx, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    return ctx_set_error(ctx, code, buf);
+}
+
+#define EIG_HIPCHK(ctx, expr)                                                                                 \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess) return set_err((ctx), PSF_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+#define EIG_BLASCHK(ctx, expr)                                                                                \
+    do {                                                                                                      \
+        rocblas_status s_ = (expr);                                                                           \
+        if (s_ != rocblas_status_success)                                                                     \
+            return set_err((ctx), PSF_ERR_HIP, "%s: rocBLAS status %d", #expr, static_cast<int>(s_));         \
+    } while (0)
+
+// out = alpha * (A Y) + beta * Y + gamma * Z (Z may be null): one wave per
+// row, the m columns of a row spread over the lanes
+__global__ __launch_bounds__(256) void k_csr_block(uint32_t n, uint32_t m, const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ ci, const double* __restrict__ av,
+                                                   const double* __restrict__ Y, const double* __restrict__ Z,
+                                                   double alpha, double beta, double gamma, double* __restrict__ out)
+{
+    const uint32_t row = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6));
+    if (row >= n) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int32_t b = rp[row], e = rp[row + 1];
+    for (uint32_t c = lane; c < m; c += 64u) {
+        double s = 0.0;
+        for (int32_t q = b; q < e; ++q) s += av[q] * Y[static_cast<size_t>(ci[q]) * m + c];
+        const size_t o = static_cast<size_t>(row) * m + c;
+        double r = alpha * s + beta * Y[o];
+        if (Z) r += gamma * Z[o];
+        out[o] = r;
+    }
+}
+
+// part[b * k + c] = sum over rows r = b, b + grid, ... of
+// (AX[r][c] - w[c] X[r][c])^2, for c < k
+__global__ __launch_bounds__(256) void k_residual_part(uint32_t n, uint32_t m, uint32_t k, const double* __restrict__ AX,
+                                                       const double* __restrict__ X, const double* __restrict__ w,
+                                                       double* __restrict__ part)
+{
+    const uint32_t c = threadIdx.x;
+    if (c >= k) return;
+    const double wc = w[c];
+    double s = 0.0;
+    for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {
+        const double d = AX[static_cast<size_t>(r) * m + c] - wc * X[static_cast<size_t>(r) * m + c];
+        s += d * d;
+    }
+    part[static_cast<size_t>(blockIdx.x) * k + c] = s;
+}
+
+__global__ __launch_bounds__(256) void k_residual_sum(uint32_t k, uint32_t parts, const double* __restrict__ part,
+                                                      double* __restrict__ sq)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= k) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < parts; ++b) s += part[static_cast<size_t>(b) * k + c];
+    sq[c] = s;
+}
+
+// H <- (H + H^T) / 2 for the column-major m x m H
+__global__ void k_symmetrise(uint32_t m, double* __restrict__ H)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < m && i > j) {
+        const double v = 0.5 * (H[static_cast<size_t>(j) * m + i] + H[static_cast<size_t>(i) * m + j]);
+        H[static_cast<size_t>(j) * m + i] = v;
+        H[static_cast<size_t>(i) * m + j] = v;
+    }
+}
+
+struct DevMem {
+    std::vector<void*> ptrs;
+    ~DevMem()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    hipError_t alloc(T** p, size_t n)
+    {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+struct Blas {
+    rocblas_handle h = nullptr;
+    ~Blas()
+    {
+        if (h) (void)rocblas_destroy_handle(h);
+    }
+};
+
+}  // namespace
+}  // namespace psf
+
+using namespace psf;
+
+extern "C" {
+
+int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* values,
+                       uint32_t K, uint32_t guard, uint32_t degree, double tol, uint32_t max_products, uint64_t seed,
+                       double* lambda_out, double* X_out, uint32_t* products_out)
+{
+    if (!ctx) return PSF_ERR_ARG;
+    if (!row_ptr || !col_idx || !values || !lambda_out || !X_out) return set_err(ctx, PSF_ERR_ARG, "null argument");
+    const uint64_t m64 = uint64_t(K) + guard;
+    if (K == 0 || K > kMaxK || guard == 0 || m64 > kMaxBlock || m64 >= n || degree == 0 || degree > 1000 ||
+        !(tol > 0.0) || n > (1u << 30))
+        return set_err(ctx, PSF_ERR_ARG,
+                       "eigensolve: need 1 <= K <= %u, guard >= 1, K + guard <= %u and < n, 1 <= degree <= 1000, tol > 0",
+                       kMaxK, kMaxBlock);
+    const uint32_t m = static_cast<uint32_t>(m64);
+    if (row_ptr[0] != 0) return set_err(ctx, PSF_ERR_ARG, "eigensolve: row_ptr[0] must be 0");
+    const int32_t nnz = row_ptr[n];
+    if (nnz <= 0) return set_err(ctx, PSF_ERR_ARG, "eigensolve: empty matrix");
+    // Gershgorin bound of the top of the spectrum, and the CSR checks
+    double lmax = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (row_ptr[i + 1] < row_ptr[i]) return set_err(ctx, PSF_ERR_ARG, "eigensolve: row pointers decrease at %u", i);
+        double s = 0.0;
+        for (int32_t q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
+            if (col_idx[q] < 0 || static_cast<uint32_t>(col_idx[q]) >= n)
+                return set_err(ctx, PSF_ERR_ARG, "eigensolve: column index out of range in row %u", i);
+            if (!std::isfinite(values[q])) return set_err(ctx, PSF_ERR_ARG, "eigensolve: non-finite value in row %u", i);
+            s += std::fabs(values[q]);
+        }
+        lmax = std::max(lmax, s);
+    }
+    if (!(lmax > 0.0)) return set_err(ctx, PSF_ERR_ARG, "eigensolve: zero matrix");
+    EIG_HIPCHK(ctx, hipSetDevice(ctx->device));
+    const hipStream_t st = ctx->stream;
+    const size_t blk = size_t(n) * m;
+    const uint32_t rparts = std::min<uint32_t>(1024, n);
+    DevMem mem;
+    int32_t *drp, *dci, *dinfo;
+    double *dav, *X, *Y, *P, *Q, *H, *dw, *dE, *dpart, *dsq;
+    EIG_HIPCHK(ctx, mem.alloc(&drp, size_t(n) + 1));
+    EIG_HIPCHK(ctx, mem.alloc(&dci, size_t(nnz)));
+    EIG_HIPCHK(ctx, mem.alloc(&dav, size_t(nnz)));
+    EIG_HIPCHK(ctx, mem.alloc(&X, blk));
+    EIG_HIPCHK(ctx, mem.alloc(&Y, blk));
+    EIG_HIPCHK(ctx, mem.alloc(&P, blk));
+    EIG_HIPCHK(ctx, mem.alloc(&Q, blk));
+    EIG_HIPCHK(ctx, mem.alloc(&H, size_t(m) * m));
+    EIG_HIPCHK(ctx, mem.alloc(&dw, m));
+    EIG_HIPCHK(ctx, mem.alloc(&dE, m));
+    EIG_HIPCHK(ctx, mem.alloc(&dpart, size_t(rparts) * K));
+    EIG_HIPCHK(ctx, mem.alloc(&dsq, K));
+    EIG_HIPCHK(ctx, mem.alloc(&dinfo, 1));
+    EIG_HIPCHK(ctx, hipMemcpy(drp, row_ptr, (size_t(n) + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    EIG_HIPCHK(ctx, hipMemcpy(dci, col_idx, size_t(nnz) * sizeof(int32_t), hipMemcpyHostToDevice));
+    EIG_HIPCHK(ctx, hipMemcpy(dav, values, size_t(nnz) * sizeof(double), hipMemcpyHostToDevice));
+    {
+        std::mt19937_64 rng(seed);
+        std::normal_distribution<double> nd(0.0, 1.0);
+        std::vector<double> r0(blk);
+        for (double& v : r0) v = nd(rng);
+        EIG_HIPCHK(ctx, hipMemcpy(Y, r0.data(), blk * sizeof(double), hipMemcpyHostToDevice));
+    }
+    Blas blas;
+    EIG_BLASCHK(ctx, rocblas_create_handle(&blas.h));
+    EIG_BLASCHK(ctx, rocblas_set_stream(blas.h, st));
+    const rocblas_int mi = static_cast<rocblas_int>(m), ni = static_cast<rocblas_int>(n);
+    const double one = 1.0, zero = 0.0;
+    uint32_t products = 0;
+    const unsigned grid = (n + 3) / 4;
+    auto apply = [&](const double* in, const double* z, double a, double b, double g, double* out) {
+        k_csr_block<<<grid, 256, 0, st>>>(n, m, drp, dci, dav, in, z, a, b, g, out);
+        ++products;
+        return hipGetLastError();
+    };
+    int32_t info = 0;
+    // x <- x R^-1 with x^T x = R^T R, twice (Cholesky QR2)
+    auto orthonormalise = [&](double* x) -> int {
+        for (int pass = 0; pass < 2; ++pass) {
+            EIG_BLASCHK(ctx, rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_transpose, mi, mi, ni,
+                                           &one, x, mi, x, mi, &zero, H, mi));
+            EIG_BLASCHK(ctx, rocsolver_dpotrf(blas.h, rocblas_fill_upper, mi, H, mi, dinfo));
+            EIG_HIPCHK(ctx, hipMemcpyAsync(&info, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            EIG_HIPCHK(ctx, hipStreamSynchronize(st));
+            if (info != 0)
+                return set_err(ctx, PSF_ERR_NONFINITE, "eigensolve: the block lost rank (Cholesky pivot %d)", info);
+            // (x R^-1)^T = R^-T x^T: a left solve with R^T on the column-major x^T
+            EIG_BLASCHK(ctx, rocblas_dtrsm(blas.h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                                           rocblas_diagonal_non_unit, mi, ni, &one, H, mi, x, mi));
+        }
+        return PSF_OK;
+    };
+    // out = in V (row-major n x m times the column-major m x m V in H):
+    // out^T = V^T in^T
+    auto rotate = [&](const double* in, double* out) -> int {
+        EIG_BLASCHK(ctx, rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, mi, ni, mi, &one, H,
+                                       mi, in, mi, &zero, out, mi));
+        return PSF_OK;
+    };
+    // X, Y, P and Q always name the four block buffers in some order
+    std::swap(X, Y);   // the Gaussian start
+    if (int rc = orthonormalise(X)) return rc;
+    std::vector<double> w(m), sq(K);
+    bool converged = false;
+    double worst = 0.0;
+    for (;;) {
+        // Rayleigh-Ritz: Y = A X, H = X^T Y, then X <- X V and Y <- Y V
+        EIG_HIPCHK(ctx, apply(X, nullptr, 1.0, 0.0, 0.0, Y));
+        EIG_BLASCHK(ctx, rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_transpose, mi, mi, ni, &one, X,
+                                       mi, Y, mi, &zero, H, mi));
+        k_symmetrise<<<dim3((m + 255) / 256, m), 256, 0, st>>>(m, H);
+        EIG_HIPCHK(ctx, hipGetLastError());
+        EIG_BLASCHK(ctx, rocsolver_dsyevd(blas.h, rocblas_evect_original, rocblas_fill_upper, mi, H, mi, dw, dE, dinfo));
+        EIG_HIPCHK(ctx, hipMemcpyAsync(&info, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        EIG_HIPCHK(ctx, hipMemcpyAsync(w.data(), dw, m * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (int rc = rotate(X, P)) return rc;
+        std::swap(X, P);
+        if (int rc = rotate(Y, P)) return rc;
+        std::swap(Y, P);
+        k_residual_part<<<rparts, 256, 0, st>>>(n, m, K, Y, X, dw, dpart);
+        EIG_HIPCHK(ctx, hipGetLastError());
+        k_residual_sum<<<(K + 255) / 256, 256, 0, st>>>(K, rparts, dpart, dsq);
+        EIG_HIPCHK(ctx, hipGetLastError());
+        EIG_HIPCHK(ctx, hipMemcpyAsync(sq.data(), dsq, K * sizeof(double), hipMemcpyDeviceToHost, st));
+        EIG_HIPCHK(ctx, hipStreamSynchronize(st));
+        if (info != 0) return set_err(ctx, PSF_ERR_NONFINITE, "eigensolve: SYEVD did not converge (%d)", info);
+        worst = 0.0;
+        for (uint32_t j = 0; j < K; ++j) worst = std::max(worst, std::sqrt(sq[j]));
+        if (!std::isfinite(worst)) return set_err(ctx, PSF_ERR_NONFINITE, "eigensolve: non-finite residual");
+        if (worst <= tol * lmax) {
+            converged = true;
+            break;
+        }
+        if (uint64_t(products) + degree + 2 > max_products) break;
+        // Chebyshev filter of the given degree on [w[m-1], lmax]:
+        // T1 = (A X - c X) / e, T(k+1) = 2 (A Tk - c Tk) / e - T(k-1)
+        const double lo = w[m - 1], c = 0.5 * (lmax + lo), e = 0.5 * (lmax - lo);
+        if (!(e > 0.0)) return set_err(ctx, PSF_ERR_NONFINITE, "eigensolve: empty filter interval");
+        double* t[3] = {X, P, Q};   // T(k-1), T(k), next; rotates through X, P, Q
+        EIG_HIPCHK(ctx, apply(t[0], nullptr, 1.0 / e, -c / e, 0.0, t[1]));
+        for (uint32_t d = 2; d <= degree; ++d) {
+            EIG_HIPCHK(ctx, apply(t[1], t[0], 2.0 / e, -2.0 * c / e, -1.0, t[2]));
+            std::rotate(t, t + 1, t + 3);
+        }
+        if (t[1] != X) std::swap(X, t[1] == P ? P : Q);   // X <- T(degree); the old X is scratch
+        if (int rc = orthonormalise(X)) return rc;
+    }
+    if (products_out) *products_out = products;
+    if (!converged)
+        return set_err(ctx, PSF_ERR_STATE, "eigensolve: residual %.3e x lambda_max after %u products", worst / lmax,
+                       products);
+    // eigenvalues and the K smallest vectors, n x K column-major
+    std::vector<double> xb(blk);
+    EIG_HIPCHK(ctx, hipMemcpy(xb.data(), X, blk * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t j = 0; j < K; ++j) {
+        lambda_out[j] = w[j];
+        for (uint32_t i = 0; i < n; ++i) X_out[size_t(j) * n + i] = xb[size_t(i) * m + j];
+    }
+    return PSF_OK;
+}
+
+}  // extern "C"

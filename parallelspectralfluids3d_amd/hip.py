@@ -134,6 +134,33 @@ def read_cache(path):
     return dict(desc=d, lam=lam, C=C, Phi=P, gravity=g)
 
 
+def eigensolve_csr(ctx: Context, A, K: int, guard: int = 0, degree: int = 20, tol: float = 1e-11,
+                   max_products: int = 20000, seed: int = 0):
+    """The K smallest nonzero eigenpairs of the symmetric positive
+    semi-definite sparse A (anything scipy.sparse.csr_matrix accepts) on the
+    context's GPU (psf_eigensolve_csr: Chebyshev-filtered subspace iteration,
+    replacing the shift-invert eigensolve of psfsolver.cpp:267-341).
+    Returns (lam[K] ascending, X[n, K] orthonormal columns, products with A).
+    guard = 0 picks max(16, K // 2) extra block vectors."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    if A.shape != (n, n):
+        raise ValueError("A must be square")
+    if guard <= 0:
+        guard = max(16, K // 2)
+    rp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+    ci = np.ascontiguousarray(A.indices, dtype=np.int32)
+    av = np.ascontiguousarray(A.data, dtype=np.float64)
+    lam = np.empty(K)
+    X = np.empty((K, n))                 # column-major n x K
+    used = ct.c_uint32(0)
+    rc = L.load().psf_eigensolve_csr(ctx.h, n, _ptr(rp), _ptr(ci), _ptr(av), K, guard, degree, tol, max_products,
+                                     seed, _ptr(lam), _ptr(X), ct.byref(used))
+    ctx.check(rc, "psf_eigensolve_csr")
+    return lam, X.T, int(used.value)
+
+
 class Scene:
     """One scene resident on the device (C, Phi, omega, u, particles)."""
 
