@@ -40,10 +40,10 @@ FP64_PEAK = 78.6e12
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --counter-pass`
 # (fetched KiB x 2, the factor a 1 GiB hipMemcpy reads back, plus written
 # KiB); counters cannot be read from inside this process.
-RECON_TRAFFIC_64_256 = 821_148_672          # k_reconstruct<float, 1>: 399,392 KiB read, 3,120 KiB written
-WINDOW_TRAFFIC_64_256_W32 = 928_985_152     # k_reconstruct_mfma<float, 2>: 403,683 / 99,847 KiB
-TRACE_TRAFFIC_64_1M = 37_635_776            # k_trace: 10,185 / 16,384 KiB
-TRACE_FRAMES_TRAFFIC_64_1M = 65_561_664     # k_trace_frames (8 frames per launch): 23,821 / 16,384 KiB
+RECON_TRAFFIC_64_256 = 821_147_648          # k_reconstruct<float, 1>: 399,391 KiB read, 3,120 KiB written
+WINDOW_TRAFFIC_64_256_W32 = 929_118_528     # k_reconstruct_mfma<float, 2>: 403,749 / 99,845 KiB
+TRACE_TRAFFIC_64_1M = 37_685_248            # k_trace: 10,209 / 16,384 KiB
+TRACE_FRAMES_TRAFFIC_64_1M = 66_196_608     # k_trace_frames (8 frames per launch): 24,131 / 16,384 KiB
 
 
 def log(msg):
