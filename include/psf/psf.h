@@ -211,6 +211,27 @@ PSF_API int psf_cache_read_header(const char* path, psf_scene_desc* desc, uint32
 PSF_API int psf_cache_read(const char* path, double* Phi, double* lambda, double* gravity, double* C);
 PSF_API int psf_scene_load_cache(psf_scene* scene, const char* path);
 
+/* ---- structure tensor from any basis, on the device -------------------------------
+ * Replaces the voxel loop of PSFSolver::buildAdvection (psfsolver.cpp:431-551)
+ * for any basis (eigensolved, obstacle scenes); psf_scene_build_box_advection
+ * covers the analytic closed box.  Inputs as buildAdvection holds them:
+ * Phi (F x K, column-major FP64, velBasisField), V = curl Phi (nE x K,
+ * column-major FP64, psfsolver.cpp:342), lambda[K] (finite, non-zero), and
+ * one term per (inside voxel, edge slot) of psfsolver.cpp:456-533: its edge,
+ * its faces a and b, and the coefficient s_h * c(a, b).  C_i = (1/lambda_i)
+ * V^T W_i, W_i[e][j] = sum over e's terms of coef (Phi[a][i] Phi[b][j] -
+ * Phi[a][j] Phi[b][i]), is formed on the GPU (FP64 matrix cores) and stored
+ * in the scene's C precision (FP16 is refused: upload it).  Synchronous;
+ * allocates Phi, V, the terms and a few GB of wedge matrices for the call. */
+PSF_API int psf_scene_build_advection(psf_scene* scene, const double* Phi, const double* V, uint32_t nE,
+                                      const double* lambda, uint32_t nterms, const uint32_t* term_edge,
+                                      const uint32_t* term_a, const uint32_t* term_b, const double* term_coef);
+/* C back to the host in the upload layout (K matrices C_i, column-major), for
+ * scenes whose C precision is FP64 or FP32 (others store it packed:
+ * PSF_ERR_UNSUPPORTED).  No counterpart in the reference; it lets a C built on
+ * the device go into a scene cache (psf_cache_write).  Synchronous. */
+PSF_API int psf_scene_get_advection(psf_scene* scene, double* C);
+
 /* ---- eigensolve for the basis ---------------------------------------------------
  * Replaces the shift-invert eigensolve of the pruned Laplacian in
  * PSFSolver::buildLaplace (psfsolver.cpp:267-341, Spectra SymEigsShiftSolver

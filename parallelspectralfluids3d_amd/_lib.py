@@ -86,6 +86,8 @@ def load():
         "psf_cache_read_header": (i32, [ct.c_char_p, vp, vp, vp]),
         "psf_cache_read": (i32, [ct.c_char_p, vp, vp, vp, vp]),
         "psf_scene_load_cache": (i32, [vp, ct.c_char_p]),
+        "psf_scene_build_advection": (i32, [vp, vp, vp, u32, vp, u32, vp, vp, vp, vp]),
+        "psf_scene_get_advection": (i32, [vp, vp]),
         "psf_eigensolve_csr": (i32, [vp, u32, vp, vp, vp, u32, u32, u32, dbl, u32, ct.c_uint64, vp, vp,
                                      ct.POINTER(ct.c_uint32)]),
         "psf_density": (i32, [vp, vp, vp]),

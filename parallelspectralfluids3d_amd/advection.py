@@ -77,6 +77,20 @@ def build_advection(g: Grid, resolution: float, inside, Phi_cm, lam, V_cm=None, 
     return C
 
 
+def advection_terms(g: Grid, resolution: float, inside):
+    """(edge, a, b, s_h * c) per (inside voxel, edge slot): the inputs of
+    hip.Scene.build_advection, which forms C on the device."""
+    e, a, b, c = wedge_terms(g, inside)
+    return e, a, b, scale_sh(resolution) * c
+
+
+def build_advection_device(scene, g: Grid, resolution: float, inside, Phi_cm, lam, V_cm=None):
+    """build_advection on the GPU, into the scene's C (any precision but FP16)."""
+    if V_cm is None:
+        V_cm = (curl_operator(g, resolution) @ Phi_cm.T).T
+    scene.build_advection(Phi_cm, V_cm, lam, advection_terms(g, resolution, inside))
+
+
 # ---------------------------------------------------------------------------
 # Closed boxes with the analytic sin/cos basis.
 #

@@ -22,7 +22,8 @@ SOLVER_CHECK = os.path.join(PKG, "solver_check")
 ARCH = os.environ.get("PSF_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["psf_kernels.hip", "psf_persistent.hip", "psf_api.hip", "psf_server.hip", "psf_box.hip",
-           "psf_density.hip", "psf_cache.hip", "psf_eigen.hip"]
+           "psf_density.hip", "psf_cache.hip", "psf_eigen.hip",
+           "psf_advection.hip"]
 
 CXXFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -410,6 +410,7 @@ int scene_read_results(psf_scene* s, double* omega, float* xyzw)
 
 TraceParams scene_trace_params(const psf_scene* s) { return trace_params(s); }
 int scene_flush_traces(psf_scene* s) { return flush_traces(s); }
+int scene_window_flush(psf_scene* s) { return window_flush(s); }
 int scene_check_status(psf_scene* s) { return check_status(s); }
 int ctx_set_error(psf_ctx* ctx, int code, const std::string& msg)
 {

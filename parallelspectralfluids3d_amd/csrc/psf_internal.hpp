@@ -25,6 +25,8 @@ int scene_read_results(psf_scene* s, double* omega, float* xyzw);
 TraceParams scene_trace_params(const psf_scene* s);
 // launches the particle passes of deferred frames
 int scene_flush_traces(psf_scene* s);
+// ends a frame window early (the state-changing calls' first step)
+int scene_window_flush(psf_scene* s);
 int scene_check_status(psf_scene* s);
 int ctx_set_error(psf_ctx* ctx, int code, const std::string& msg);
 size_t pool_class_bytes(size_t bytes);   // bytes a pooled allocation of this size holds

@@ -209,6 +209,29 @@ class Scene:
         assert C.shape == (self.K, self.K, self.K)
         self._c(self.ctx.lib.psf_scene_upload_advection(self.h, _ptr(C)), "upload_advection")
 
+    def build_advection(self, Phi_cm, V_cm, lam, terms):
+        """C from any basis on the device (psf_scene_build_advection).  Phi_cm
+        (K, F) and V_cm = curl Phi (K, E) as basis.py returns them; terms =
+        (edge, face a, face b, coefficient) arrays, one entry per (inside
+        voxel, edge slot), as advection.wedge_terms and scale_sh give them."""
+        Phi = np.ascontiguousarray(Phi_cm, dtype=np.float64)     # (K, F): column-major F x K
+        V = np.ascontiguousarray(V_cm, dtype=np.float64)         # (K, E): column-major E x K
+        assert Phi.shape == (self.K, self.grid.nF) and V.shape[0] == self.K
+        e, a, b, c = terms
+        e = np.ascontiguousarray(e, dtype=np.uint32)
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self._c(self.ctx.lib.psf_scene_build_advection(self.h, _ptr(Phi), _ptr(V), V.shape[1], _ptr(lam), e.size,
+                                                       _ptr(e), _ptr(a), _ptr(b), _ptr(c)), "build_advection")
+
+    def get_advection(self):
+        """C (K, K, K) in the upload layout; FP64 and FP32 scenes only."""
+        C = np.empty((self.K, self.K, self.K))
+        self._c(self.ctx.lib.psf_scene_get_advection(self.h, _ptr(C)), "get_advection")
+        return C
+
     def generate_box_basis(self, table, gravity=None):
         """Phi, lambda (and gravity) of a closed box's analytic modes, written on
         the device from the normalised mode table (columns lambda, a, b, c and
