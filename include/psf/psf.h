@@ -245,8 +245,7 @@ PSF_API int psf_scene_get_advection(psf_scene* scene, double* C);
  * lift an unwanted one with a penalty term first.
  * Out: lambda[K] ascending, X (n x K, column-major, orthonormal columns; the
  * sign and the basis inside a repeated eigenvalue are arbitrary), and the
- * number of products with A used.  Limits: K <= 256, K + guard <= 1024 and
- * < n.  PSF_ERR_STATE if max_products is reached first.  Synchronous. */
+ * number of products with A used.  Limits: K + guard <= 1024 and < n.  PSF_ERR_STATE if max_products is reached first.  Synchronous. */
 PSF_API int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                                const double* values, uint32_t K, uint32_t guard, uint32_t degree, double tol,
                                uint32_t max_products, uint64_t seed, double* lambda, double* X,

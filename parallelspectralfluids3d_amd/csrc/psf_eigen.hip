@@ -24,7 +24,6 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
-#include <random>
 #include <string>
 #include <vector>
 
@@ -37,7 +36,6 @@ namespace psf {
 namespace {
 
 constexpr uint32_t kMaxBlock = 1024;
-constexpr uint32_t kMaxK = 256;   // the residual kernel gives one thread per wanted mode
 
 int set_err(psf_ctx* ctx, int code, const char* fmt, ...)
 {
@@ -62,7 +60,9 @@ int set_err(psf_ctx* ctx, int code, const char* fmt, ...)
     } while (0)
 
 // out = alpha * (A Y) + beta * Y + gamma * Z (Z may be null): one wave per
-// row, the m columns of a row spread over the lanes
+// row; lane l holds columns l, l + 64, ..., l + 64 (R - 1) (R = ceil(m / 64)),
+// so each nonzero issues R independent loads of one block row
+template <int R>
 __global__ __launch_bounds__(256) void k_csr_block(uint32_t n, uint32_t m, const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ ci, const double* __restrict__ av,
                                                    const double* __restrict__ Y, const double* __restrict__ Z,
@@ -72,14 +72,57 @@ __global__ __launch_bounds__(256) void k_csr_block(uint32_t n, uint32_t m, const
     if (row >= n) return;
     const uint32_t lane = threadIdx.x & 63u;
     const int32_t b = rp[row], e = rp[row + 1];
-    for (uint32_t c = lane; c < m; c += 64u) {
-        double s = 0.0;
-        for (int32_t q = b; q < e; ++q) s += av[q] * Y[static_cast<size_t>(ci[q]) * m + c];
-        const size_t o = static_cast<size_t>(row) * m + c;
-        double r = alpha * s + beta * Y[o];
-        if (Z) r += gamma * Z[o];
-        out[o] = r;
+    double s[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) s[r] = 0.0;
+    for (int32_t q = b; q < e; ++q) {
+        const double a = av[q];
+        const double* y = Y + static_cast<size_t>(ci[q]) * m + lane;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (lane + 64u * r < m) s[r] += a * y[64 * r];
     }
+    const size_t o = static_cast<size_t>(row) * m + lane;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (lane + 64u * r >= m) break;
+        double v = alpha * s[r] + beta * Y[o + 64 * r];
+        if (Z) v += gamma * Z[o + 64 * r];
+        out[o + 64 * r] = v;
+    }
+}
+
+template <int R>
+void launch_csr(unsigned grid, hipStream_t st, uint32_t n, uint32_t m, const int32_t* rp, const int32_t* ci,
+                const double* av, const double* Y, const double* Z, double a, double b, double g, double* out)
+{
+    k_csr_block<R><<<grid, 256, 0, st>>>(n, m, rp, ci, av, Y, Z, a, b, g, out);
+}
+
+using CsrLaunch = void (*)(unsigned, hipStream_t, uint32_t, uint32_t, const int32_t*, const int32_t*, const double*,
+                           const double*, const double*, double, double, double, double*);
+constexpr CsrLaunch kCsrLaunch[16] = {launch_csr<1>,  launch_csr<2>,  launch_csr<3>,  launch_csr<4>,
+                                      launch_csr<5>,  launch_csr<6>,  launch_csr<7>,  launch_csr<8>,
+                                      launch_csr<9>,  launch_csr<10>, launch_csr<11>, launch_csr<12>,
+                                      launch_csr<13>, launch_csr<14>, launch_csr<15>, launch_csr<16>};
+
+// Gaussian start block: Box-Muller on two SplitMix64 draws of (seed, index)
+__device__ __forceinline__ uint64_t splitmix(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_gaussian(uint64_t seed, size_t count, double* __restrict__ out)
+{
+    const size_t t = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= count) return;
+    const uint64_t a = splitmix(seed ^ (2 * t)), b = splitmix(seed ^ (2 * t + 1));
+    const double u1 = (static_cast<double>(a >> 11) + 1.0) * 0x1p-53;   // (0, 1]
+    const double u2 = static_cast<double>(b >> 11) * 0x1p-53;            // [0, 1)
+    out[t] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
 // part[b * k + c] = sum over rows r = b, b + grid, ... of
@@ -88,15 +131,15 @@ __global__ __launch_bounds__(256) void k_residual_part(uint32_t n, uint32_t m, u
                                                        const double* __restrict__ X, const double* __restrict__ w,
                                                        double* __restrict__ part)
 {
-    const uint32_t c = threadIdx.x;
-    if (c >= k) return;
-    const double wc = w[c];
-    double s = 0.0;
-    for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {
-        const double d = AX[static_cast<size_t>(r) * m + c] - wc * X[static_cast<size_t>(r) * m + c];
-        s += d * d;
+    for (uint32_t c = threadIdx.x; c < k; c += 256u) {
+        const double wc = w[c];
+        double s = 0.0;
+        for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {
+            const double d = AX[static_cast<size_t>(r) * m + c] - wc * X[static_cast<size_t>(r) * m + c];
+            s += d * d;
+        }
+        part[static_cast<size_t>(blockIdx.x) * k + c] = s;
     }
-    part[static_cast<size_t>(blockIdx.x) * k + c] = s;
 }
 
 __global__ __launch_bounds__(256) void k_residual_sum(uint32_t k, uint32_t parts, const double* __restrict__ part,
@@ -157,11 +200,11 @@ int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const i
     if (!ctx) return PSF_ERR_ARG;
     if (!row_ptr || !col_idx || !values || !lambda_out || !X_out) return set_err(ctx, PSF_ERR_ARG, "null argument");
     const uint64_t m64 = uint64_t(K) + guard;
-    if (K == 0 || K > kMaxK || guard == 0 || m64 > kMaxBlock || m64 >= n || degree == 0 || degree > 1000 ||
-        !(tol > 0.0) || n > (1u << 30))
+    if (K == 0 || guard == 0 || m64 > kMaxBlock || m64 >= n || degree == 0 || degree > 1000 || !(tol > 0.0) ||
+        n > (1u << 30))
         return set_err(ctx, PSF_ERR_ARG,
-                       "eigensolve: need 1 <= K <= %u, guard >= 1, K + guard <= %u and < n, 1 <= degree <= 1000, tol > 0",
-                       kMaxK, kMaxBlock);
+                       "eigensolve: need K >= 1, guard >= 1, K + guard <= %u and < n, 1 <= degree <= 1000, tol > 0",
+                       kMaxBlock);
     const uint32_t m = static_cast<uint32_t>(m64);
     if (row_ptr[0] != 0) return set_err(ctx, PSF_ERR_ARG, "eigensolve: row_ptr[0] must be 0");
     const int32_t nnz = row_ptr[n];
@@ -203,13 +246,8 @@ int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const i
     EIG_HIPCHK(ctx, hipMemcpy(drp, row_ptr, (size_t(n) + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
     EIG_HIPCHK(ctx, hipMemcpy(dci, col_idx, size_t(nnz) * sizeof(int32_t), hipMemcpyHostToDevice));
     EIG_HIPCHK(ctx, hipMemcpy(dav, values, size_t(nnz) * sizeof(double), hipMemcpyHostToDevice));
-    {
-        std::mt19937_64 rng(seed);
-        std::normal_distribution<double> nd(0.0, 1.0);
-        std::vector<double> r0(blk);
-        for (double& v : r0) v = nd(rng);
-        EIG_HIPCHK(ctx, hipMemcpy(Y, r0.data(), blk * sizeof(double), hipMemcpyHostToDevice));
-    }
+    k_gaussian<<<static_cast<unsigned>((blk + 255) / 256), 256, 0, st>>>(seed, blk, Y);
+    EIG_HIPCHK(ctx, hipGetLastError());
     Blas blas;
     EIG_BLASCHK(ctx, rocblas_create_handle(&blas.h));
     EIG_BLASCHK(ctx, rocblas_set_stream(blas.h, st));
@@ -217,8 +255,9 @@ int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const i
     const double one = 1.0, zero = 0.0;
     uint32_t products = 0;
     const unsigned grid = (n + 3) / 4;
+    const CsrLaunch csr = kCsrLaunch[(m + 63) / 64 - 1];
     auto apply = [&](const double* in, const double* z, double a, double b, double g, double* out) {
-        k_csr_block<<<grid, 256, 0, st>>>(n, m, drp, dci, dav, in, z, a, b, g, out);
+        csr(grid, st, n, m, drp, dci, dav, in, z, a, b, g, out);
         ++products;
         return hipGetLastError();
     };
