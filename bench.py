@@ -492,6 +492,67 @@ def config2_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def obstacle_bench(args):
+    """A 64^3 box with the voxelised central sphere of config 2, K=256, built
+    entirely on the device: the pruned Laplacian's eigenbasis
+    (psf_eigensolve_csr, penalty-lifted), C from that basis
+    (psf_scene_build_advection), then the windowed frame loop with 2^20
+    particles in fluid cells, BF16 hi+lo C and FP32 Phi.  Checks that need
+    no oracle at this size: every mode's residual against M_II and its
+    divergence on deep voxels, and omega's norm after the frames (no
+    viscosity or forcing: the renormalisation keeps it 1)."""
+    from parallelspectralfluids3d_amd import _lib as L
+    from parallelspectralfluids3d_amd import advection, basis, hip
+    from parallelspectralfluids3d_amd.grid import Grid, sphere_obstacle_mask
+    n, K, N = 64, 256, 1 << 20
+    g = Grid(n, n, n)
+    h = 1.0 / n
+    amin, amax = (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)
+    inside = sphere_obstacle_mask(g, h, amin).astype(bool).reshape(-1)
+    ctx = hip.Context(0)
+    t = time.time()
+    Phi, lam, info = basis.laplacian_eigenbasis_device(g, h, inside, K, ctx)
+    t_eig = time.time() - t
+    log(f"obstacle: eigenbasis K={K} in {t_eig:.1f}s ({info['products']} products)")
+    MII, act = basis.pruned_laplacian(g, h, inside)
+    Va = Phi[:, act].T
+    res = float(np.linalg.norm(MII @ Va - Va * lam, axis=0).max())
+    div = float(np.abs(basis.deep_divergence(g, inside, act) @ Va).max())
+    del Va, MII
+    t = time.time()
+    s = hip.Scene(ctx, hip.make_desc(g, h, K, L.C_BF16X2, L.PHI_FP32, N, amin, amax))
+    s.upload_basis(Phi, lam, basis.project(Phi, basis.gravity_field(g, inside)))
+    advection.build_advection_device(s, g, h, inside, Phi, lam)
+    t_c = time.time() - t
+    log(f"obstacle: C and upload in {t_c:.1f}s")
+    del Phi
+    rng = np.random.default_rng(3)
+    x, y, z = g.voxel_coords()
+    vox = np.flatnonzero(inside)
+    pick = vox[rng.integers(0, vox.size, N)]
+    parts = np.zeros((N, 4), np.float32)
+    parts[:, :3] = np.asarray(amin) + (np.stack([x[pick], y[pick], z[pick]], 1) + rng.random((N, 3))) * h
+    parts[:, 3] = 60.0
+    s.set_omega(basis.unit_energy_omega(K, 0))
+    s.set_params(0.01, 0.0, False, lifetime=60.0)
+    s.set_particles(parts)
+    s.set_frame_window(32)
+    s.time_phase(L.PHASE_FRAME, 64)
+    frames = 256
+    ms = s.time_phase(L.PHASE_FRAME, frames) / frames
+    w = s.get_omega()
+    s.close()
+    ctx.close()
+    out = {"obstacle": {
+        "workload": f"{n}^3 box, central sphere obstacle r=0.25, K={K} eigenbasis of the pruned M and C both built on "
+                    f"the GPU, BF16 hi+lo C, FP32 Phi, frame window 32, {N} particles in fluid cells, dt=0.01",
+        "active_faces": int(act.size), "eigensolve_s": t_eig, "eigensolve": info, "C_build_and_upload_s": t_c,
+        "max_mode_residual": res, "max_mode_divergence": div,
+        "ms_per_frame": ms, "particle_steps_per_s": N / (ms * 1e-3), "frames": frames + 64,
+        "omega_norm_minus_1": float(np.linalg.norm(w) - 1.0)}}
+    print(json.dumps(out), flush=True)
+
+
 def counter_pass_copies(frames, ms):
     """Three device-to-device copies of 1 GiB with the HIP runtime's own copy
     kernel: the reference for the FETCH_SIZE / WRITE_SIZE counters (each copy
@@ -547,6 +608,9 @@ def main():
     ap.add_argument("--serving-profile", type=int, default=0, metavar="N",
                     help="only run the first N requests of the serving stream, all queued at once, through the "
                          "batched server (a short run for a profiler); prints one line and exits")
+    ap.add_argument("--obstacle", action="store_true",
+                    help="only the 64^3 sphere-obstacle scene (K=256) built on the GPU from its mask: eigensolve, "
+                         "C, frame loop; prints one line and exits (about a minute)")
     ap.add_argument("--large", action="store_true",
                     help="only the large single scene (128^3, K=512, 2^24 particles, 500 frames): timing and "
                          "size-independent checks; prints one line and exits (several minutes, ~60 GB of host memory)")
@@ -569,6 +633,10 @@ def main():
 
     if args.config2:
         config2_bench(args)
+        return
+
+    if args.obstacle:
+        obstacle_bench(args)
         return
 
     if args.serving_only:
