@@ -364,22 +364,44 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
             }
         }
         // ---- gather all y (one wave polls; bounded wait) ----------------------------
+        // Lane l reads records l, l + 64, ..., four at a time: a poll pass
+        // issues the loads of all still-missing records of the group before
+        // checking any, so for K <= 256 a pass costs one round trip to the
+        // other XCDs' writes, not one per record.
         if (wv == 0) {
             const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-            for (int i = lane; i < K; i += 64) {
-                const unsigned long long* rec = xb + static_cast<size_t>(kXStride) * i;
-                unsigned long long v0, v1;
-                for (;;) {
-                    v0 = __hip_atomic_load(rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    v1 = __hip_atomic_load(rec + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if ((v0 >> 32) == tag && (v1 >> 32) == tag) break;
-                    if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
-                        s_flag = 2;
-                        break;
+            constexpr int kGroup = 4;
+            const int nr = lane < K ? (K - 1 - lane) / 64 + 1 : 0;
+            for (int r0 = 0; r0 < nr && !s_flag; r0 += kGroup) {
+                uint32_t pend = (1u << min(kGroup, nr - r0)) - 1u;
+                while (pend) {
+                    unsigned long long v0[kGroup], v1[kGroup];
+#pragma unroll
+                    for (int r = 0; r < kGroup; ++r) {
+                        v0[r] = v1[r] = 0ull;
+                        if ((pend >> r) & 1u) {
+                            const unsigned long long* rec =
+                                xb + static_cast<size_t>(kXStride) * (lane + 64 * (r0 + r));
+                            v0[r] = __hip_atomic_load(rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            v1[r] = __hip_atomic_load(rec + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
                     }
-                    __builtin_amdgcn_s_sleep(2);
+#pragma unroll
+                    for (int r = 0; r < kGroup; ++r) {
+                        if (((pend >> r) & 1u) && (v0[r] >> 32) == tag && (v1[r] >> 32) == tag) {
+                            s_y[lane + 64 * (r0 + r)] = __longlong_as_double(
+                                static_cast<long long>(((v0[r] & 0xFFFFFFFFull) << 32) | (v1[r] & 0xFFFFFFFFull)));
+                            pend &= ~(1u << r);
+                        }
+                    }
+                    if (pend) {
+                        if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                            s_flag = 2;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
                 }
-                s_y[i] = __longlong_as_double(static_cast<long long>(((v0 & 0xFFFFFFFFull) << 32) | (v1 & 0xFFFFFFFFull)));
             }
         }
         __syncthreads();
