@@ -77,3 +77,45 @@ def test_mode_table_without_phi(dims):
     assert np.allclose(t2, tab, rtol=0, atol=1e-14)
     faces = np.random.default_rng(0).choice(g.nF, 200, replace=False)
     assert np.allclose(basis.box_mode_values(g, t2, faces), Phi[:, faces], rtol=0, atol=1e-14)
+
+
+def test_pruned_laplacian_null_space_and_penalty():
+    """The device eigensolve's inputs: M_II (pruned, symmetrised) annihilates
+    every deep voxel's gradient D^T e_c, exact eigenvectors of nonzero
+    eigenvalue are divergence-free on deep voxels, and M_II + p D^T D has
+    the same nonzero eigenpairs (so the penalty lifts only gradients)."""
+    from parallelspectralfluids3d_amd.grid import sphere_obstacle_mask
+    import scipy.sparse.linalg as sla
+    g = Grid(8, 8, 8)
+    h = 1.0 / 8
+    inside = sphere_obstacle_mask(g, h, (-0.5, -0.5, -0.5)).astype(bool).reshape(-1)
+    MII, act = basis.pruned_laplacian(g, h, inside)
+    D = basis.deep_divergence(g, inside, act)
+    assert D.shape == (int((inside & (g.face_inside_count(inside)[g.voxel_faces] == 2).all(1)).sum()), act.size)
+    assert abs(MII - MII.T).max() == 0.0
+    scale = abs(MII).max()
+    assert abs(MII @ D.T).max() < 1e-13 * scale                 # gradients are null vectors
+    w, V = np.linalg.eigh(MII.toarray())
+    nz = w > 1e-10 * w.max()
+    assert np.abs(D @ V[:, nz]).max() < 1e-10                    # nonzero modes are divergence-free
+    for p in (1.0, 16.0):
+        A = (MII + p * (D.T @ D)).tocsc()
+        wk = sla.eigsh(A, k=6, sigma=w[nz][0] * 0.5, which="LM", return_eigenvectors=False)
+        got = np.sort(wk)
+        # the six eigenvalues of A nearest half the first nonzero one include
+        # the first nonzero eigenvalues of M_II unchanged
+        assert np.min(np.abs(got - w[nz][0])) < 1e-12 * w.max()
+
+
+def test_device_eigenbasis_helpers_reorder():
+    """laplacian_eigenbasis_device puts the active faces in spatial order and
+    maps the eigenvectors back: the permutation is a bijection of the rows."""
+    from parallelspectralfluids3d_amd.grid import sphere_obstacle_mask
+    g = Grid(6, 6, 6)
+    inside = sphere_obstacle_mask(g, 1 / 6, (-0.5, -0.5, -0.5)).astype(bool).reshape(-1)
+    _, act = basis.pruned_laplacian(g, 1 / 6, inside)
+    fc = g.face_coords[act]
+    perm = np.lexsort((fc[:, 0], fc[:, 1], fc[:, 2]))
+    assert np.array_equal(np.sort(perm), np.arange(act.size))
+    z = fc[perm, 2]
+    assert np.all(np.diff(z) >= 0)
