@@ -519,157 +519,211 @@ __device__ __forceinline__ uint32_t respawn_draw(uint32_t seed, uint32_t step, u
     return static_cast<uint32_t>(z & 1023u);
 }
 
+// A particle's cell indices on each axis (O: its voxel, M/P: the voxels
+// below/above the half-voxel-shifted point), clamped to the grid.
+struct TraceCell {
+    int xO, xM, xP, yO, yM, yP, zO, zM, zP;
+};
+
+// Pow2: the voxel size is a power of two (cell_indices_pow2)
+template <bool Pow2>
+__device__ __forceinline__ TraceCell trace_cell(const TraceParams& P, const float4 p)
+{
+#pragma clang fp contract(off)
+    const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny), nz = static_cast<int>(P.nz);
+    const float sx = (P.aabb_min[0] + P.ext[0]) + p.x;
+    const float sy = (P.aabb_min[1] + P.ext[1]) + p.y;
+    const float sz = (P.aabb_min[2] + P.ext[2]) + p.z;
+    TraceCell c;
+    if constexpr (Pow2) {
+        cell_indices_pow2(sx, P.inv_h, c.xO, c.xM, c.xP);
+        cell_indices_pow2(sy, P.inv_h, c.yO, c.yM, c.yP);
+        cell_indices_pow2(sz, P.inv_h, c.zO, c.zM, c.zP);
+    } else {
+        const double res = P.res, hres = P.res / 2, rr = P.inv_res;
+        cell_indices(static_cast<double>(sx), res, hres, rr, c.xO, c.xM, c.xP);
+        cell_indices(static_cast<double>(sy), res, hres, rr, c.yO, c.yM, c.yP);
+        cell_indices(static_cast<double>(sz), res, hres, rr, c.zO, c.zM, c.zP);
+    }
+    c.xO = clampi(c.xO, 0, nx - 1); c.yO = clampi(c.yO, 0, ny - 1); c.zO = clampi(c.zO, 0, nz - 1);
+    c.xM = clampi(c.xM, 0, nx - 1); c.xP = clampi(c.xP, 0, nx - 1);
+    c.yM = clampi(c.yM, 0, ny - 1); c.yP = clampi(c.yP, 0, ny - 1);
+    c.zM = clampi(c.zM, 0, nz - 1); c.zP = clampi(c.zP, 0, nz - 1);
+    return c;
+}
+
+// The 12 flux pairs a particle pass loads: x faces of voxel xO at rows
+// (yM|yP, zM|zP), and y and z faces of columns (xM, xM+1) at their rows.
+struct TraceFlux {
+    FluxPair x[4], y[4], z[4];
+};
+
+__device__ __forceinline__ uint32_t cell_dy(const TraceCell& c) { return static_cast<uint32_t>(c.yP - c.yM); }
+__device__ __forceinline__ uint32_t cell_dz(const TraceCell& c) { return static_cast<uint32_t>(c.zP - c.zM); }
+
+// Byte offsets of the first face of each y and z pair (the x pairs are
+// derived in trace_fetch).  Face indices (decmesh3d.cpp:17-30): Z faces, then
+// Y, then X, each block z-major.  Strides stay below 2^24 and indices below
+// 2^30, so the products run on the 24-bit multiplier and byte offsets from
+// the uniform base fit 32 bits (4 F < 2^32); neighbouring faces are reached
+// by adding strides (the M/P pairs differ by 0, 1 or 2 cells).
+__device__ __forceinline__ void yz_offsets(const TraceParams& P, const TraceCell& c, uint32_t iy[4], uint32_t iz[4])
+{
+    const uint32_t sZy = P.nx, sZz = P.nx * P.ny;
+    const uint32_t sYy = P.nx, sYz = (P.ny + 1) * P.nx;
+    iy[0] = (__umul24(c.zM, sYz) + __umul24(c.yO, sYy) + c.xM + P.nZF) << 2;
+    iy[1] = iy[0] + (sYy << 2);
+    iy[2] = iy[0] + (__umul24(cell_dz(c), sYz) << 2);
+    iy[3] = iy[2] + (sYy << 2);
+    iz[0] = (__umul24(c.zO, sZz) + __umul24(c.yM, sZy) + c.xM) << 2;
+    iz[1] = iz[0] + (__umul24(cell_dy(c), sZy) << 2);
+    iz[2] = iz[0] + (sZz << 2);
+    iz[3] = iz[1] + (sZz << 2);
+}
+
+// The flux pairs of cell c in field u.  Faces of neighbouring x are adjacent
+// in every block of the layout, so each (x, x+1) couple comes in one load:
+// the x faces of voxel xO, and the y and z faces of columns xM and xP
+// (xP = xM + 1 but at clamps).  Only loads are issued here; the values are
+// first used in trace_advance.
+__device__ __forceinline__ TraceFlux trace_fetch(const TraceParams& P, const gptr<const float> u, const TraceCell& c)
+{
+    const uint32_t sXy = P.nx + 1, sXz = P.ny * (P.nx + 1);
+    const gptr<const char> ub = reinterpret_cast<gptr<const char>>(u);
+    auto pair = [&](uint32_t b) { return *reinterpret_cast<gptr<const FluxPair>>(ub + b); };
+    TraceFlux f;
+    const uint32_t bx = (__umul24(c.zM, sXz) + __umul24(c.yM, sXy) + c.xO + P.nZF + P.nYF) << 2;
+    const uint32_t dyx = __umul24(cell_dy(c), sXy) << 2, dzx = __umul24(cell_dz(c), sXz) << 2;
+    f.x[0] = pair(bx); f.x[1] = pair(bx + dyx); f.x[2] = pair(bx + dzx); f.x[3] = pair(bx + dyx + dzx);
+    uint32_t iy[4], iz[4];
+    yz_offsets(P, c, iy, iz);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        f.y[k] = pair(iy[k]);
+        f.z[k] = pair(iz[k]);
+    }
+    return f;
+}
+
+// One step of a live particle (psfsolver.cpp:102-215) from its cell and flux
+// pairs (u: the field the pairs came from, for the rare patch below)
+__device__ __forceinline__ float4 trace_advance(const TraceParams& P, const gptr<const float> u, float4 p,
+                                                const TraceCell& c, const TraceFlux& f)
+{
+#pragma clang fp contract(off)
+    // the x+ neighbour from each y and z pair: dx is 1 (or 0 at a clamp); a
+    // dx of any other value is patched with its own loads
+    const int dx = c.xP - c.xM;
+    auto second = [&](const FluxPair& q) { return dx == 1 ? q.y : q.x; };
+    float r3y = second(f.y[0]), r4y = second(f.y[1]), r7y = second(f.y[2]), r8y = second(f.y[3]);
+    float r3z = second(f.z[0]), r4z = second(f.z[1]), r7z = second(f.z[2]), r8z = second(f.z[3]);
+    if (__builtin_expect(dx != 0 && dx != 1, 0)) {
+        const gptr<const char> ub = reinterpret_cast<gptr<const char>>(u);
+        auto one = [&](uint32_t b) { return *reinterpret_cast<gptr<const float>>(ub + b); };
+        uint32_t iy[4], iz[4];
+        yz_offsets(P, c, iy, iz);
+        const uint32_t db = static_cast<uint32_t>(dx) << 2;
+        r3y = one(iy[0] + db); r4y = one(iy[1] + db); r7y = one(iy[2] + db); r8y = one(iy[3] + db);
+        r3z = one(iz[0] + db); r4z = one(iz[1] + db); r7z = one(iz[2] + db); r8z = one(iz[3] + db);
+    }
+    const float s0 = face_sign(0, 0, 0, c.zO), s1 = face_sign(1, 0, 0, c.zO);
+    const float s2 = face_sign(2, 0, c.yO, 0), s3 = face_sign(3, 0, c.yO, 0);
+    const float s4 = face_sign(4, c.xO, 0, 0), s5 = face_sign(5, c.xO, 0, 0);
+    // Face centres: min + h * i, plus h/2 on the axes the face spans.  On the
+    // face's own axis the CPU adds 0.5f * 0.0f; min + h * i is never -0
+    // (h > 0, i >= 0, and -0 + +0 = +0) and finite for a valid AABB, so adding
+    // +0 leaves it unchanged and is skipped.
+    const float hf = P.hf;
+    const float cx0 = P.aabb_min[0] + hf * static_cast<float>(c.xO);
+    const float cx1 = (P.aabb_min[1] + hf * static_cast<float>(c.yM)) + 0.5f * hf;
+    const float cx2 = (P.aabb_min[2] + hf * static_cast<float>(c.zM)) + 0.5f * hf;
+    const float cy0 = (P.aabb_min[0] + hf * static_cast<float>(c.xM)) + 0.5f * hf;
+    const float cy1 = P.aabb_min[1] + hf * static_cast<float>(c.yO);
+    const float cy2 = (P.aabb_min[2] + hf * static_cast<float>(c.zM)) + 0.5f * hf;
+    const float cz0 = (P.aabb_min[0] + hf * static_cast<float>(c.xM)) + 0.5f * hf;
+    const float cz1 = (P.aabb_min[1] + hf * static_cast<float>(c.yM)) + 0.5f * hf;
+    const float cz2 = P.aabb_min[2] + hf * static_cast<float>(c.zO);
+    const float inv = P.inv_h;
+    const float nx0 = inv * (p.x - cx0), nx1 = inv * (p.y - cx1), nx2 = inv * (p.z - cx2);
+    const float ny0 = inv * (p.x - cy0), ny1 = inv * (p.y - cy1), ny2 = inv * (p.z - cy2);
+    const float nz0 = inv * (p.x - cz0), nz1 = inv * (p.y - cz1), nz2 = inv * (p.z - cz2);
+    // The CPU path mixes the signed face velocities vel_k = s * u_k
+    // (psfsolver.cpp:160-206): first pairwise along one axis, then the pairs,
+    // then the halves.  In the first level glm::mix(s a, s' b, t) =
+    // (s a)(1 - t) + (s' b) t, and (s a) w = a (s w) exactly for s = +-1
+    // (round to nearest commutes with negation), so the signs ride on the
+    // four weights instead of the eight fluxes.
+    float vx, vy, vz;
+    {
+        // vel1,2,5,6 = -s4 * (left faces), vel3,4,7,8 = s5 * (right faces)
+        const float w = 1.0f - nx1, la = -s4 * w, lb = -s4 * nx1, ra = s5 * w, rb = s5 * nx1;
+        const float a0 = f.x[0].x * la + f.x[1].x * lb, a1 = f.x[0].y * ra + f.x[1].y * rb;
+        const float a2 = f.x[2].x * la + f.x[3].x * lb, a3 = f.x[2].y * ra + f.x[3].y * rb;
+        vx = mixf(mixf(a0, a1, nx0), mixf(a2, a3, nx0), nx2);
+    }
+    {
+        // odd vel = -s2 * (bottom faces), even vel = s3 * (top faces)
+        const float wa = -s2 * (1.0f - ny1), wb = s3 * ny1;
+        const float a0 = f.y[0].x * wa + f.y[1].x * wb, a1 = r3y * wa + r4y * wb;
+        const float a2 = f.y[2].x * wa + f.y[3].x * wb, a3 = r7y * wa + r8y * wb;
+        vy = mixf(mixf(a0, a1, ny0), mixf(a2, a3, ny0), ny2);
+    }
+    {
+        // vel1..4 = s0 * (front faces), vel5..8 = -s1 * (back faces)
+        const float w = 1.0f - nz1, fa = s0 * w, fb = s0 * nz1, ba = -s1 * w, bb = -s1 * nz1;
+        const float a0 = f.z[0].x * fa + f.z[1].x * fb, a1 = r3z * fa + r4z * fb;
+        const float a2 = f.z[2].x * ba + f.z[3].x * bb, a3 = r7z * ba + r8z * bb;
+        vz = mixf(mixf(a0, a1, nz0), mixf(a2, a3, nz0), nz2);
+    }
+    // glm::clamp(v, lo, hi) = min(max(v, lo), hi)
+    auto clampf = [](float v, float lo, float hi) {
+        const float t = (v < lo) ? lo : v;
+        return (hi < t) ? hi : t;
+    };
+    p.x = clampf(p.x + P.dtf * vx, P.aabb_min[0], P.aabb_max[0]);
+    p.y = clampf(p.y + P.dtf * vy, P.aabb_min[1], P.aabb_max[1]);
+    p.z = clampf(p.z + P.dtf * vz, P.aabb_min[2], P.aabb_max[2]);
+    p.w = p.w - 1.0f;
+    return p;
+}
+
+// respawn of an expired particle (psfsolver.cpp:219-222, with the build's
+// counter-based draws); the draws follow the caller's particle index
+__device__ __forceinline__ float4 trace_respawn(const TraceParams& P, const gptr<const uint32_t> pid, uint32_t id)
+{
+    const uint32_t who = pid[id];
+    const uint32_t r0 = respawn_draw(P.seed, P.frame, who, 0);
+    const uint32_t r1 = respawn_draw(P.seed, P.frame, who, 1);
+    const uint32_t r2 = respawn_draw(P.seed, P.frame, who, 2);
+    float4 p;
+    p.x = static_cast<float>((r0 / 1024.0 - 0.5) * 0.25);
+    p.y = P.aabb_min[1] + 0.2f;
+    p.z = static_cast<float>(-static_cast<double>(P.center_z) + (r1 / 1024.0 - 0.5) * 0.25);
+    p.w = static_cast<float>(P.lifetime * (r2 / 1024.0));
+    return p;
+}
+
 // one particle record through one frame; id is its storage slot (cells,
-// caller index).  Pow2: the voxel size is a power of two (cell_indices_pow2).
-// u is addressed by 32-bit byte offsets from a uniform base (4 F < 2^32).
+// caller index)
 template <bool Pow2>
 __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<const float> u, float4 p,
                                             const gptr<const uint32_t> pid, uint32_t id, int32_t* __restrict__ cells)
 {
-#pragma clang fp contract(off)
-    const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny), nz = static_cast<int>(P.nz);
     if (p.w > 0.0f) {
-        const float sx = (P.aabb_min[0] + P.ext[0]) + p.x;
-        const float sy = (P.aabb_min[1] + P.ext[1]) + p.y;
-        const float sz = (P.aabb_min[2] + P.ext[2]) + p.z;
-        int xO, xM, xP, yO, yM, yP, zO, zM, zP;
-        if constexpr (Pow2) {
-            cell_indices_pow2(sx, P.inv_h, xO, xM, xP);
-            cell_indices_pow2(sy, P.inv_h, yO, yM, yP);
-            cell_indices_pow2(sz, P.inv_h, zO, zM, zP);
-        } else {
-            const double res = P.res, hres = P.res / 2, rr = P.inv_res;
-            cell_indices(static_cast<double>(sx), res, hres, rr, xO, xM, xP);
-            cell_indices(static_cast<double>(sy), res, hres, rr, yO, yM, yP);
-            cell_indices(static_cast<double>(sz), res, hres, rr, zO, zM, zP);
-        }
-        xO = clampi(xO, 0, nx - 1); yO = clampi(yO, 0, ny - 1); zO = clampi(zO, 0, nz - 1);
-        xM = clampi(xM, 0, nx - 1); xP = clampi(xP, 0, nx - 1);
-        yM = clampi(yM, 0, ny - 1); yP = clampi(yP, 0, ny - 1);
-        zM = clampi(zM, 0, nz - 1); zP = clampi(zP, 0, nz - 1);
+        const TraceCell c = trace_cell<Pow2>(P, p);
         if (cells) {
-            int32_t* c = cells + 12ull * id;
-            c[0] = xO; c[1] = yO; c[2] = zO; c[3] = xM; c[4] = xP; c[5] = yM; c[6] = yP; c[7] = zM; c[8] = zP;
-            c[9] = 0; c[10] = 0; c[11] = 0;
+            int32_t* o = cells + 12ull * id;
+            o[0] = c.xO; o[1] = c.yO; o[2] = c.zO; o[3] = c.xM; o[4] = c.xP; o[5] = c.yM; o[6] = c.yP;
+            o[7] = c.zM; o[8] = c.zP; o[9] = 0; o[10] = 0; o[11] = 0;
         }
-        // Face indices (decmesh3d.cpp:17-30): Z faces, then Y, then X, each
-        // block z-major.  Strides stay below 2^24 and indices below 2^30, so
-        // the products run on the 24-bit multiplier and byte offsets fit 32
-        // bits; neighbouring faces are reached by adding strides (the M/P
-        // pairs differ by 0, 1 or 2 cells).
-        const uint32_t sZy = P.nx, sZz = P.nx * P.ny;
-        const uint32_t sYy = P.nx, sYz = (P.ny + 1) * P.nx;
-        const uint32_t sXy = P.nx + 1, sXz = P.ny * (P.nx + 1);
-        const uint32_t dy = static_cast<uint32_t>(yP - yM), dz = static_cast<uint32_t>(zP - zM);
-        // Faces of neighbouring x are adjacent in every block of the layout, so
-        // each (x, x+1) couple comes in one load: the x faces of voxel xO, and
-        // the y and z faces of columns xM and xP (xP = xM + 1 but at clamps).
-        const gptr<const char> ub = reinterpret_cast<gptr<const char>>(u);
-        auto pair = [&](uint32_t b) { return *reinterpret_cast<gptr<const FluxPair>>(ub + b); };
-        auto one = [&](uint32_t b) { return *reinterpret_cast<gptr<const float>>(ub + b); };
-        const int dx = xP - xM;
-        // the x+ neighbour from the pair: dx is 1 (or 0 at a clamp); a dx of
-        // any other value is patched once, below, with its own loads
-        auto second = [&](const FluxPair& q, uint32_t) { return dx == 1 ? q.y : q.x; };
-        // x component: voxels (xO, yM|yP, zM|zP), faces f5 (left) / f6 (right)
-        const float s4 = face_sign(4, xO, 0, 0), s5 = face_sign(5, xO, 0, 0);
-        const uint32_t bx = (__umul24(zM, sXz) + __umul24(yM, sXy) + xO + P.nZF + P.nYF) << 2;
-        const uint32_t dyx = __umul24(dy, sXy) << 2, dzx = __umul24(dz, sXz) << 2;
-        const FluxPair x1 = pair(bx), x2 = pair(bx + dyx);
-        const FluxPair x3 = pair(bx + dzx), x4 = pair(bx + dyx + dzx);
-        // y component: voxels (xM|xP, yO, zM|zP), faces f3 (bottom) / f4 (top)
-        const float s2 = face_sign(2, 0, yO, 0), s3 = face_sign(3, 0, yO, 0);
-        const uint32_t iy1 = (__umul24(zM, sYz) + __umul24(yO, sYy) + xM + P.nZF) << 2;
-        const uint32_t iy2 = iy1 + (sYy << 2), iy3 = iy1 + (__umul24(dz, sYz) << 2), iy4 = iy3 + (sYy << 2);
-        const FluxPair y1 = pair(iy1), y2 = pair(iy2), y3 = pair(iy3), y4 = pair(iy4);
-        float r3y = second(y1, iy1), r4y = second(y2, iy2);
-        float r7y = second(y3, iy3), r8y = second(y4, iy4);
-        // z component: voxels (xM|xP, yM|yP, zO), faces f1 (front) / f2 (back)
-        const float s0 = face_sign(0, 0, 0, zO), s1 = face_sign(1, 0, 0, zO);
-        const uint32_t iz1 = (__umul24(zO, sZz) + __umul24(yM, sZy) + xM) << 2;
-        const uint32_t iz2 = iz1 + (__umul24(dy, sZy) << 2), iz3 = iz1 + (sZz << 2), iz4 = iz2 + (sZz << 2);
-        const FluxPair z1 = pair(iz1), z2 = pair(iz2), z3 = pair(iz3), z4 = pair(iz4);
-        float r3z = second(z1, iz1), r4z = second(z2, iz2);
-        float r7z = second(z3, iz3), r8z = second(z4, iz4);
-        if (__builtin_expect(dx != 0 && dx != 1, 0)) {
-            const uint32_t db = static_cast<uint32_t>(dx) << 2;
-            r3y = one(iy1 + db); r4y = one(iy2 + db);
-            r7y = one(iy3 + db); r8y = one(iy4 + db);
-            r3z = one(iz1 + db); r4z = one(iz2 + db);
-            r7z = one(iz3 + db); r8z = one(iz4 + db);
-        }
-
-        // Face centres: min + h * i, plus h/2 on the axes the face spans.  On
-        // the face's own axis the CPU adds 0.5f * 0.0f; min + h * i is never
-        // -0 (h > 0, i >= 0, and -0 + +0 = +0) and finite for a valid AABB,
-        // so adding +0 leaves it unchanged and is skipped.
-        const float hf = P.hf;
-        float cx0 = P.aabb_min[0] + hf * static_cast<float>(xO);
-        float cx1 = (P.aabb_min[1] + hf * static_cast<float>(yM)) + 0.5f * hf;
-        float cx2 = (P.aabb_min[2] + hf * static_cast<float>(zM)) + 0.5f * hf;
-        float cy0 = (P.aabb_min[0] + hf * static_cast<float>(xM)) + 0.5f * hf;
-        float cy1 = P.aabb_min[1] + hf * static_cast<float>(yO);
-        float cy2 = (P.aabb_min[2] + hf * static_cast<float>(zM)) + 0.5f * hf;
-        float cz0 = (P.aabb_min[0] + hf * static_cast<float>(xM)) + 0.5f * hf;
-        float cz1 = (P.aabb_min[1] + hf * static_cast<float>(yM)) + 0.5f * hf;
-        float cz2 = P.aabb_min[2] + hf * static_cast<float>(zO);
-        const float inv = P.inv_h;
-        const float nx0 = inv * (p.x - cx0), nx1 = inv * (p.y - cx1), nx2 = inv * (p.z - cx2);
-        const float ny0 = inv * (p.x - cy0), ny1 = inv * (p.y - cy1), ny2 = inv * (p.z - cy2);
-        const float nz0 = inv * (p.x - cz0), nz1 = inv * (p.y - cz1), nz2 = inv * (p.z - cz2);
-        // The CPU path mixes the signed face velocities vel_k = s * u_k
-        // (psfsolver.cpp:160-206): first pairwise along one axis, then the
-        // pairs, then the halves.  In the first level glm::mix(s a, s' b, t)
-        // = (s a)(1 - t) + (s' b) t, and (s a) w = a (s w) exactly for s = +-1
-        // (round to nearest commutes with negation), so the signs ride on the
-        // four weights instead of the eight fluxes.
-        float vx, vy, vz;
-        {
-            // vel1,2,5,6 = -s4 * (left faces), vel3,4,7,8 = s5 * (right faces)
-            const float w = 1.0f - nx1, la = -s4 * w, lb = -s4 * nx1, ra = s5 * w, rb = s5 * nx1;
-            const float a0 = x1.x * la + x2.x * lb, a1 = x1.y * ra + x2.y * rb;
-            const float a2 = x3.x * la + x4.x * lb, a3 = x3.y * ra + x4.y * rb;
-            vx = mixf(mixf(a0, a1, nx0), mixf(a2, a3, nx0), nx2);
-        }
-        {
-            // odd vel = -s2 * (bottom faces), even vel = s3 * (top faces)
-            const float wa = -s2 * (1.0f - ny1), wb = s3 * ny1;
-            const float a0 = y1.x * wa + y2.x * wb, a1 = r3y * wa + r4y * wb;
-            const float a2 = y3.x * wa + y4.x * wb, a3 = r7y * wa + r8y * wb;
-            vy = mixf(mixf(a0, a1, ny0), mixf(a2, a3, ny0), ny2);
-        }
-        {
-            // vel1..4 = s0 * (front faces), vel5..8 = -s1 * (back faces)
-            const float w = 1.0f - nz1, fa = s0 * w, fb = s0 * nz1, ba = -s1 * w, bb = -s1 * nz1;
-            const float a0 = z1.x * fa + z2.x * fb, a1 = r3z * fa + r4z * fb;
-            const float a2 = z3.x * ba + z4.x * bb, a3 = r7z * ba + r8z * bb;
-            vz = mixf(mixf(a0, a1, nz0), mixf(a2, a3, nz0), nz2);
-        }
-        // glm::clamp(v, lo, hi) = min(max(v, lo), hi)
-        auto clampf = [](float v, float lo, float hi) {
-            const float t = (v < lo) ? lo : v;
-            return (hi < t) ? hi : t;
-        };
-        p.x = clampf(p.x + P.dtf * vx, P.aabb_min[0], P.aabb_max[0]);
-        p.y = clampf(p.y + P.dtf * vy, P.aabb_min[1], P.aabb_max[1]);
-        p.z = clampf(p.z + P.dtf * vz, P.aabb_min[2], P.aabb_max[2]);
-        p.w = p.w - 1.0f;
-    } else {
-        if (cells) {
-            int32_t* c = cells + 12ull * id;
-#pragma unroll
-            for (int k = 0; k < 12; ++k) c[k] = -1;
-        }
-        const uint32_t who = pid[id];   // draws follow the caller's particle index
-        const uint32_t r0 = respawn_draw(P.seed, P.frame, who, 0);
-        const uint32_t r1 = respawn_draw(P.seed, P.frame, who, 1);
-        const uint32_t r2 = respawn_draw(P.seed, P.frame, who, 2);
-        p.x = static_cast<float>((r0 / 1024.0 - 0.5) * 0.25);
-        p.y = P.aabb_min[1] + 0.2f;
-        p.z = static_cast<float>(-static_cast<double>(P.center_z) + (r1 / 1024.0 - 0.5) * 0.25);
-        p.w = static_cast<float>(P.lifetime * (r2 / 1024.0));
+        return trace_advance(P, u, p, c, trace_fetch(P, u, c));
     }
-    return p;
+    if (cells) {
+        int32_t* o = cells + 12ull * id;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) o[k] = -1;
+    }
+    return trace_respawn(P, pid, id);
 }
 
 // One counter increment per particle, with equal cells of neighbouring lanes
@@ -719,10 +773,11 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
 // respawn frame P.frame + t, and stores it once (the same trace_one per
 // frame, so the bits of nframes single passes).  With D.counts set the
 // positions after the last frame are counted.
-// 8 waves per SIMD (64 VGPRs; three 64-bit values are spilled to scratch and
-// reloaded only in the respawn branch and before the final store): alone it
-// is slightly slower than at 70 VGPRs, but the windowed frame loop, where
-// the next window's reconstruction and steps share the CUs, runs 5% faster.
+// 8 waves per SIMD (at most 64 VGPRs; both instances fit without spilling):
+// the windowed frame loop, where the next window's reconstruction and steps
+// share the CUs, ran 5% faster than at 70 VGPRs.  Loading the next frame's
+// fluxes one frame early (with the current cell) measured 8% slower: it
+// needs 96 VGPRs (5 waves per SIMD).
 template <bool Pow2>
 __global__ __launch_bounds__(256, 8) void k_trace_frames(TraceParams P, const float* __restrict__ u0, size_t ldU,
                                                       uint32_t nframes, float4* __restrict__ parts,
