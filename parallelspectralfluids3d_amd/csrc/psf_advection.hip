@@ -171,6 +171,8 @@ int psf_scene_build_advection(psf_scene* s, const double* Phi, const double* V, 
     }
     ADV_HIPCHK(ctx, hipSetDevice(ctx->device));
     if (int rc = scene_window_flush(s)) return rc;
+    // an FP64 C is written in place: until the build completes, the scene has none
+    s->has_C = false;
     const hipStream_t st = ctx->stream;
     DevMem mem;
     uint32_t *dptr, *dta, *dtb;
