@@ -10,8 +10,8 @@
 //   - k_wedge_rows writes W_i / lambda_i row by row for a chunk of modes i,
 //     one wave per (edge, i), the K columns j across the lanes (rows of Phi
 //     are read contiguously from a row-major copy);
-//   - one strided-batched DGEMM per chunk gives C_i = V^T (W_i / lambda_i)
-//     on the FP64 matrix cores;
+//   - a split-reduction DGEMM (psf_blas.hpp) per mode gives
+//     C_i = V^T (W_i / lambda_i) on the FP64 matrix cores;
 //   - C (FP64) is then stored in the scene's precision like an upload.
 // Memory: Phi and V once, plus one chunk of wedge matrices sized to a few
 // GB, instead of the reference's 2 K E K doubles.
@@ -24,6 +24,7 @@
 
 #include <rocblas/rocblas.h>
 
+#include "psf_blas.hpp"
 #include "psf_internal.hpp"
 #include "psf_kernels.hpp"
 #include "psf_persistent.hpp"
@@ -176,7 +177,7 @@ int psf_scene_build_advection(psf_scene* s, const double* Phi, const double* V, 
     const hipStream_t st = ctx->stream;
     DevMem mem;
     uint32_t *dptr, *dta, *dtb;
-    double *dtc, *dPhi, *dPhiR, *dV, *dlam, *dW, *Cf;
+    double *dtc, *dPhi, *dPhiR, *dV, *dlam, *dW, *Cf, *dsplit;
     ADV_HIPCHK(ctx, mem.alloc(&dptr, ptr.size()));
     ADV_HIPCHK(ctx, mem.alloc(&dta, nterms));
     ADV_HIPCHK(ctx, mem.alloc(&dtb, nterms));
@@ -188,6 +189,7 @@ int psf_scene_build_advection(psf_scene* s, const double* Phi, const double* V, 
     const size_t row_bytes = size_t(nE) * K * sizeof(double);   // one wedge matrix
     const uint32_t ic = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(K, kChunkBytes / row_bytes)));
     ADV_HIPCHK(ctx, mem.alloc(&dW, size_t(ic) * nE * K));
+    ADV_HIPCHK(ctx, mem.alloc(&dsplit, splitk_work_doubles(static_cast<int>(K), static_cast<int>(K), nE)));
     const size_t n = size_t(K) * K * K;
     if (prec == PSF_C_FP64) {
         Cf = static_cast<double*>(s->dev.C);
@@ -206,20 +208,18 @@ int psf_scene_build_advection(psf_scene* s, const double* Phi, const double* V, 
     Blas blas;
     ADV_BLASCHK(ctx, rocblas_create_handle(&blas.h));
     ADV_BLASCHK(ctx, rocblas_set_stream(blas.h, st));
-    const double one = 1.0, zero = 0.0;
     const rocblas_int Ki = static_cast<rocblas_int>(K), Ei = static_cast<rocblas_int>(nE);
     for (uint32_t i0 = 0; i0 < K; i0 += ic) {
         const uint32_t cnt = std::min(ic, K - i0);
         k_wedge_rows<<<dim3((nE + 3) / 4, cnt), 256, 0, st>>>(nE, K, i0, dptr, dta, dtb, dtc, dPhiR, dlam, dW);
         ADV_HIPCHK(ctx, hipGetLastError());
         // C_i (column-major K x K: element (k, j) = C_i(k, j)) = V^T W_i, with
-        // V column-major E x K and W_i row-major E x K = column-major K x E
-        ADV_BLASCHK(ctx, rocblas_dgemm_strided_batched(blas.h, rocblas_operation_transpose, rocblas_operation_transpose,
-                                                       Ki, Ki, Ei, &one, dV, Ei, 0, dW, Ki,
-                                                       static_cast<rocblas_stride>(size_t(nE) * K), &zero,
-                                                       Cf + size_t(i0) * K * K, Ki,
-                                                       static_cast<rocblas_stride>(size_t(K) * K),
-                                                       static_cast<rocblas_int>(cnt)));
+        // V column-major E x K and W_i row-major E x K = column-major K x E;
+        // the output is small and the reduction long, so it is split
+        for (uint32_t q = 0; q < cnt; ++q)
+            ADV_BLASCHK(ctx, gemm_splitk(blas.h, rocblas_operation_transpose, rocblas_operation_transpose, Ki, Ki, nE,
+                                         dV, Ei, 1, dW + size_t(q) * nE * K, Ki, Ki, Cf + size_t(i0 + q) * K * K, Ki,
+                                         dsplit));
     }
     if (prec == PSF_C_FP32) {
         k_to_f32<<<static_cast<unsigned>((n + 255) / 256), 256, 0, st>>>(Cf, static_cast<float*>(s->dev.C), n);

@@ -14,8 +14,8 @@
 // A null space is not special: its vectors are the smallest eigenpairs.  The
 // caller lifts one it does not want (the DEC Laplacian's gradients) with a
 // penalty term, as the host shift-invert path does.
-// Products with A: one fused CSR kernel per polynomial degree.  Gram matrices
-// and rotations: rocBLAS DGEMM; orthonormalisation: Cholesky QR twice
+// Products with A: one fused CSR kernel per polynomial degree.  Gram matrices:
+// split-reduction DGEMM (psf_blas.hpp); rotations: rocBLAS DGEMM; orthonormalisation: Cholesky QR twice
 // (rocSOLVER POTRF, rocBLAS TRSM in place); H's eigendecomposition: rocSOLVER
 // SYEVD.  The block is stored row-major (n x m), so each nonzero of a CSR row
 // reads one contiguous row of the block; for the BLAS calls that is the
@@ -30,6 +30,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include "psf_blas.hpp"
 #include "psf_internal.hpp"
 
 namespace psf {
@@ -229,7 +230,7 @@ int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const i
     const uint32_t rparts = std::min<uint32_t>(1024, n);
     DevMem mem;
     int32_t *drp, *dci, *dinfo;
-    double *dav, *X, *Y, *P, *Q, *H, *dw, *dE, *dpart, *dsq;
+    double *dav, *X, *Y, *P, *Q, *H, *dw, *dE, *dpart, *dsq, *dsplit;
     EIG_HIPCHK(ctx, mem.alloc(&drp, size_t(n) + 1));
     EIG_HIPCHK(ctx, mem.alloc(&dci, size_t(nnz)));
     EIG_HIPCHK(ctx, mem.alloc(&dav, size_t(nnz)));
@@ -243,6 +244,7 @@ int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const i
     EIG_HIPCHK(ctx, mem.alloc(&dpart, size_t(rparts) * K));
     EIG_HIPCHK(ctx, mem.alloc(&dsq, K));
     EIG_HIPCHK(ctx, mem.alloc(&dinfo, 1));
+    EIG_HIPCHK(ctx, mem.alloc(&dsplit, splitk_work_doubles(static_cast<int>(m), static_cast<int>(m), n)));
     EIG_HIPCHK(ctx, hipMemcpy(drp, row_ptr, (size_t(n) + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
     EIG_HIPCHK(ctx, hipMemcpy(dci, col_idx, size_t(nnz) * sizeof(int32_t), hipMemcpyHostToDevice));
     EIG_HIPCHK(ctx, hipMemcpy(dav, values, size_t(nnz) * sizeof(double), hipMemcpyHostToDevice));
@@ -265,8 +267,8 @@ int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const i
     // x <- x R^-1 with x^T x = R^T R, twice (Cholesky QR2)
     auto orthonormalise = [&](double* x) -> int {
         for (int pass = 0; pass < 2; ++pass) {
-            EIG_BLASCHK(ctx, rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_transpose, mi, mi, ni,
-                                           &one, x, mi, x, mi, &zero, H, mi));
+            EIG_BLASCHK(ctx, gemm_splitk(blas.h, rocblas_operation_none, rocblas_operation_transpose, mi, mi, ni, x, mi,
+                                         mi, x, mi, mi, H, mi, dsplit));
             EIG_BLASCHK(ctx, rocsolver_dpotrf(blas.h, rocblas_fill_upper, mi, H, mi, dinfo));
             EIG_HIPCHK(ctx, hipMemcpyAsync(&info, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
             EIG_HIPCHK(ctx, hipStreamSynchronize(st));
@@ -294,8 +296,8 @@ int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const i
     for (;;) {
         // Rayleigh-Ritz: Y = A X, H = X^T Y, then X <- X V and Y <- Y V
         EIG_HIPCHK(ctx, apply(X, nullptr, 1.0, 0.0, 0.0, Y));
-        EIG_BLASCHK(ctx, rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_transpose, mi, mi, ni, &one, X,
-                                       mi, Y, mi, &zero, H, mi));
+        EIG_BLASCHK(ctx, gemm_splitk(blas.h, rocblas_operation_none, rocblas_operation_transpose, mi, mi, ni, X, mi, mi,
+                                     Y, mi, mi, H, mi, dsplit));
         k_symmetrise<<<dim3((m + 255) / 256, m), 256, 0, st>>>(m, H);
         EIG_HIPCHK(ctx, hipGetLastError());
         EIG_BLASCHK(ctx, rocsolver_dsyevd(blas.h, rocblas_evect_original, rocblas_fill_upper, mi, H, mi, dw, dE, dinfo));
