@@ -10,6 +10,8 @@
 // With "extras" it then also writes the 16^3 density volume of the final
 // particles (<dir>/volume_out.bin, uint16 half bits) and runs the reference's
 // benchmark (startBenchmark, 4 recorded frames) into <dir>/benchmark.csv.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <string>
@@ -86,10 +88,56 @@ private:
 
 }  // namespace
 
+// psf_eigensolve_csr from C++: the Dirichlet path Laplacian (tridiagonal
+// 2, -1), whose eigenpairs are 2 - 2 cos(pi k / (n + 1)) and sine vectors
+int eigen_check()
+{
+    const uint32_t n = 3000, K = 16;
+    std::vector<int32_t> rp(n + 1), ci;
+    std::vector<double> av;
+    for (uint32_t i = 0; i < n; ++i) {
+        rp[i] = static_cast<int32_t>(ci.size());
+        if (i > 0) { ci.push_back(static_cast<int32_t>(i - 1)); av.push_back(-1.0); }
+        ci.push_back(static_cast<int32_t>(i)); av.push_back(2.0);
+        if (i + 1 < n) { ci.push_back(static_cast<int32_t>(i + 1)); av.push_back(-1.0); }
+    }
+    rp[n] = static_cast<int32_t>(ci.size());
+    psf_ctx* ctx = nullptr;
+    if (psf_ctx_create(0, &ctx) != PSF_OK) {
+        std::fprintf(stderr, "psf_ctx_create failed\n");
+        return 1;
+    }
+    std::vector<double> lam(K), X(size_t(n) * K);
+    uint32_t products = 0;
+    const int rc = psf_eigensolve_csr(ctx, n, rp.data(), ci.data(), av.data(), K, 16, 30, 1e-12, 20000, 7, lam.data(),
+                                      X.data(), &products);
+    if (rc != PSF_OK) {
+        std::fprintf(stderr, "psf_eigensolve_csr: %d %s\n", rc, psf_last_error(ctx));
+        psf_ctx_destroy(ctx);
+        return 1;
+    }
+    double err = 0.0, vec = 0.0;
+    const double pi = 3.14159265358979323846;
+    for (uint32_t k = 0; k < K; ++k) {
+        err = std::max(err, std::fabs(lam[k] - (2.0 - 2.0 * std::cos(pi * (k + 1) / (n + 1)))));
+        double dot = 0.0, nrm = 0.0;
+        for (uint32_t j = 0; j < n; ++j) {
+            const double v = std::sin(pi * (k + 1) * (j + 1) / (n + 1));
+            dot += v * X[size_t(k) * n + j];
+            nrm += v * v;
+        }
+        vec = std::max(vec, std::fabs(std::fabs(dot) / std::sqrt(nrm) - 1.0));
+    }
+    psf_ctx_destroy(ctx);
+    std::printf("eigen_check: %u products, max eigenvalue error %.3e, max |cos| - 1 %.3e\n", products, err, vec);
+    return (err < 1e-12 && vec < 1e-9) ? 0 : 1;
+}
+
 int main(int argc, char** argv)
 {
+    if (argc == 2 && std::string(argv[1]) == "eigen") return eigen_check();
     if (argc != 2 && argc != 3) {
-        std::fprintf(stderr, "usage: %s <dir> [extras]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <dir> [extras] | eigen\n", argv[0]);
         return 2;
     }
     const bool extras = argc == 3 && std::string(argv[2]) == "extras";

@@ -77,3 +77,12 @@ def test_hip_solver_density_and_benchmark(tmp_path):
     assert all(len(t.split(",")) == 2 and min(float(x) for x in t.split(",")) > 0.0 for t in times)
     avg = [float(x) for x in rows[-1].split(",")]
     assert avg[0] > 0.0 and avg[1] > 0.0
+
+
+def test_eigensolve_from_cpp():
+    """psf_eigensolve_csr called from C++ (no Python in between) on the
+    Dirichlet path Laplacian: eigenvalues to 1e-12, eigenvectors the sines."""
+    print("[gpu-case] solver_check eigen", flush=True)
+    r = subprocess.run([EXE, "eigen"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "eigen_check:" in r.stdout
