@@ -60,30 +60,63 @@ int set_err(psf_ctx* ctx, int code, const char* fmt, ...)
             return set_err((ctx), PSF_ERR_HIP, "%s: rocBLAS status %d", #expr, static_cast<int>(s_));         \
     } while (0)
 
-// out = alpha * (A Y) + beta * Y + gamma * Z (Z may be null): one wave per
-// row; lane l holds columns l, l + 64, ..., l + 64 (R - 1) (R = ceil(m / 64)),
-// so each nonzero issues R independent loads of one block row
+// Launch blocks go to the 8 XCDs round-robin (block b to XCD b % 8), each
+// with its own L2: logical block (b % 8) * per + b / 8 gives every XCD a
+// contiguous range of rows, whose neighbours (rows in spatial order) then
+// stay in that XCD's L2.
+constexpr uint32_t kXcds = 8;
+
+// out = alpha * (A Y) + beta * Y + gamma * Z (Z may be null) on the m columns
+// starting at each pointer, rows ld apart: one wave per row; lane l holds
+// columns l, l + 64, ..., l + 64 (R - 1) (R = ceil(m / 64)), so each nonzero
+// issues R independent loads of one block row.  (Running the Chebyshev
+// recurrence on one 64-column slice at a time, R = 1, was 30% slower per
+// product than the whole block at R = 7 on the 64^3 obstacle scene.)
 template <int R>
-__global__ __launch_bounds__(256) void k_csr_block(uint32_t n, uint32_t m, const int32_t* __restrict__ rp,
-                                                   const int32_t* __restrict__ ci, const double* __restrict__ av,
-                                                   const double* __restrict__ Y, const double* __restrict__ Z,
-                                                   double alpha, double beta, double gamma, double* __restrict__ out)
+__global__ __launch_bounds__(256) void k_csr_block(uint32_t n, uint32_t m, uint32_t ld, uint32_t per,
+                                                   const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                   const double* __restrict__ av, const double* __restrict__ Y,
+                                                   const double* __restrict__ Z, double alpha, double beta,
+                                                   double gamma, double* __restrict__ out)
 {
-    const uint32_t row = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6));
+    const uint32_t blk = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    const uint32_t row = __builtin_amdgcn_readfirstlane(blk * 4u + (threadIdx.x >> 6));
     if (row >= n) return;
     const uint32_t lane = threadIdx.x & 63u;
     const int32_t b = rp[row], e = rp[row + 1];
     double s[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) s[r] = 0.0;
-    for (int32_t q = b; q < e; ++q) {
+    // four nonzeros' loads are issued before their products are summed (in
+    // the same order as one at a time)
+    int32_t q = b;
+    for (; q + 4 <= e; q += 4) {
+        double a[4];
+        const double* y[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            a[j] = av[q + j];
+            y[j] = Y + static_cast<size_t>(ci[q + j]) * ld + lane;
+        }
+        double v[4][R];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[j][r] = lane + 64u * r < m ? y[j][64 * r] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (lane + 64u * r < m) s[r] += a[j] * v[j][r];
+    }
+    for (; q < e; ++q) {
         const double a = av[q];
-        const double* y = Y + static_cast<size_t>(ci[q]) * m + lane;
+        const double* y = Y + static_cast<size_t>(ci[q]) * ld + lane;
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (lane + 64u * r < m) s[r] += a * y[64 * r];
     }
-    const size_t o = static_cast<size_t>(row) * m + lane;
+    const size_t o = static_cast<size_t>(row) * ld + lane;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (lane + 64u * r >= m) break;
@@ -94,13 +127,14 @@ __global__ __launch_bounds__(256) void k_csr_block(uint32_t n, uint32_t m, const
 }
 
 template <int R>
-void launch_csr(unsigned grid, hipStream_t st, uint32_t n, uint32_t m, const int32_t* rp, const int32_t* ci,
+void launch_csr(hipStream_t st, uint32_t n, uint32_t m, uint32_t ld, const int32_t* rp, const int32_t* ci,
                 const double* av, const double* Y, const double* Z, double a, double b, double g, double* out)
 {
-    k_csr_block<R><<<grid, 256, 0, st>>>(n, m, rp, ci, av, Y, Z, a, b, g, out);
+    const uint32_t per = ((n + 3) / 4 + kXcds - 1) / kXcds;
+    k_csr_block<R><<<per * kXcds, 256, 0, st>>>(n, m, ld, per, rp, ci, av, Y, Z, a, b, g, out);
 }
 
-using CsrLaunch = void (*)(unsigned, hipStream_t, uint32_t, uint32_t, const int32_t*, const int32_t*, const double*,
+using CsrLaunch = void (*)(hipStream_t, uint32_t, uint32_t, uint32_t, const int32_t*, const int32_t*, const double*,
                            const double*, const double*, double, double, double, double*);
 constexpr CsrLaunch kCsrLaunch[16] = {launch_csr<1>,  launch_csr<2>,  launch_csr<3>,  launch_csr<4>,
                                       launch_csr<5>,  launch_csr<6>,  launch_csr<7>,  launch_csr<8>,
@@ -256,10 +290,9 @@ int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const i
     const rocblas_int mi = static_cast<rocblas_int>(m), ni = static_cast<rocblas_int>(n);
     const double one = 1.0, zero = 0.0;
     uint32_t products = 0;
-    const unsigned grid = (n + 3) / 4;
     const CsrLaunch csr = kCsrLaunch[(m + 63) / 64 - 1];
     auto apply = [&](const double* in, const double* z, double a, double b, double g, double* out) {
-        csr(grid, st, n, m, drp, dci, dav, in, z, a, b, g, out);
+        csr(st, n, m, m, drp, dci, dav, in, z, a, b, g, out);
         ++products;
         return hipGetLastError();
     };
