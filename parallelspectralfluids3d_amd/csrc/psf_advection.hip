@@ -8,7 +8,8 @@
 // (edge, a, b, s_h c(a, b)) and V as the reference computes them; here:
 //   - the terms are grouped by edge on the host (a CSR over edges);
 //   - k_wedge_rows writes W_i / lambda_i row by row for a chunk of modes i,
-//     one wave per (edge, i), the K columns j across the lanes (rows of Phi
+//     one wave per (edge, i), the K columns j across the lanes, four per
+//     lane (rows of Phi
 //     are read contiguously from a row-major copy);
 //   - a split-reduction DGEMM (psf_blas.hpp) per mode gives
 //     C_i = V^T (W_i / lambda_i) on the FP64 matrix cores;
@@ -83,14 +84,26 @@ __global__ __launch_bounds__(256) void k_wedge_rows(uint32_t E, uint32_t K, uint
     const uint32_t t0 = ptr[e], t1 = ptr[e + 1];
     const double inv = 1.0 / lambda[i];
     double* out = W + (static_cast<size_t>(blockIdx.y) * E + e) * K;
-    for (uint32_t j = lane; j < K; j += 64u) {
-        double w = 0.0;
+    // 256 columns per pass, four per lane: a term's Phi(a, i), Phi(b, i) and
+    // coefficient are loaded once for all four, and its eight row loads are
+    // in flight together
+    for (uint32_t j0 = 0; j0 < K; j0 += 256u) {
+        double w[4] = {0.0, 0.0, 0.0, 0.0};
         for (uint32_t t = t0; t < t1; ++t) {
             const double* pa = PhiR + static_cast<size_t>(ta[t]) * K;
             const double* pb = PhiR + static_cast<size_t>(tb[t]) * K;
-            w += tc[t] * (pa[i] * pb[j] - pa[j] * pb[i]);
+            const double c = tc[t], ai = pa[i], bi = pb[i];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t j = j0 + lane + 64u * r;
+                if (j < K) w[r] += c * (ai * pb[j] - pa[j] * bi);
+            }
         }
-        out[j] = w * inv;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t j = j0 + lane + 64u * r;
+            if (j < K) out[j] = w[r] * inv;
+        }
     }
 }
 
