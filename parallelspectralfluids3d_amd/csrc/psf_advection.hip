@@ -9,8 +9,7 @@
 //   - the terms are grouped by edge on the host (a CSR over edges);
 //   - k_wedge_rows writes W_i / lambda_i row by row for a chunk of modes i,
 //     one wave per (edge, i), the K columns j across the lanes, four per
-//     lane (rows of Phi
-//     are read contiguously from a row-major copy);
+//     lane (rows of Phi are read contiguously from a row-major copy);
 //   - a split-reduction DGEMM (psf_blas.hpp) per mode gives
 //     C_i = V^T (W_i / lambda_i) on the FP64 matrix cores;
 //   - C (FP64) is then stored in the scene's precision like an upload.
