@@ -739,7 +739,10 @@ def main():
         roof["k_trace_frames"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
                                       achieved=fb / (ms_trf * 1e-3) / 1e9, algorithmic_bytes_per_launch=fb,
                                       frames_per_launch=L.TRACE_FUSE,
-                                      traffic=TRACE_FRAMES_TRAFFIC_64_1M if (n, N) == (64, 1 << 20) else None)
+                                      traffic=TRACE_FRAMES_TRAFFIC_64_1M if (n, N) == (64, 1 << 20) else None,
+                                      note="latency-bound gathers, not HBM: 12 dependent flux-pair loads per "
+                                           "particle-step, address unit busy 75-77% (TA_BUSY_avr), 69% of "
+                                           "wave-cycles waiting on memory (SQ_WAIT_INST_ANY), L1 hit rate 94%")
     rec_bytes = F * K * 4 + F * 4 + K * 8
     roof["k_reconstruct"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
                                  achieved=rec_bytes / (ms_rec * 1e-3) / 1e9, algorithmic_bytes_per_launch=rec_bytes,
