@@ -187,6 +187,26 @@ __global__ __launch_bounds__(256) void k_residual_sum(uint32_t k, uint32_t parts
     sq[c] = s;
 }
 
+// out[j n + i] = X[i m + j] for i < n, j < k: the first k columns of the
+// row-major n x m block as a column-major n x k matrix, through 32 x 32 LDS
+// tiles so that both sides are read and written along rows
+__global__ __launch_bounds__(256) void k_take_columns(uint32_t n, uint32_t m, uint32_t k, const double* __restrict__ X,
+                                                      double* __restrict__ out)
+{
+    __shared__ double tile[32][33];
+    const uint32_t i0 = blockIdx.x * 32u, j0 = blockIdx.y * 32u;
+    const uint32_t tx = threadIdx.x & 31u, ty = threadIdx.x >> 5;   // ty in 0..7
+    for (uint32_t r = ty; r < 32u; r += 8u) {
+        const uint32_t i = i0 + r, j = j0 + tx;
+        if (i < n && j < k) tile[r][tx] = X[static_cast<size_t>(i) * m + j];
+    }
+    __syncthreads();
+    for (uint32_t r = ty; r < 32u; r += 8u) {
+        const uint32_t j = j0 + r, i = i0 + tx;
+        if (i < n && j < k) out[static_cast<size_t>(j) * n + i] = tile[tx][r];
+    }
+}
+
 // H <- (H + H^T) / 2 for the column-major m x m H
 __global__ void k_symmetrise(uint32_t m, double* __restrict__ H)
 {
@@ -372,13 +392,13 @@ int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const i
     if (!converged)
         return set_err(ctx, PSF_ERR_STATE, "eigensolve: residual %.3e x lambda_max after %u products", worst / lmax,
                        products);
-    // eigenvalues and the K smallest vectors, n x K column-major
-    std::vector<double> xb(blk);
-    EIG_HIPCHK(ctx, hipMemcpy(xb.data(), X, blk * sizeof(double), hipMemcpyDeviceToHost));
-    for (uint32_t j = 0; j < K; ++j) {
-        lambda_out[j] = w[j];
-        for (uint32_t i = 0; i < n; ++i) X_out[size_t(j) * n + i] = xb[size_t(i) * m + j];
-    }
+    // eigenvalues and the K smallest vectors, n x K column-major (P is free
+    // scratch of n x m >= n x K doubles)
+    k_take_columns<<<dim3((n + 31) / 32, (K + 31) / 32), 256, 0, st>>>(n, m, K, X, P);
+    EIG_HIPCHK(ctx, hipGetLastError());
+    EIG_HIPCHK(ctx, hipMemcpyAsync(X_out, P, size_t(n) * K * sizeof(double), hipMemcpyDeviceToHost, st));
+    EIG_HIPCHK(ctx, hipStreamSynchronize(st));
+    for (uint32_t j = 0; j < K; ++j) lambda_out[j] = w[j];
     return PSF_OK;
 }
 
