@@ -279,6 +279,20 @@ def _basis_from_active(g: Grid, act, vecs, vals):
     return Phi, vals
 
 
+def _sparse_times_dense(S, V, workers=None):
+    """S @ V with V's columns split over host threads (scipy's sparse times
+    dense product runs on one core and releases the GIL)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    w = max(1, min(16, os.cpu_count() or 1, V.shape[1])) if workers is None else workers
+    if w == 1:
+        return S @ V
+    cols = np.array_split(np.arange(V.shape[1]), w)
+    with ThreadPoolExecutor(w) as ex:
+        parts = list(ex.map(lambda c: S @ np.ascontiguousarray(V[:, c]), cols))
+    return np.concatenate(parts, axis=1)
+
+
 def laplacian_eigenbasis_device(g: Grid, resolution: float, inside, K: int, ctx, tol=1e-11, degree=40,
                                 guard=0, seed=0, penalty=16.0):
     """laplacian_eigenbasis on the GPU: hip.eigensolve_csr on
@@ -323,11 +337,11 @@ def laplacian_eigenbasis_device(g: Grid, resolution: float, inside, K: int, ctx,
             extra += K - int(keep.sum()) + 8
             continue
         vals, vecs = vals[keep][:K], vecs[:, keep][:, :K]
-        step["max_div"] = float(np.abs(D @ vecs).max())
+        step["max_div"] = float(np.abs(_sparse_times_dense(D, vecs)).max())
         if step["max_div"] <= 1e-6:
             break
         penalty *= 4.0
-    res = np.linalg.norm(MII @ vecs - vecs * vals, axis=0)
+    res = np.linalg.norm(_sparse_times_dense(MII, vecs) - vecs * vals, axis=0)
     if np.any(res > 1e-8 * max(1.0, float(vals.max()))):
         raise RuntimeError(f"eigenpair residual {res.max():.3e}")
     Phi, lam = _basis_from_active(g, act, vecs, vals)

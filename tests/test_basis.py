@@ -119,3 +119,17 @@ def test_device_eigenbasis_helpers_reorder():
     assert np.array_equal(np.sort(perm), np.arange(act.size))
     z = fc[perm, 2]
     assert np.all(np.diff(z) >= 0)
+
+
+def test_sparse_times_dense_threads():
+    """the column-split product equals scipy's one-thread product, for more
+    workers than columns and for Fortran-ordered operands too"""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(3)
+    S = sp.random(300, 200, density=0.05, random_state=4, format="csr")
+    for cols in (1, 5, 37):
+        V = rng.standard_normal((200, cols))
+        ref = S @ V
+        for w in (None, 1, 3, 16):
+            assert np.array_equal(basis._sparse_times_dense(S, V, w), ref)
+        assert np.array_equal(basis._sparse_times_dense(S, np.asfortranarray(V)), ref)
