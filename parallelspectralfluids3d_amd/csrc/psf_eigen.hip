@@ -141,6 +141,84 @@ constexpr CsrLaunch kCsrLaunch[16] = {launch_csr<1>,  launch_csr<2>,  launch_csr
                                       launch_csr<9>,  launch_csr<10>, launch_csr<11>, launch_csr<12>,
                                       launch_csr<13>, launch_csr<14>, launch_csr<15>, launch_csr<16>};
 
+// The same for an even m, with rows ld = m apart: lane l holds the column
+// pairs (2l, 2l + 1) + 128 r, r < R2 = ceil(m / 128), each loaded as one
+// 16-byte load, so a nonzero costs R2 load instructions instead of
+// ceil(m / 64) (even m keeps every row start and pair 16-byte aligned)
+template <int R2>
+__global__ __launch_bounds__(256) void k_csr_block2(uint32_t n, uint32_t m, uint32_t per,
+                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                    const double* __restrict__ av, const double* __restrict__ Y,
+                                                    const double* __restrict__ Z, double alpha, double beta,
+                                                    double gamma, double* __restrict__ out)
+{
+    const uint32_t blk = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    const uint32_t row = __builtin_amdgcn_readfirstlane(blk * 4u + (threadIdx.x >> 6));
+    if (row >= n) return;
+    const uint32_t c0 = 2u * (threadIdx.x & 63u);
+    const int32_t b = rp[row], e = rp[row + 1];
+    double2 s[R2];
+#pragma unroll
+    for (int r = 0; r < R2; ++r) s[r] = make_double2(0.0, 0.0);
+    auto add = [&](double a, const double2& v, double2& acc) {
+        acc.x += a * v.x;
+        acc.y += a * v.y;
+    };
+    int32_t q = b;
+    for (; q + 4 <= e; q += 4) {
+        double a[4];
+        const double2* y[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            a[j] = av[q + j];
+            y[j] = reinterpret_cast<const double2*>(Y + static_cast<size_t>(ci[q + j]) * m + c0);
+        }
+        double2 v[4][R2];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < R2; ++r) v[j][r] = c0 + 128u * r < m ? y[j][64 * r] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < R2; ++r)
+                if (c0 + 128u * r < m) add(a[j], v[j][r], s[r]);
+    }
+    for (; q < e; ++q) {
+        const double a = av[q];
+        const double2* y = reinterpret_cast<const double2*>(Y + static_cast<size_t>(ci[q]) * m + c0);
+#pragma unroll
+        for (int r = 0; r < R2; ++r)
+            if (c0 + 128u * r < m) add(a, y[64 * r], s[r]);
+    }
+    const size_t o = static_cast<size_t>(row) * m + c0;
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+        if (c0 + 128u * r >= m) break;
+        const double2 yv = *reinterpret_cast<const double2*>(Y + o + 128 * r);
+        double2 v;
+        v.x = alpha * s[r].x + beta * yv.x;
+        v.y = alpha * s[r].y + beta * yv.y;
+        if (Z) {
+            const double2 zv = *reinterpret_cast<const double2*>(Z + o + 128 * r);
+            v.x += gamma * zv.x;
+            v.y += gamma * zv.y;
+        }
+        *reinterpret_cast<double2*>(out + o + 128 * r) = v;
+    }
+}
+
+template <int R2>
+void launch_csr2(hipStream_t st, uint32_t n, uint32_t m, uint32_t, const int32_t* rp, const int32_t* ci,
+                 const double* av, const double* Y, const double* Z, double a, double b, double g, double* out)
+{
+    const uint32_t per = ((n + 3) / 4 + kXcds - 1) / kXcds;
+    k_csr_block2<R2><<<per * kXcds, 256, 0, st>>>(n, m, per, rp, ci, av, Y, Z, a, b, g, out);
+}
+
+constexpr CsrLaunch kCsr2Launch[8] = {launch_csr2<1>, launch_csr2<2>, launch_csr2<3>, launch_csr2<4>,
+                                      launch_csr2<5>, launch_csr2<6>, launch_csr2<7>, launch_csr2<8>};
+
 // Gaussian start block: Box-Muller on two SplitMix64 draws of (seed, index)
 __device__ __forceinline__ uint64_t splitmix(uint64_t z)
 {
@@ -310,7 +388,7 @@ int psf_eigensolve_csr(psf_ctx* ctx, uint32_t n, const int32_t* row_ptr, const i
     const rocblas_int mi = static_cast<rocblas_int>(m), ni = static_cast<rocblas_int>(n);
     const double one = 1.0, zero = 0.0;
     uint32_t products = 0;
-    const CsrLaunch csr = kCsrLaunch[(m + 63) / 64 - 1];
+    const CsrLaunch csr = m % 2 == 0 ? kCsr2Launch[(m + 127) / 128 - 1] : kCsrLaunch[(m + 63) / 64 - 1];
     auto apply = [&](const double* in, const double* z, double a, double b, double g, double* out) {
         csr(st, n, m, m, drp, dci, dav, in, z, a, b, g, out);
         ++products;

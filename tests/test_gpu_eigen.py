@@ -31,11 +31,14 @@ def _path(n, neumann):
     return sp.diags([main, -np.ones(n - 1), -np.ones(n - 1)], [0, 1, -1], format="csr")
 
 
-def test_path_laplacian_closed_form(hip_ctx):
+# block widths K + guard: 40 and 150 (even: two columns per 16-byte load; 150
+# leaves the second 128-column chunk partly empty) and 39 (odd: one per load)
+@pytest.mark.parametrize("K,guard", [(24, 16), (24, 15), (40, 110)])
+def test_path_laplacian_closed_form(hip_ctx, K, guard):
     from parallelspectralfluids3d_amd import hip
-    _say("eigensolve: Dirichlet path Laplacian n=2000 K=24")
-    n, K = 2000, 24
-    lam, X, used = hip.eigensolve_csr(hip_ctx, _path(n, False), K, guard=16, degree=30, tol=1e-12)
+    _say(f"eigensolve: Dirichlet path Laplacian n=2000 K={K} guard={guard}")
+    n = 2000
+    lam, X, used = hip.eigensolve_csr(hip_ctx, _path(n, False), K, guard=guard, degree=30, tol=1e-12)
     k = np.arange(1, K + 1)
     exact = 2.0 - 2.0 * np.cos(np.pi * k / (n + 1))
     assert np.max(np.abs(lam - exact)) < 1e-12
