@@ -293,7 +293,7 @@ def _sparse_times_dense(S, V, workers=None):
     return np.concatenate(parts, axis=1)
 
 
-def laplacian_eigenbasis_device(g: Grid, resolution: float, inside, K: int, ctx, tol=1e-11, degree=40,
+def laplacian_eigenbasis_device(g: Grid, resolution: float, inside, K: int, ctx, tol=1e-11, degree=80,
                                 guard=0, seed=0, penalty=16.0):
     """laplacian_eigenbasis on the GPU: hip.eigensolve_csr on
     A = M_II + penalty * D^T D, which lifts M_II's gradient null space and
