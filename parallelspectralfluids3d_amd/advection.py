@@ -17,7 +17,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .basis import curl_operator
+from .basis import _sparse_times_dense, curl_operator
 from .grid import Grid
 
 # (face slot holding the edge, edge slot in that face, face a, face b), faces 0-based f1..f6
@@ -53,7 +53,7 @@ def build_advection(g: Grid, resolution: float, inside, Phi_cm, lam, V_cm=None, 
     import scipy.sparse as sp
     K, F = Phi_cm.shape
     if V_cm is None:
-        V_cm = (curl_operator(g, resolution) @ Phi_cm.T).T
+        V_cm = _sparse_times_dense(curl_operator(g, resolution), Phi_cm.T).T
     PhiT = np.ascontiguousarray(Phi_cm.T)           # F x K
     VT = np.ascontiguousarray(V_cm.T)               # E x K
     e, a, b, c = wedge_terms(g, inside)
@@ -87,7 +87,7 @@ def advection_terms(g: Grid, resolution: float, inside):
 def build_advection_device(scene, g: Grid, resolution: float, inside, Phi_cm, lam, V_cm=None):
     """build_advection on the GPU, into the scene's C (any precision but FP16)."""
     if V_cm is None:
-        V_cm = (curl_operator(g, resolution) @ Phi_cm.T).T
+        V_cm = _sparse_times_dense(curl_operator(g, resolution), Phi_cm.T).T
     scene.build_advection(Phi_cm, V_cm, lam, advection_terms(g, resolution, inside))
 
 
@@ -136,7 +136,7 @@ def build_advection_box_analytic(g: Grid, resolution: float, Phi_cm, lam, table,
     if given, is called after each of the 12 edge slots."""
     K = Phi_cm.shape[0]
     if V_cm is None:
-        V_cm = (curl_operator(g, resolution) @ Phi_cm.T).T        # (K, E)
+        V_cm = _sparse_times_dense(curl_operator(g, resolution), Phi_cm.T).T        # (K, E)
     dims = {"x": g.nx, "y": g.ny, "z": g.nz}
     freq = {"x": table[:, 1].astype(int), "y": table[:, 2].astype(int), "z": table[:, 3].astype(int)}
     amp = {"x": table[:, 4], "y": table[:, 5], "z": table[:, 6]}
