@@ -318,6 +318,8 @@ def laplacian_eigenbasis_device(g: Grid, resolution: float, inside, K: int, ctx,
     # face numbering would put them a third of the faces apart
     fc = g.face_coords[act]
     perm = np.lexsort((fc[:, 0], fc[:, 1], fc[:, 2]))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
     products = 0
     attempt = 0
     history = []
@@ -327,8 +329,7 @@ def laplacian_eigenbasis_device(g: Grid, resolution: float, inside, K: int, ctx,
             raise RuntimeError("no K accepted divergence-free eigenpairs after raising the penalty and the count")
         A = (MII + penalty * DtD).tocsr()[perm][:, perm]
         vals, vp, used = hip.eigensolve_csr(ctx, A, K + extra, guard=guard, degree=degree, tol=tol, seed=seed)
-        vecs = np.empty_like(vp)
-        vecs[perm] = vp
+        vecs = np.take(vp, inv, axis=0)   # vecs[perm] = vp, as one gather
         products += used
         keep = vals > 1e-10
         step = dict(modes=K + extra, penalty=penalty, products=used, accepted=int(keep.sum()))
