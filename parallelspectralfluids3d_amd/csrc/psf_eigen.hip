@@ -14,7 +14,8 @@
 // A null space is not special: its vectors are the smallest eigenpairs.  The
 // caller lifts one it does not want (the DEC Laplacian's gradients) with a
 // penalty term, as the host shift-invert path does.
-// Products with A: one fused CSR kernel per polynomial degree.  Gram matrices:
+// Products with A: one fused CSR kernel per polynomial degree (k_csr_block2
+// with 16-byte column-pair loads for an even m, else k_csr_block).  Gram matrices:
 // split-reduction DGEMM (psf_blas.hpp); rotations: rocBLAS DGEMM; orthonormalisation: Cholesky QR twice
 // (rocSOLVER POTRF, rocBLAS TRSM in place); H's eigendecomposition: rocSOLVER
 // SYEVD.  The block is stored row-major (n x m), so each nonzero of a CSR row
