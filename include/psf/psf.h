@@ -192,6 +192,21 @@ PSF_API int psf_integrate(psf_scene* scene, int nframes);
    holds W * (4 * F' + 8 * K) + 8 * K more bytes, F' = F rounded up to 256. */
 PSF_API int psf_set_frame_window(psf_scene* scene, uint32_t frames);
 
+/* What a scene's calls have queued since it was created (no counterpart in
+   the reference; for tests and measurement).  A window fill is one launch of
+   reduced steps plus one reconstruction of their velocity fields. */
+typedef struct psf_scene_counters {
+    uint64_t frames;             /* integrate() frames queued */
+    uint64_t window_fills;       /* frame windows computed */
+    uint64_t window_frames;      /* frames those fills computed */
+    uint64_t windows_dropped;    /* windows ended early by a state change */
+    uint64_t frames_dropped;     /* frames computed ahead and discarded by those drops */
+    uint64_t step_launches;      /* reduced-step launches (a persistent launch, or one plain step) */
+    uint64_t recon_launches;     /* velocity-field launches (one frame or one window) */
+    uint64_t particle_launches;  /* particle-pass launches (one or several frames each) */
+} psf_scene_counters;
+PSF_API int psf_scene_get_counters(psf_scene* scene, psf_scene_counters* out);
+
 /* ---- scene cache -----------------------------------------------------------------
  * A versioned file of a scene's precomputed inputs: Phi (optional, FP64, F x K
  * column-major), lambda, g (optional) and C (FP64, K matrices K x K), i.e. what
@@ -272,16 +287,28 @@ PSF_API int psf_density(psf_scene* scene, uint32_t* counts, uint16_t* volume);
 
 /* ---- measurement -----------------------------------------------------------
  * Runs `count` launches of one phase back to back on the context stream
- * between two HIP events and returns the elapsed milliseconds.
+ * between two HIP events and returns the elapsed milliseconds.  The timed
+ * region is bracketed by two launches of an empty kernel,
+ * k_timed_region_mark, so that kernel traces show what it contains.
  * phase: 0 reduced step (one step per launch), 1 reconstruction, 2 trace,
- *        3 full integrate frame, 4 one launch of `count` reduced steps,
+ *        3 `count` full integrate frames from a frame-window boundary: the
+ *          frames computed ahead are dropped before the start event, window
+ *          fills compute no frame past the count, and the end event follows
+ *          every stream's work, so the region holds exactly the work of the
+ *          `count` frames (their reduced steps, velocity fields and particle
+ *          passes),
+ *        4 one launch of `count` reduced steps,
  *        5 the frame window's reconstruction (all W frames; window > 1,
  *          recomputes the u of the frames already computed ahead),
  *        6 a density count pass of the stored particles (adds to the
  *          counters), 7 a density volume pass (clears the counters),
  *        8 a full integrate frame followed by psf_density without copies,
  *        9 one launch of the particle passes the windowed frame loop fuses
- *          (8 frames; needs a frame window of at least 8). */
+ *          (8 frames; needs a frame window of at least 8),
+ *       10 `count` integrate() frames continuing the current frame window
+ *          (as a caller's frame loop runs them; frames computed ahead before
+ *          the start event are not in the region), then the deferred
+ *          particle passes: count = 1 times one call's frame. */
 PSF_API int psf_time_phase(psf_scene* scene, int phase, int count, double* ms);
 
 /* ---- scene server (continuous batching) ----------------------------------------

@@ -275,6 +275,7 @@ public:
         check(psf_set_particles(scene_, particles.data(), d.max_particles), "set_particles");
         if (densityDims_[0]) check(psf_set_density_grid(scene_, densityDims_[0], densityDims_[1], densityDims_[2], 1),
                                    "set_density_grid");
+        if (frameWindow_ > 1) check(psf_set_frame_window(scene_, frameWindow_), "set_frame_window");
         paramsChanged();
         deviceAhead_ = false;
         particlesDirty_ = false;
@@ -302,6 +303,28 @@ public:
         }
         simTime += 1.0;
         deviceAhead_ = true;
+    }
+
+    // Frame window (psf_set_frame_window; no counterpart in the reference):
+    // integrate() then computes W frames' reduced steps and velocity fields
+    // at once, ahead of the particle passes, with the bits of single frames.
+    // The reference's call pattern -- setTimestep and friends with unchanged
+    // values, then integrate(), once per frame -- keeps the window.  Kept
+    // across upload(); W is 1 (off), 2, 4, 8, 16 or 32.
+    void setFrameWindow(uint32_t frames)
+    {
+        if (scene_) check(psf_set_frame_window(scene_, frames), "set_frame_window");
+        else if (frames != 1 && frames != 2 && frames != 4 && frames != 8 && frames != 16 && frames != 32)
+            throw Error(PSF_ERR_ARG, "frame window must be 1, 2, 4, 8, 16 or 32");
+        frameWindow_ = frames;
+    }
+    uint32_t getFrameWindow() const { return frameWindow_; }
+    // what the device path has queued (psf_scene_get_counters)
+    psf_scene_counters counters()
+    {
+        psf_scene_counters c{};
+        if (scene_) check(psf_scene_get_counters(scene_, &c), "get_counters");
+        return c;
     }
 
     // The particle density volume the reference draws (psfsolvergpu.cpp:876-913):
@@ -386,6 +409,7 @@ private:
     bool deviceAhead_ = false;
     bool particlesDirty_ = false;    // the host records changed since the last upload
     uint32_t densityDims_[3] = {0, 0, 0};
+    uint32_t frameWindow_ = 1;
     std::vector<uint16_t> volume_;
 };
 

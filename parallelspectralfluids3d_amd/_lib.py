@@ -19,6 +19,7 @@ PHI_FP64, PHI_FP32 = 0, 1
 PATH_PLAIN, PATH_PERSISTENT = 0, 1
 PHASE_STEP, PHASE_RECON, PHASE_TRACE, PHASE_FRAME, PHASE_NSTEP, PHASE_WINDOW_RECON = 0, 1, 2, 3, 4, 5
 PHASE_DENSITY_COUNT, PHASE_DENSITY_VOLUME, PHASE_FRAME_DENSITY, PHASE_TRACE_FUSED = 6, 7, 8, 9
+PHASE_FRAME_CONTINUE = 10   # frames continuing the current window (a caller's frame loop)
 TRACE_FUSE = 8   # frames per fused particle launch of the windowed frame loop
 
 _STATUS = {0: "PSF_OK", -1: "PSF_ERR_ARG", -2: "PSF_ERR_HIP", -3: "PSF_ERR_NOMEM", -4: "PSF_ERR_STATE",
@@ -42,6 +43,13 @@ class SceneDesc(ct.Structure):
 class ServerConfig(ct.Structure):
     _fields_ = [("window", ct.c_uint32), ("max_active", ct.c_uint32), ("mem_budget", ct.c_uint64),
                 ("modes_per_wg", ct.c_uint32)]
+
+
+class SceneCounters(ct.Structure):
+    _fields_ = [("frames", ct.c_uint64), ("window_fills", ct.c_uint64), ("window_frames", ct.c_uint64),
+                ("windows_dropped", ct.c_uint64), ("frames_dropped", ct.c_uint64),
+                ("step_launches", ct.c_uint64), ("recon_launches", ct.c_uint64),
+                ("particle_launches", ct.c_uint64)]
 
 
 class ServerStats(ct.Structure):
@@ -103,6 +111,7 @@ def load():
         "psf_trace": (i32, [vp, vp]),
         "psf_integrate": (i32, [vp, i32]),
         "psf_set_frame_window": (i32, [vp, ct.c_uint32]),
+        "psf_scene_get_counters": (i32, [vp, ct.POINTER(SceneCounters)]),
         "psf_time_phase": (i32, [vp, i32, i32, ct.POINTER(ct.c_double)]),
         "psf_server_create": (i32, [vp, ct.POINTER(ServerConfig), pp]),
         "psf_server_destroy": (i32, [vp]),

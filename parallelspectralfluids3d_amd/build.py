@@ -18,7 +18,6 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libpsf_hip.so")
-SOLVER_CHECK = os.path.join(PKG, "solver_check")
 ARCH = os.environ.get("PSF_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["psf_kernels.hip", "psf_persistent.hip", "psf_api.hip", "psf_server.hip", "psf_box.hip",
@@ -71,16 +70,8 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    # the C++ solver driver (include/psf/solver.hpp over the C ABI), host code only
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
-           os.path.join(CSRC, "solver_check.cpp"), f"-L{PKG}", "-lpsf_hip", "-Wl,-rpath,$ORIGIN",
-           "-o", SOLVER_CHECK + ".tmp"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"solver_check build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(SOLVER_CHECK + ".tmp", SOLVER_CHECK)
     if verbose:
-        print(f"built {LIB} and {SOLVER_CHECK}")
+        print(f"built {LIB}")
     return LIB
 
 

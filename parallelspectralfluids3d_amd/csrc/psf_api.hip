@@ -225,6 +225,7 @@ int enqueue_steps(psf_scene* s, int n, double* hist = nullptr, int hist_ld = 0, 
     if (s->path == PSF_PATH_PERSISTENT) {
         hipError_t e = launch_persistent_steps(s, n, st, hist, hist_ld);
         if (e != hipSuccess) return set_err(s->ctx, PSF_ERR_HIP, "persistent step launch: %s", hipGetErrorString(e));
+        if (n > 0) s->cnt.step_launches++;
         return PSF_OK;
     }
     if (!is_plain_precision(s->d.c_precision))
@@ -233,6 +234,7 @@ int enqueue_steps(psf_scene* s, int n, double* hist = nullptr, int hist_ld = 0, 
         HIPCHK(s->ctx, launch_contract_plain(s->d.c_precision, s->dev.C, s->dev.omega, s->dev.y, K, st));
         HIPCHK(s->ctx, launch_update(s->dev.omega, s->dev.y, s->dev.visc, s->dev.gravity, K, s->dt, s->gravity_active,
                                      s->dev.status, hist, hist_ld, it, st));
+        s->cnt.step_launches++;
     }
     return PSF_OK;
 }
@@ -251,6 +253,7 @@ int enqueue_reconstruct(psf_scene* s)
 {
     HIPCHK(s->ctx, launch_reconstruct(s->d.phi_precision, s->dev.Phi, s->dev.ldF, s->F, static_cast<int>(s->d.K),
                                       s->dev.omega, s->dev.u, s->ctx->stream));
+    s->cnt.recon_launches++;
     return PSF_OK;
 }
 
@@ -265,6 +268,7 @@ int flush_traces(psf_scene* s)
     HIPCHK(s->ctx, launch_trace_frames(p, win_u(s, s->win_buf) + size_t(s->trace_c0) * s->dev.ldF, s->dev.ldF,
                                        s->trace_pending, s->dev.parts, s->dev.pid, s->n_particles, D,
                                        s->ctx->stream));
+    s->cnt.particle_launches++;
     s->frame += s->trace_pending;
     s->trace_pending = 0;
     return PSF_OK;
@@ -277,20 +281,31 @@ int enqueue_trace(psf_scene* s, int32_t* dcells)
     DensityGrid D;
     if (int rc = density_for_trace(s, D)) return rc;
     HIPCHK(s->ctx, launch_trace(p, current_u(s), s->dev.parts, s->dev.pid, s->n_particles, dcells, D, s->ctx->stream));
+    s->cnt.particle_launches++;
     s->frame++;
     return PSF_OK;
 }
 
-// W reduced steps from dev.omega into buffer b's history, then the W velocity
-// fields into buffer b's u rows, on stream st
-int enqueue_window_fill(psf_scene* s, uint32_t b, hipStream_t st)
+// n <= W reduced steps from dev.omega into columns 0..n-1 of buffer b's
+// history, then their n velocity fields into buffer b's u rows, on stream st
+int enqueue_window_fill(psf_scene* s, uint32_t b, hipStream_t st, uint32_t n)
 {
     const int W = static_cast<int>(s->window);
-    int rc = enqueue_steps(s, W, win_hist(s, b), W, st);
+    int rc = enqueue_steps(s, static_cast<int>(n), win_hist(s, b), W, st);
     if (rc) return rc;
-    HIPCHK(s->ctx, launch_reconstruct_window(s->d.phi_precision, W, s->dev.Phi, s->dev.ldF, s->F,
+    HIPCHK(s->ctx, launch_reconstruct_window(s->d.phi_precision, W, static_cast<int>(n), s->dev.Phi, s->dev.ldF, s->F,
                                              static_cast<int>(s->d.K), win_hist(s, b), win_u(s, b), st));
+    s->cnt.recon_launches++;
+    s->cnt.window_fills++;
+    s->cnt.window_frames += n;
     return PSF_OK;
+}
+
+// frames the next window fill computes: the window, or in a bounded run no
+// more than the frames still to come (0: none)
+uint32_t next_fill_size(const psf_scene* s)
+{
+    return s->bounded ? std::min(s->window, s->fill_budget) : s->window;
 }
 
 // Ends a frame window early: w and u of the current frame become dev.omega and
@@ -300,9 +315,15 @@ int window_flush(psf_scene* s)
 {
     if (int rc = flush_traces(s)) return rc;
     hipStream_t m = s->ctx->stream;
+    uint64_t dropped = s->win_filled - s->win_cursor;
     if (s->win_ahead) {   // the side stream's window also moved dev.omega: wait for it
         HIPCHK(s->ctx, hipStreamWaitEvent(m, s->ev_ready[s->win_buf ^ 1u], 0));
         s->win_ahead = false;
+        dropped += s->win_ahead_n;
+    }
+    if (dropped) {
+        s->cnt.windows_dropped++;
+        s->cnt.frames_dropped += dropped;
     }
     if (s->win_filled == 0) return PSF_OK;
     const int t = static_cast<int>(s->win_cursor) - 1;
@@ -325,6 +346,7 @@ int window_flush(psf_scene* s)
 int enqueue_frame(psf_scene* s)
 {
     int rc;
+    s->cnt.frames++;
     if (s->window <= 1) {
         if ((rc = enqueue_steps(s, 1))) return rc;
         if ((rc = enqueue_reconstruct(s))) return rc;
@@ -336,22 +358,30 @@ int enqueue_frame(psf_scene* s)
         if (s->win_filled > 0 && s->win_pipeline) next = s->win_buf ^ 1u;
         s->win_filled = 0;
         s->win_cursor = 0;
+        uint32_t n;
         if (s->win_ahead) {
             HIPCHK(s->ctx, hipStreamWaitEvent(m, s->ev_ready[next], 0));
             s->win_ahead = false;
-        } else if ((rc = enqueue_window_fill(s, next, m))) {
-            return rc;
+            n = s->win_ahead_n;
+        } else {
+            n = next_fill_size(s);
+            if (n == 0) n = s->window;   // a bounded run asked for more frames than it announced
+            if ((rc = enqueue_window_fill(s, next, m, n))) return rc;
+            if (s->bounded) s->fill_budget -= std::min(n, s->fill_budget);
         }
         s->win_buf = next;
-        s->win_filled = s->window;
-        if (s->win_pipeline) {
+        s->win_filled = n;
+        const uint32_t n2 = s->win_pipeline ? next_fill_size(s) : 0u;
+        if (n2 > 0) {
             // the window after this one, on the side stream, behind everything
             // queued so far on the context stream (omega, the buffer's last traces)
             HIPCHK(s->ctx, hipEventRecord(s->ev_fork, m));
             HIPCHK(s->ctx, hipStreamWaitEvent(s->side, s->ev_fork, 0));
-            if ((rc = enqueue_window_fill(s, next ^ 1u, s->side))) return rc;
+            if ((rc = enqueue_window_fill(s, next ^ 1u, s->side, n2))) return rc;
             HIPCHK(s->ctx, hipEventRecord(s->ev_ready[next ^ 1u], s->side));
+            if (s->bounded) s->fill_budget -= n2;
             s->win_ahead = true;
+            s->win_ahead_n = n2;
         }
     }
     s->win_cursor++;
@@ -360,6 +390,19 @@ int enqueue_frame(psf_scene* s)
     if (s->trace_pending == 0) s->trace_c0 = s->win_cursor - 1;
     s->trace_pending++;
     if (s->trace_pending == kTraceFuse || s->win_cursor == s->win_filled) return flush_traces(s);
+    return PSF_OK;
+}
+
+// Host-to-device copy into a scene buffer on the context stream, so that it
+// lands after every kernel queued there that reads or writes the buffer (and,
+// through window_flush, after the side stream's window); returns when the
+// copy is complete, so the host source may be freed.  (A null-stream copy is
+// not ordered against the context stream, which is non-blocking.)
+int upload_sync(psf_scene* s, void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0) return PSF_OK;
+    HIPCHK(s->ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
     return PSF_OK;
 }
 
@@ -464,6 +507,8 @@ int psf_ctx_sync(psf_ctx* ctx)
     for (psf_scene* s : ctx->scenes)
         if (int rc = flush_traces(s)) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (psf_scene* s : ctx->scenes)   // windows computed ahead on the side streams
+        if (s->side) HIPCHK(ctx, hipStreamSynchronize(s->side));
     return PSF_OK;
 }
 
@@ -651,11 +696,11 @@ int psf_scene_upload_advection(psf_scene* s, const double* C)
     if (int rc0 = window_flush(s)) return rc0;
     const size_t K = s->d.K, n = K * K * K;
     if (s->d.c_precision == PSF_C_FP64) {
-        HIPCHK(ctx, hipMemcpy(s->dev.C, C, n * 8, hipMemcpyHostToDevice));
+        if (int rc = upload_sync(s, s->dev.C, C, n * 8)) return rc;
     } else if (s->d.c_precision == PSF_C_FP32) {
         std::vector<float> t(n);
         for (size_t i = 0; i < n; ++i) t[i] = static_cast<float>(C[i]);
-        HIPCHK(ctx, hipMemcpy(s->dev.C, t.data(), n * 4, hipMemcpyHostToDevice));
+        if (int rc = upload_sync(s, s->dev.C, t.data(), n * 4)) return rc;
     } else {
         int rc = persistent_upload_c(s, C);
         if (rc) return rc;
@@ -706,6 +751,11 @@ int psf_set_params(psf_scene* s, double dt, double viscosity, int gravity_active
     if (!s) return PSF_ERR_ARG;
     if (!std::isfinite(dt) || !std::isfinite(viscosity)) return set_err(s->ctx, PSF_ERR_ARG, "non-finite parameter");
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    // a caller that passes the same values every frame (the reference's
+    // integrate() reads them every frame) keeps its frames computed ahead
+    if (dt == s->dt && viscosity == s->nu && (gravity_active ? 1 : 0) == s->gravity_active &&
+        std::memcmp(&lifetime, &s->lifetime, sizeof(float)) == 0)
+        return PSF_OK;
     if (int rc0 = window_flush(s)) return rc0;
     s->dt = dt;
     s->nu = viscosity;
@@ -841,8 +891,9 @@ int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
     std::vector<float> sorted(size_t(n) * 4);
     for (uint32_t j = 0; j < n; ++j) std::memcpy(&sorted[4 * size_t(j)], xyzw + 4 * size_t(pid[j]), 16);
     if (n) {
-        HIPCHK(s->ctx, hipMemcpy(s->dev.parts, sorted.data(), size_t(n) * 16, hipMemcpyHostToDevice));
-        HIPCHK(s->ctx, hipMemcpy(s->dev.pid, pid.data(), size_t(n) * 4, hipMemcpyHostToDevice));
+        // after the passes just launched (and any earlier ones) on the context stream
+        if (int rc = upload_sync(s, s->dev.parts, sorted.data(), size_t(n) * 16)) return rc;
+        if (int rc = upload_sync(s, s->dev.pid, pid.data(), size_t(n) * 4)) return rc;
     }
     s->n_particles = n;
     density_particles_changed(s);
@@ -918,9 +969,16 @@ int psf_integrate(psf_scene* s, int nframes)
     return PSF_OK;
 }
 
+int psf_scene_get_counters(psf_scene* s, psf_scene_counters* out)
+{
+    if (!s || !out) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    *out = s->cnt;
+    return PSF_OK;
+}
+
 int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
 {
-    if (!s || !ms || count <= 0 || phase < 0 || phase > 9) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    if (!s || !ms || count <= 0 || phase < 0 || phase > 10) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
     if (phase == 9 && (s->window < kTraceFuse || !s->has_phi))
         return set_err(s->ctx, PSF_ERR_STATE, "phase 9 needs Phi and a frame window of at least %u", kTraceFuse);
     if (phase == 5 && s->window <= 1) return set_err(s->ctx, PSF_ERR_STATE, "phase 5 needs a frame window > 1");
@@ -930,31 +988,44 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
         int rc = ready_for_step(s);
         if (rc) return rc;
     }
-    if ((phase == 1 || phase == 3 || phase == 5 || phase == 8) && !s->has_phi) return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
+    if ((phase == 1 || phase == 3 || phase == 5 || phase == 8 || phase == 10) && !s->has_phi)
+        return set_err(s->ctx, PSF_ERR_STATE, "Phi not uploaded");
     psf_ctx* ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    if (phase == 0 || phase == 1 || phase == 4 || phase == 5) {
+    // frames from a window boundary (3, 8) and the single phases start with
+    // nothing computed ahead
+    if (phase != 2 && phase != 6 && phase != 7 && phase != 9 && phase != 10) {
         int rc0 = window_flush(s);
         if (rc0) return rc0;
     }
     if (int rc0 = flush_traces(s)) return rc0;   // deferred passes belong before the timed region
+    if (phase == 3 || phase == 8) {
+        // everything queued so far, on either stream, ends before the start event
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (s->side) HIPCHK(ctx, hipStreamSynchronize(s->side));
+        s->bounded = true;
+        s->fill_budget = static_cast<uint32_t>(count);
+    }
     hipEvent_t a, b;
     HIPCHK(ctx, hipEventCreate(&a));
     HIPCHK(ctx, hipEventCreate(&b));
     int rc = PSF_OK;
-    hipError_t e = hipEventRecord(a, ctx->stream);
+    hipError_t e = launch_timed_region_mark(0, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
     if (e != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "event record: %s", hipGetErrorString(e));
     if (rc == PSF_OK) {
         switch (phase) {
         case 0: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_steps(s, 1); break;
         case 1: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_reconstruct(s); break;
         case 2: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_trace(s, nullptr); break;
-        case 3: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_frame(s); break;
+        case 3:
+        case 10: for (int i = 0; i < count && rc == PSF_OK; ++i) rc = enqueue_frame(s); break;
         case 5:
             for (int i = 0; i < count && rc == PSF_OK; ++i) {
-                hipError_t le = launch_reconstruct_window(s->d.phi_precision, static_cast<int>(s->window), s->dev.Phi,
-                                                          s->dev.ldF, s->F, static_cast<int>(s->d.K), s->dev.hist,
-                                                          s->dev.u_hist, ctx->stream);   // buffer 0, after the flush
+                const int W = static_cast<int>(s->window);
+                hipError_t le = launch_reconstruct_window(s->d.phi_precision, W, W, s->dev.Phi, s->dev.ldF, s->F,
+                                                          static_cast<int>(s->d.K), s->dev.hist, s->dev.u_hist,
+                                                          ctx->stream);   // buffer 0, after the flush
                 if (le != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "window reconstruction: %s", hipGetErrorString(le));
             }
             break;
@@ -978,9 +1049,17 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
         default: rc = enqueue_steps(s, count); break;
         }
     }
+    s->bounded = false;
+    s->fill_budget = 0;
     if (rc == PSF_OK) rc = flush_traces(s);      // the timed frames' deferred passes
+    if (rc == PSF_OK && (phase == 3 || phase == 8) && s->win_ahead) {
+        // (a bounded run computes nothing ahead; should it have, that work is in the region)
+        e = hipStreamWaitEvent(ctx->stream, s->ev_ready[s->win_buf ^ 1u], 0);
+        if (e != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "stream wait: %s", hipGetErrorString(e));
+    }
     if (rc == PSF_OK) {
         e = hipEventRecord(b, ctx->stream);
+        if (e == hipSuccess) e = launch_timed_region_mark(1, ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(b);
         float t = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
@@ -989,7 +1068,7 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    if (rc == PSF_OK && (phase == 0 || phase == 3 || phase == 4 || phase == 8)) rc = check_status(s);
+    if (rc == PSF_OK && (phase == 0 || phase == 3 || phase == 4 || phase == 8 || phase == 10)) rc = check_status(s);
     return rc;
 }
 

@@ -271,6 +271,9 @@ int psf_scene_generate_box_basis(psf_scene* s, const int32_t* freqs, const doubl
     BoxModes m;
     if (prepare(s, freqs, m)) return fail(ctx, PSF_ERR_ARG, "a mode frequency lies outside [0, n) of its axis");
     BOX_HIPCHK(ctx, hipSetDevice(ctx->device));
+    // Phi changes: frames computed ahead are dropped (and the side stream's
+    // reads of Phi come before the writes below)
+    if (int rc = scene_window_flush(s)) return rc;
     // 1-D profiles: node sines then cell cosines of each axis
     std::vector<double> prof;
     int noff[3], coff[3];
@@ -317,6 +320,7 @@ int psf_scene_build_box_advection(psf_scene* s, const int32_t* freqs, const doub
     for (int k = 0; k < m.K; ++k)
         if (!(lambda[k] != 0.0) || !std::isfinite(lambda[k])) return fail(ctx, PSF_ERR_ARG, "lambda must be finite and non-zero");
     BOX_HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (int rc = scene_window_flush(s)) return rc;   // C changes: as in psf_scene_generate_box_basis
     // Hf = 1/(2h) and s_h = (h/2)^2 with h the float resolution (dec.cpp:143-193, psfsolver.cpp:444)
     const float hf32 = static_cast<float>(s->d.resolution);
     const double Hf = 1.0 / (2.0 * static_cast<double>(hf32));

@@ -89,6 +89,12 @@ struct psf_scene {
     // trace_c0 .. trace_c0 + trace_pending - 1 of the current window buffer
     // (they run as one launch; every read of the particles launches them)
     uint32_t trace_pending = 0, trace_c0 = 0;
+    uint32_t win_ahead_n = 0;    // frames in the window queued on the side stream
+    // a bounded run (psf_time_phase's frame phases): window fills compute at
+    // most fill_budget more frames, so nothing past the run is computed ahead
+    bool bounded = false;
+    uint32_t fill_budget = 0;
+    psf_scene_counters cnt{};    // psf_scene_get_counters
     hipStream_t side = nullptr;
     hipEvent_t ev_ready[2] = {nullptr, nullptr};   // side stream: buffer filled
     hipEvent_t ev_fork = nullptr;                  // context stream position the side stream starts from

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_update(double* __restrict__ omega, cons
 // 64*VEC consecutive faces (VEC per lane, one aligned 16-byte load per lane
 // and mode) and runs the whole sum for them: an FP64 fma chain over
 // k = 0, 1, ..., K-1 from 0.0, rounded to float at the end.  The w_k are
-// uniform across the wave and come through scalar loads.  k_reconstruct_window
+// uniform across the wave and come through scalar loads.  k_reconstruct_mfma
 // runs the same chain per face and frame, so both give the same bits.
 // FP32 Phi: one face per lane (measured faster than 2 or 4 here), 16 modes in flight
 constexpr int kGemvVecF32 = 1;
@@ -175,13 +175,6 @@ __global__ __launch_bounds__(256) void k_reconstruct(const T* __restrict__ Phi, 
 }
 
 // ---------------------------------------------------------------------------
-// Reconstruction for a window of TT frames: u_t = float(Phi w_t), t < TT,
-// from the history hist[k * TT + t] of w.  One pass over Phi serves all TT
-// frames, so the kernel is bound by FP64 FMA throughput instead of HBM.
-// Every wave owns 64*FPT consecutive faces, each lane FPT of them for all TT
-// frames, and runs the same per-face fma chain over k = 0..K-1 as
-// k_reconstruct.  The w_t of one mode are uniform across the wave and come
-// through scalar loads.
 // Blocks of a launch are dispatched to the 8 XCDs round-robin (block b to XCD
 // b % 8), and each XCD has its own L2.  xcd_block maps launch block b to
 // logical block (b % 8) * per + b / 8, so XCD x runs the contiguous logical
@@ -201,57 +194,6 @@ __device__ __forceinline__ int scene_of_block(const Tab* __restrict__ tab, int n
     const int lane = threadIdx.x & (kWave - 1);
     const bool le = lane < n && tab[lane].block0 <= blk;
     return __popcll(__ballot(le)) - 1;
-}
-
-template <typename T, int FPT, int TT>
-__device__ __forceinline__ void window_body(const T* Phi_, size_t ldF, uint32_t F, int K, const double* hist_,
-                                            float* uh_, uint32_t blk)
-{
-    const gptr<const T> Phi = as_global(Phi_);
-    const cptr<double> hist = as_const(hist_);
-    const gptr<float> uh = as_global(uh_);
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const size_t fw = (static_cast<size_t>(blk) * 4 + wv) * (kWave * FPT);
-    if (fw >= ldF) return;
-    const size_t f0 = fw + static_cast<size_t>(lane) * FPT;
-    double acc[FPT][TT];
-#pragma unroll
-    for (int j = 0; j < FPT; ++j)
-#pragma unroll
-        for (int t = 0; t < TT; ++t) acc[j][t] = 0.0;
-    const gptr<const T> base = Phi + f0;
-#pragma unroll 2
-    for (int k = 0; k < K; ++k) {
-        double ph[FPT];
-        if constexpr (sizeof(T) == 4 && FPT == 4) {
-            const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4>>(base + static_cast<size_t>(k) * ldF));
-            ph[0] = v.x; ph[1] = v.y; ph[2] = v.z; ph[3] = v.w;
-        } else if constexpr (sizeof(T) == 4 && FPT == 2) {
-            typedef float f32x2 __attribute__((ext_vector_type(2)));
-            const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x2>>(base + static_cast<size_t>(k) * ldF));
-            ph[0] = v.x; ph[1] = v.y;
-        } else {
-#pragma unroll
-            for (int j = 0; j < FPT; j += 2) {
-                const f64x2 v = __builtin_nontemporal_load(
-                    reinterpret_cast<gptr<const f64x2>>(base + static_cast<size_t>(k) * ldF + j));
-                ph[j] = v.x; ph[j + 1] = v.y;
-            }
-        }
-        const cptr<double> wk = hist + static_cast<size_t>(k) * TT;
-#pragma unroll
-        for (int t = 0; t < TT; ++t) {
-            const double w = wk[t];
-#pragma unroll
-            for (int j = 0; j < FPT; ++j) acc[j][t] = __builtin_fma(ph[j], w, acc[j][t]);
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < TT; ++t)
-#pragma unroll
-        for (int j = 0; j < FPT; ++j)
-            if (f0 + j < F) uh[static_cast<size_t>(t) * ldF + f0 + j] = static_cast<float>(acc[j][t]);
 }
 
 // ---------------------------------------------------------------------------
@@ -410,23 +352,6 @@ __global__ __launch_bounds__(256) void k_reconstruct_mfma_multi(const WindowScen
     const WindowScene& w = tab[scene_of_block(tab, n, blockIdx.x)];
     mfma_window_body<float, NT>(static_cast<const float*>(w.Phi), w.ldF, w.F, w.K, w.hist, W, W, w.uh, w.ldF,
                                 blockIdx.x - w.block0);
-}
-
-template <typename T, int FPT, int TT>
-__global__ __launch_bounds__(256) void k_reconstruct_window(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
-                                                            const double* __restrict__ hist,
-                                                            float* __restrict__ uh)
-{
-    window_body<T, FPT, TT>(Phi, ldF, F, K, hist, uh, blockIdx.x);
-}
-
-// several scenes' windows in one launch; scene j owns the blocks from
-// tab[j].block0 (ascending) to the next scene's
-template <typename T, int FPT, int TT>
-__global__ __launch_bounds__(256) void k_reconstruct_window_multi(const WindowScene* __restrict__ tab, int n)
-{
-    const WindowScene& w = tab[scene_of_block(tab, n, blockIdx.x)];
-    window_body<T, FPT, TT>(static_cast<const T*>(w.Phi), w.ldF, w.F, w.K, w.hist, w.uh, blockIdx.x - w.block0);
 }
 
 // out[k] = hist[k * ld + t]: one column of a window history
@@ -988,20 +913,20 @@ hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F,
     return hipGetLastError();
 }
 
-hipError_t launch_reconstruct_window(int prec, int window, const void* Phi, size_t ldF, uint32_t F, int K,
+hipError_t launch_reconstruct_window(int prec, int hist_ld, int ncols, const void* Phi, size_t ldF, uint32_t F, int K,
                                      const double* hist, float* uh, hipStream_t s)
 {
     if (ldF % 256 != 0) return hipErrorInvalidValue;
-    if (window != 2 && window != 4 && window != 8 && window != 16 && window != 32) return hipErrorInvalidValue;
+    if (ncols < 1 || ncols > 32 || hist_ld < ncols) return hipErrorInvalidValue;
     const unsigned grid = static_cast<unsigned>(ldF / 256);
     if (prec == PSF_PHI_FP32) {
         const float* P = static_cast<const float*>(Phi);
-        if (window == 32) k_reconstruct_mfma<float, 2><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, window, window, uh, ldF);
-        else k_reconstruct_mfma<float, 1><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, window, window, uh, ldF);
+        if (ncols > 16) k_reconstruct_mfma<float, 2><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
+        else k_reconstruct_mfma<float, 1><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
     } else {
         const double* P = static_cast<const double*>(Phi);
-        if (window == 32) k_reconstruct_mfma<double, 2><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, window, window, uh, ldF);
-        else k_reconstruct_mfma<double, 1><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, window, window, uh, ldF);
+        if (ncols > 16) k_reconstruct_mfma<double, 2><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
+        else k_reconstruct_mfma<double, 1><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
     }
     return hipGetLastError();
 }
@@ -1085,6 +1010,15 @@ hipError_t launch_fill_f64(double* p, int n, double v, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     k_fill_f64<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+// empty kernel that marks the ends of a timed region in kernel traces
+__global__ void k_timed_region_mark(int) {}
+
+hipError_t launch_timed_region_mark(int which, hipStream_t s)
+{
+    k_timed_region_mark<<<1, 64, 0, s>>>(which);
     return hipGetLastError();
 }
 

@@ -83,10 +83,10 @@ hipError_t launch_update(double* omega, const double* y, const double* visc, con
 // u = float(Phi w)
 hipError_t launch_reconstruct(int phi_precision, const void* Phi, size_t ldF, uint32_t F, int K,
                               const double* omega, float* u, hipStream_t s);
-// u_t = float(Phi w_t) for t < window (2, 4, 8, 16 or 32), w_t = hist[k * window + t],
+// u_t = float(Phi w_t) for t < ncols (1..32), w_t = hist[k * hist_ld + t],
 // u_t at uh + t * ldF; bitwise equal to launch_reconstruct for each w_t
-hipError_t launch_reconstruct_window(int phi_precision, int window, const void* Phi, size_t ldF, uint32_t F, int K,
-                                     const double* hist, float* uh, hipStream_t s);
+hipError_t launch_reconstruct_window(int phi_precision, int hist_ld, int ncols, const void* Phi, size_t ldF, uint32_t F,
+                                     int K, const double* hist, float* uh, hipStream_t s);
 // all scenes of d_tab (device memory, ascending block0) in one launch of
 // `blocks` blocks; window_blocks() gives one scene's block count
 hipError_t launch_reconstruct_window_multi(int phi_precision, int window, const WindowScene* d_tab, int n,
@@ -109,6 +109,8 @@ hipError_t launch_density_count(const DensityGrid& D, const float4* parts, uint3
 // volume[v] = half(float(counts[v]) / 1e6) and counts[v] = 0, for v < n2048 * 2048
 hipError_t launch_density_volume(uint32_t* counts, uint16_t* volume, size_t n2048, hipStream_t s);
 hipError_t launch_zero_int(int* p, int n, hipStream_t s);
+// k_timed_region_mark: an empty launch (which = 0 start, 1 end of a timed region)
+hipError_t launch_timed_region_mark(int which, hipStream_t s);
 hipError_t launch_fill_f64(double* p, int n, double v, hipStream_t s);
 
 }  // namespace psf

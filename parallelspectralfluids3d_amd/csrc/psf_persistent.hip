@@ -628,8 +628,12 @@ int persistent_upload_c(psf_scene* s, const double* C)
                 }
             }
         }
-    hipError_t e = hipMemcpy(p->blocks, host.data(), host.size() * 2, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(p->inv_scale, inv.data(), inv.size() * 8, hipMemcpyHostToDevice);
+    // on the context stream, after the steps queued there that read the blocks
+    // (a null-stream copy is not ordered against that non-blocking stream)
+    hipStream_t cs = s->ctx->stream;
+    hipError_t e = hipMemcpyAsync(p->blocks, host.data(), host.size() * 2, hipMemcpyHostToDevice, cs);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->inv_scale, inv.data(), inv.size() * 8, hipMemcpyHostToDevice, cs);
+    if (e == hipSuccess) e = hipStreamSynchronize(cs);
     if (e != hipSuccess) {
         s->ctx->err = std::string("persistent upload: ") + hipGetErrorString(e);
         return PSF_ERR_HIP;

@@ -317,6 +317,12 @@ class Scene:
         (1 = one frame at a time)."""
         self._c(self.ctx.lib.psf_set_frame_window(self.h, int(frames)), "set_frame_window")
 
+    def counters(self) -> dict:
+        """What the scene's calls have queued (psf_scene_get_counters)."""
+        c = L.SceneCounters()
+        self._c(self.ctx.lib.psf_scene_get_counters(self.h, ct.byref(c)), "get_counters")
+        return {name: int(getattr(c, name)) for name, _ in L.SceneCounters._fields_}
+
     def set_density_grid(self, dims, count_in_trace=True):
         """Particle density grid of dims = (rx, ry, rz) cells over the AABB
         ((0, 0, 0) removes it); count_in_trace makes every particle pass count

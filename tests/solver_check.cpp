@@ -1,15 +1,19 @@
-// solver_check: drives psf::HipSolver (include/psf/solver.hpp) the way the
-// reference's GLCanvas drives its solver: preprocessing fills the protected
-// state (here read from files), buildAdvection() uploads it, integrate() runs
-// once per frame, getters read the results back.
+// solver_check: test driver (tests/test_gpu_solver_cpp.py) of psf::HipSolver
+// (include/psf/solver.hpp), driven the way the reference's GLCanvas drives its
+// solver: preprocessing fills the protected state (here read from files),
+// buildAdvection() uploads it, integrate() runs once per frame, getters read
+// the results back.
 //
-//   solver_check <dir> [extras]
+//   solver_check <dir> [extras | window <W>]
 // <dir>/meta.txt: nx ny nz resolution K N frames dt viscosity gravity lifetime c_precision phi_precision
 // <dir>/{phi,lambda,gravity,C,omega}.bin (float64), parts.bin (float32 N x 4)
 // writes <dir>/{omega_out.bin (float64), u_out.bin, parts_out.bin (float32)}
 // With "extras" it then also writes the 16^3 density volume of the final
 // particles (<dir>/volume_out.bin, uint16 half bits) and runs the reference's
 // benchmark (startBenchmark, 4 recorded frames) into <dir>/benchmark.csv.
+// With "window W" the solver runs with a frame window of W and, like the
+// reference's canvas loop, sets the (unchanged) timestep before every
+// integrate(); it prints the scene's counters.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -136,11 +140,12 @@ int eigen_check()
 int main(int argc, char** argv)
 {
     if (argc == 2 && std::string(argv[1]) == "eigen") return eigen_check();
-    if (argc != 2 && argc != 3) {
-        std::fprintf(stderr, "usage: %s <dir> [extras] | eigen\n", argv[0]);
+    if (argc != 2 && argc != 3 && !(argc == 4 && std::string(argv[2]) == "window")) {
+        std::fprintf(stderr, "usage: %s <dir> [extras | window <W>] | eigen\n", argv[0]);
         return 2;
     }
     const bool extras = argc == 3 && std::string(argv[2]) == "extras";
+    const unsigned window = argc == 4 ? static_cast<unsigned>(std::stoul(argv[3])) : 1u;
     const std::string dir = argv[1];
     std::ifstream meta(dir + "/meta.txt");
     uint32_t nx, ny, nz;
@@ -159,10 +164,23 @@ int main(int argc, char** argv)
         solver.setViscosity(nu);
         solver.setGravityActive(grav != 0);
         solver.setLifeTime(lifetime / 60.0f);
-        for (unsigned f = 0; f < frames; ++f) solver.integrate();
+        if (window > 1) solver.setFrameWindow(window);
+        for (unsigned f = 0; f < frames; ++f) {
+            if (window > 1) solver.setTimestep(dt);   // the canvas loop's per-frame parameters
+            solver.integrate();
+        }
         write_bin(dir + "/omega_out.bin", solver.getBasisCoefficients());
         write_bin(dir + "/u_out.bin", solver.getVelocityField());
         write_bin(dir + "/parts_out.bin", solver.getParticles());
+        if (window > 1) {
+            const psf_scene_counters c = solver.counters();
+            std::printf("counters frames=%llu window_fills=%llu window_frames=%llu windows_dropped=%llu "
+                        "step_launches=%llu recon_launches=%llu particle_launches=%llu\n",
+                        (unsigned long long)c.frames, (unsigned long long)c.window_fills,
+                        (unsigned long long)c.window_frames, (unsigned long long)c.windows_dropped,
+                        (unsigned long long)c.step_launches, (unsigned long long)c.recon_launches,
+                        (unsigned long long)c.particle_launches);
+        }
         if (extras) {
             solver.enableDensity(16, 16, 16);
             write_bin(dir + "/volume_out.bin", solver.getDensityVolume());

@@ -13,7 +13,7 @@ from scenes import make_scene, particles_uniform
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "parallelspectralfluids3d_amd", "solver_check")
+EXE = os.path.join(ROOT, "tests", "_build", "solver_check")   # built by __graft_entry__.build()
 
 
 @pytest.mark.parametrize("cprec,phiprec,tol_w", [(0, 0, 1e-12), (4, 1, 2e-6)])
@@ -86,3 +86,40 @@ def test_eigensolve_from_cpp():
     r = subprocess.run([EXE, "eigen"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "eigen_check:" in r.stdout
+
+
+def test_hip_solver_frame_window(tmp_path):
+    """HipSolver with a frame window of 32, driven like the reference's canvas
+    (setTimestep, then integrate(), once per frame), gives the bits of the
+    same solver without a window and computes one window per 32 frames (plus
+    the one computed ahead), dropping none."""
+    print("[gpu-case] solver_check window 32", flush=True)
+    sd = make_scene(n=8, K=16)
+    N, frames, dt = 2048, 96, 0.01
+    p0 = particles_uniform(sd, N, seed=13)
+    outs = []
+    for W in (1, 32):
+        d = str(tmp_path / f"w{W}")
+        os.makedirs(d)
+        with open(os.path.join(d, "meta.txt"), "w") as fh:
+            fh.write(f"8 8 8 {sd.resolution!r} {sd.K} {N} {frames} {dt!r} 0.0 0 60.0 4 1\n")
+        np.ascontiguousarray(sd.Phi, np.float64).tofile(os.path.join(d, "phi.bin"))
+        sd.lam.astype(np.float64).tofile(os.path.join(d, "lambda.bin"))
+        sd.g.astype(np.float64).tofile(os.path.join(d, "gravity.bin"))
+        np.ascontiguousarray(sd.C, np.float64).tofile(os.path.join(d, "C.bin"))
+        sd.omega.astype(np.float64).tofile(os.path.join(d, "omega.bin"))
+        p0.astype(np.float32).tofile(os.path.join(d, "parts.bin"))
+        args = [EXE, d] + (["window", str(W)] if W > 1 else [])
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append((d, r.stdout))
+    for name in ("omega_out.bin", "u_out.bin", "parts_out.bin"):
+        a = open(os.path.join(outs[0][0], name), "rb").read()
+        b = open(os.path.join(outs[1][0], name), "rb").read()
+        assert a == b, name
+    line = [ln for ln in outs[1][1].splitlines() if ln.startswith("counters ")][0]
+    c = {k: int(v) for k, v in (kv.split("=") for kv in line.split()[1:])}
+    assert c["frames"] == frames
+    assert c["windows_dropped"] == 0
+    assert c["window_fills"] == frames // 32 + 1
+    assert c["step_launches"] == c["window_fills"] and c["recon_launches"] == c["window_fills"]

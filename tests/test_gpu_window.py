@@ -158,3 +158,85 @@ def test_window_rejects_bad_sizes(hip_ctx):
         with pytest.raises(L.PsfError) as e:
             win.set_frame_window(bad)
         assert e.value.status == L.PSF_ERR_ARG
+
+
+@pytest.mark.parametrize("window", [1, 8])
+def test_new_particles_after_frames_match_oracle(hip_ctx, window):
+    """integrate(n), a fresh particle set, integrate(m): the upload lands
+    after the passes already queued on the scene's stream, so the fresh set is
+    advanced from its own positions (the oracle advances it from frame n)."""
+    _say(f"set_particles after frames, window={window}")
+    from parallelspectralfluids3d_amd import hip
+    sd = make_scene(n=8, K=16)
+    N, n1, n2, dt = 1 << 18, 9, 6, 0.01
+    d = hip.make_desc(sd.grid, sd.resolution, sd.K, C_FP64, PHI_FP64, N, sd.aabb_min, sd.aabb_max)
+    s = hip.Scene(hip_ctx, d)
+    s.upload_basis(sd.Phi, sd.lam, sd.g)
+    s.upload_advection(sd.C)
+    s.set_omega(sd.omega)
+    s.set_params(dt, 0.0, False, lifetime=60.0)
+    s.set_particles(particles_uniform(sd, N, seed=21))
+    s.set_frame_window(window)
+    s.integrate(n1)
+    fresh = particles_uniform(sd, N, seed=22)
+    s.set_particles(fresh)
+    s.integrate(n2)
+    pg = s.get_particles()
+    m = O.Mesh((sd.grid.nx, sd.grid.ny, sd.grid.nz), sd.resolution, sd.aabb_min, inside=sd.inside)
+    w, p = sd.omega.copy(), fresh.copy()
+    for f in range(n1 + n2):
+        w = O.reduced_step(w, sd.C, sd.lam, sd.g, dt, 0.0, False)
+        if f >= n1:
+            u = O.reconstruct(sd.Phi, w)
+            p = O.particles(m, u, p, resolution=sd.resolution, timestep=dt, aabb_min=sd.aabb_min,
+                            aabb_max=sd.aabb_max, lifetime=60.0, seed=0, step=f)
+    assert np.abs(pg[:, :3] - p[:, :3]).max() <= 1e-6
+
+
+def test_bounded_frame_phase(hip_ctx):
+    """The bench's frame phase (psf_time_phase phase 3) starts at a window
+    boundary and computes exactly the frames it times: with a window of 32,
+    20 timed frames fill 20 frames (none ahead), and the results are the bits
+    of 20 single frames; a caller's frames afterwards continue as usual."""
+    _say("bounded frame phase")
+    from parallelspectralfluids3d_amd import _lib as L
+    sd = make_scene(n=8, K=16)
+    one, win = _pair(hip_ctx, sd, C_FP64, PHI_FP32, 1024, 32)
+    win.integrate(5)                      # 5 frames consumed, a window and the next computed ahead
+    one.integrate(5)
+    c0 = win.counters()
+    win.time_phase(L.PHASE_FRAME, 20)
+    one.integrate(20)
+    _same(one, win)
+    c1 = win.counters()
+    assert c1["windows_dropped"] - c0["windows_dropped"] == 1       # the frames ahead of frame 5
+    assert c1["window_frames"] - c0["window_frames"] == 20          # exactly the timed frames
+    assert c1["frames"] - c0["frames"] == 20
+    for s in (one, win):
+        s.integrate(45)
+    _same(one, win)
+    c2 = win.counters()
+    assert c2["windows_dropped"] == c1["windows_dropped"]
+    # ordinary frames: one window of 32 at the first frame and the next one ahead
+    assert c2["window_fills"] - c1["window_fills"] == 3
+
+
+def test_unchanged_params_keep_the_window(hip_ctx):
+    """setTimestep & co. with the values already set (the reference's
+    integrate() reads them every frame) do not drop the frames computed
+    ahead; a changed value does."""
+    _say("unchanged params keep the window")
+    sd = make_scene(n=8, K=16)
+    one, win = _pair(hip_ctx, sd, C_FP64, PHI_FP32, 512, 16)
+    for f in range(40):
+        for s in (one, win):
+            s.set_params(0.01, 0.0, False, lifetime=60.0)
+            s.integrate(1)
+    _same(one, win)
+    c = win.counters()
+    assert c["windows_dropped"] == 0 and c["window_fills"] == 4      # frames 0, 16, 32 (+1 ahead)
+    for s in (one, win):
+        s.set_params(0.02, 0.0, False, lifetime=60.0)
+        s.integrate(3)
+    _same(one, win)
+    assert win.counters()["windows_dropped"] == 1
