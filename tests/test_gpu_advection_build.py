@@ -1,13 +1,14 @@
 """C built on the device from any basis (psf_scene_build_advection) against
-the host restatement of buildAdvection (advection.build_advection, pinned to
-the oracle in test_advection.py), on obstacle scenes with eigensolved bases
-and on the analytic closed box (where the host uses the box-specific build),
-in FP64, FP32 and BF16 hi+lo; plus the argument errors."""
+the oracle's literal buildAdvection (O.build_advection, psfsolver.cpp:431-551)
+and the host restatement (advection.build_advection), on obstacle scenes with
+eigensolved bases and on the analytic closed box, in FP64, FP32 and BF16
+hi+lo; plus the argument errors."""
 import sys
 
 import numpy as np
 import pytest
 
+import oracle as O
 from scenes import make_scene
 
 pytestmark = pytest.mark.gpu
@@ -46,6 +47,10 @@ def test_fp64_matches_host(hip_ctx, case):
     s.close()
     scale = np.abs(sd.C).max()
     assert np.max(np.abs(C - sd.C)) <= 1e-12 * scale
+    # the oracle's literal construction (voxel loop, wedge slots, V^T W_i / lambda_i)
+    m = O.Mesh((sd.grid.nx, sd.grid.ny, sd.grid.nz), sd.resolution, sd.aabb_min, inside=sd.inside)
+    Co = O.build_advection(m, sd.Phi, sd.lam, sd.V)
+    assert np.max(np.abs(C - Co)) <= 1e-12 * np.abs(Co).max()
     # the reference's own structure: lambda_i C_i(k, i) = 0 (the diagonal slot of each C_i vanishes)
     i = np.arange(sd.K)
     assert np.max(np.abs(C[i, i, :])) <= 1e-12 * scale

@@ -1,11 +1,13 @@
 """Closed-box basis and C built on the device from the mode table, against the
-host constructions (basis.analytic_box_modes, advection.build_advection_box_*)
-and through whole frames against scenes built from host uploads."""
+host constructions (basis.analytic_box_modes, advection.build_advection_box_*),
+against the oracle's literal buildAdvection, and through whole frames against
+scenes built from host uploads."""
 import sys
 
 import numpy as np
 import pytest
 
+import oracle as O
 from parallelspectralfluids3d_amd import advection, basis
 from parallelspectralfluids3d_amd.grid import Grid
 
@@ -46,6 +48,28 @@ def test_device_basis_equals_host(hip_ctx, dims, K):
             err = np.abs(u.astype(np.float64) - ref.astype(np.float32).astype(np.float64)).max()
             assert err <= 2 * np.spacing(np.float32(np.abs(ref).max())), (phi_prec, k, err)
         s.close()
+
+
+@pytest.mark.parametrize("dims,K", [((10, 9, 8), 40), ((8, 8, 8), 32)])
+def test_device_box_advection_equals_oracle(hip_ctx, dims, K):
+    """FP64 C from psf_scene_build_box_advection (separable 1-D sums) against
+    the oracle's literal voxel loop (psfsolver.cpp:431-551) on the same modes."""
+    _say(f"device box C vs oracle {dims} K={K}")
+    from parallelspectralfluids3d_amd import hip
+    g = Grid(*dims)
+    n = max(dims)
+    h = 1.0 / n
+    Phi, lam, tab = basis.analytic_box_modes(g, K)
+    s = hip.Scene(hip_ctx, _desc(g, K, C_FP64, PHI_FP64))
+    s.generate_box_basis(tab)
+    s.build_box_advection(tab)
+    C = s.get_advection()
+    s.close()
+    V = (basis.curl_operator(g, h) @ Phi.T).T
+    amin = tuple(-0.5 * d / n for d in dims)
+    m = O.Mesh(dims, h, amin, inside=np.ones(g.nV, bool))
+    Co = O.build_advection(m, Phi, lam, V)
+    assert np.max(np.abs(C - Co)) <= 1e-12 * np.abs(Co).max()
 
 
 @pytest.mark.parametrize("c_prec,tol", [(C_FP64, 1e-13), (C_FP32, 1e-6), (C_BF16X2, 1e-7)])

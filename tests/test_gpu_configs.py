@@ -1,0 +1,326 @@
+"""The hot path at the benchmark configurations' own sizes, against the oracle
+(psfsolver.cpp:21-234 restated in oracle/psf_oracle.c).
+
+  config 2  32^3 box with a central sphere obstacle, K = 128 eigenbasis of
+            the pruned Laplacian, upward force on modes 0-7, FP32 C on the
+            plain path, FP32 Phi, 2000 integrate() frames at dt = 0.005,
+            2^18 particles in fluid cells; the oracle runs its own literal C
+  config 3  64^3 closed box, K = 256: BF16, FP16 and BF16 hi+lo contractions
+            (FP32 accumulation, FP64 update) for 1000 steps at dt = 0.01,
+            checked every 100 steps against the FP64 oracle at the north-star
+            budget of 1e-4 relative; the FP64 path to 1e-12; one particle pass
+            of a 2^20-particle scene, sampled, bit for bit
+  config 4  the serving loop on the stream's real grids {16, 24, 32, 48}^3
+            with K up to 192: every request gives the same bits alone and in
+            the mix, and sampled requests agree with the oracle
+  config 5  128^3 closed box, K = 512 (Phi 13 GB), 2^24 particles: the
+            size-independent checks (omega against the FP64 path, unit energy,
+            sampled u against a host FP64 sum, a sampled particle pass bit for
+            bit against the oracle)
+
+Each test states its tolerance; they run for tens of seconds each (the oracle
+on the host's cores)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from parallelspectralfluids3d_amd import advection, basis
+from parallelspectralfluids3d_amd.grid import Grid, box_mask, sphere_obstacle_mask
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+NTHREADS = min(16, os.cpu_count() or 1)
+C_FP64, C_FP32, C_BF16, C_FP16, C_BF16X2 = 0, 1, 2, 3, 4
+PHI_FP64, PHI_FP32 = 0, 1
+# relative omega error of the MFMA precisions against the FP64 oracle after
+# 1000 steps at 64^3/K=256 (the north star's budget)
+NORTH_STAR_BUDGET = 1e-4
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _fluid_particles(g, h, amin, inside, N, seed, lifetime):
+    rng = np.random.default_rng(seed)
+    x, y, z = g.voxel_coords()
+    vox = np.flatnonzero(inside)
+    pick = vox[rng.integers(0, vox.size, N)]
+    cell = np.stack([x[pick], y[pick], z[pick]], 1).astype(np.float64)
+    p = np.zeros((N, 4), np.float32)
+    p[:, :3] = np.asarray(amin) + (cell + rng.random((N, 3))) * h
+    p[:, 3] = lifetime
+    return p
+
+
+def test_config2_obstacle_fp32_2000_frames(hip_ctx):
+    from parallelspectralfluids3d_amd import _lib as L
+    from parallelspectralfluids3d_amd import hip
+    n, K, N, frames, dt = 32, 128, 1 << 18, 2000, 0.005
+    _say(f"config 2: {n}^3 sphere obstacle, K={K}, FP32 plain path, {frames} frames, {N} particles")
+    g = Grid(n, n, n)
+    h = 1.0 / n
+    amin, amax = (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)
+    inside = sphere_obstacle_mask(g, h, amin)
+    Phi, lam, _ = basis.laplacian_eigenbasis_device(g, h, inside, K, hip_ctx)
+    V = (basis.curl_operator(g, h) @ Phi.T).T
+    # C on the device (FP64 build, read back): the GPU scene's C
+    s64 = hip.Scene(hip_ctx, hip.make_desc(g, h, K, C_FP64, PHI_FP64, 0, amin, amax))
+    advection.build_advection_device(s64, g, h, inside, Phi, lam, V)
+    C_dev = s64.get_advection()
+    s64.close()
+    m = O.Mesh((n, n, n), h, amin, inside=inside)
+    # the oracle's own literal construction at full size
+    C_ora = O.build_advection(m, Phi, lam, V, nthreads=NTHREADS)
+    assert np.max(np.abs(C_dev - C_ora)) <= 1e-12 * np.abs(C_ora).max()
+    force = -basis.project(Phi, basis.gravity_field(g, inside))
+    force[8:] = 0.0
+    omega = basis.unit_energy_omega(K, 0)
+    parts = _fluid_particles(g, h, amin, inside, N, 21, 60.0)
+    s = hip.Scene(hip_ctx, hip.make_desc(g, h, K, C_FP32, PHI_FP32, N, amin, amax))
+    s.upload_basis(Phi, lam, force)
+    s.upload_advection(C_dev)
+    s.set_omega(omega)
+    s.set_params(dt, 0.0, True, lifetime=60.0)
+    s.set_particles(parts)
+    s.integrate(frames)
+    w_gpu, p_gpu = s.get_omega(), s.get_particles()
+    s.close()
+    w, p = omega.copy(), parts.copy()
+    for f in range(frames):
+        w = O.reduced_step(w, C_ora, lam, force, dt, 0.0, True, nthreads=NTHREADS)
+        u = O.reconstruct(Phi, w, nthreads=NTHREADS)
+        p = O.particles(m, u, p, resolution=h, timestep=dt, aabb_min=amin, aabb_max=amax, lifetime=60.0,
+                        seed=0, step=f, nthreads=NTHREADS)
+    rel = np.linalg.norm(w_gpu - w) / np.linalg.norm(w)
+    err = np.abs(p_gpu[:, :3] - p[:, :3]).max()
+    print(f"  omega rel err {rel:.3e}, particle max err {err:.3e}", flush=True)
+    # FP32 C and Phi against FP64 after 2000 frames with forcing
+    assert rel <= 1e-8, rel
+    assert err <= 1e-5, err
+    # lifetimes and respawns (counter-based draws, psfsolver.cpp:219-222) agree exactly
+    assert np.array_equal(p_gpu[:, 3], p[:, 3])
+
+
+@pytest.fixture(scope="module")
+def box64():
+    """64^3 closed box, K = 256 analytic modes, C built on the device in FP64
+    (checked against the oracle's literal build at smaller boxes in
+    test_gpu_box.py) and read back."""
+    from parallelspectralfluids3d_amd import hip
+    n, K = 64, 256
+    g = Grid(n, n, n)
+    h = 1.0 / n
+    tab = basis.box_mode_table_normalised(g, K)
+    lam = tab[:, 0].copy()
+    ctx = hip.Context(0)
+    s = hip.Scene(ctx, hip.make_desc(g, h, K, C_FP64, PHI_FP32, 0))
+    s.upload_basis(None, lam)
+    s.build_box_advection(tab)
+    C = s.get_advection()
+    s.close()
+    ctx.close()
+    return dict(g=g, h=h, K=K, tab=tab, lam=lam, C=C, omega=basis.unit_energy_omega(K, 0))
+
+
+def test_config3_precisions_1000_steps(hip_ctx, box64):
+    from parallelspectralfluids3d_amd import hip
+    b = box64
+    g, h, K = b["g"], b["h"], b["K"]
+    _say(f"config 3: 64^3 box, K={K}, 1000 steps, omega every 100 vs the oracle")
+    zero = np.zeros(K)
+    ora = []
+    w = b["omega"].copy()
+    for blk in range(10):
+        for _ in range(100):
+            w = O.reduced_step(w, b["C"], b["lam"], zero, 0.01, 0.0, False, nthreads=NTHREADS)
+        ora.append(w.copy())
+    errs = {}
+    for prec in (C_FP64, C_BF16X2, C_BF16, C_FP16):
+        s = hip.Scene(hip_ctx, hip.make_desc(g, h, K, prec, PHI_FP32, 0))
+        s.upload_basis(None, b["lam"])
+        s.upload_advection(b["C"])
+        s.set_omega(b["omega"])
+        s.set_params(0.01, 0.0, False)
+        e = []
+        for blk in range(10):
+            s.step(100)
+            wg = s.get_omega()
+            e.append(float(np.linalg.norm(wg - ora[blk]) / np.linalg.norm(ora[blk])))
+        s.close()
+        errs[prec] = e
+        print(f"  c_precision {prec}: rel err every 100 steps {['%.2e' % x for x in e]}", flush=True)
+    assert max(errs[C_FP64]) <= 1e-12
+    for prec in (C_BF16X2, C_BF16, C_FP16):
+        assert max(errs[prec]) <= NORTH_STAR_BUDGET, (prec, errs[prec])
+
+
+def test_config3_particle_pass_bit_exact(hip_ctx, box64):
+    """2^20 particles through 40 windowed frames (BF16 hi+lo, FP32 Phi built on
+    the device), then one particle pass of 4096 sampled particles against the
+    oracle's particle loop fed the GPU's u: bit for bit."""
+    from parallelspectralfluids3d_amd import _lib as L
+    from parallelspectralfluids3d_amd import hip
+    b = box64
+    g, h, K, N, dt = b["g"], b["h"], b["K"], 1 << 20, 0.01
+    _say("config 3: 64^3 particle pass, sampled, bit-exact")
+    rng = np.random.default_rng(3)
+    parts = np.zeros((N, 4), np.float32)
+    parts[:, :3] = rng.random((N, 3), dtype=np.float32) - np.float32(0.5)
+    parts[:, 3] = np.inf
+    s = hip.Scene(hip_ctx, hip.make_desc(g, h, K, C_BF16X2, PHI_FP32, N))
+    s.generate_box_basis(b["tab"])
+    s.build_box_advection(b["tab"])
+    s.set_omega(b["omega"])
+    s.set_params(dt, 0.0, False, lifetime=60.0)
+    s.set_particles(parts)
+    s.set_frame_window(32)
+    s.integrate(40)
+    p_now = s.get_particles()
+    u = s.get_velocity()
+    frame = 40
+    sample = p_now[rng.choice(N, 4096, replace=False)].copy()
+    # a few particles exactly on voxel and half-voxel planes
+    sample[:64, 0] = (np.arange(64, dtype=np.float32) - np.float32(32)) / np.float32(64)
+    sample[64:128, 1] = (np.arange(64, dtype=np.float32) - np.float32(31.5)) / np.float32(64)
+    s.set_particles(sample)
+    s.trace()
+    p_gpu = s.get_particles()
+    s.close()
+    m = O.Mesh((64, 64, 64), h, (-0.5, -0.5, -0.5), inside=box_mask(g))
+    p_ora = O.particles(m, u, sample, resolution=h, timestep=dt, aabb_min=(-.5, -.5, -.5), aabb_max=(.5, .5, .5),
+                        lifetime=60.0, seed=0, step=frame)
+    assert np.array_equal(p_gpu.view(np.uint32), p_ora.view(np.uint32))
+
+
+SERVE_GRIDS = (16, 24, 32, 48)
+SERVE_K0 = 192
+
+
+def test_config4_serving_real_grids(hip_ctx):
+    """Requests on the serving stream's four grids with K from 16 to 192: each
+    gives the same bits alone and in the mix; two sampled requests (16^3/K=16
+    and 48^3/K=192) agree with the oracle run frame by frame on the first K
+    modes of the basis (BF16 hi+lo C and FP32 Phi against FP64)."""
+    from parallelspectralfluids3d_amd import hip
+    _say("config 4: serving on {16,24,32,48}^3, K up to 192")
+    bases = {}
+    for n in SERVE_GRIDS:
+        g = Grid(n, n, n)
+        h = 1.0 / n
+        Phi, lam, tab = basis.analytic_box_modes(g, SERVE_K0)
+        V = (basis.curl_operator(g, h) @ Phi.T).T
+        C = advection.build_advection_box_analytic(g, h, Phi, lam, tab, V)
+        gv = basis.project(Phi, basis.gravity_field(g, box_mask(g)))
+        bases[n] = dict(g=g, h=h, Phi=Phi, lam=lam, C=C, gv=gv)
+    rng = np.random.default_rng(4)
+    reqs = []
+    for n, K, frames in ((16, 16, 150), (24, 100, 230), (32, 192, 120), (48, 192, 100), (48, 57, 300),
+                         (16, 192, 64), (32, 33, 200), (24, 150, 90)):
+        w = rng.standard_normal(K)
+        w /= np.linalg.norm(w)
+        p = np.zeros((1 << 14, 4), np.float32)
+        p[:, :3] = rng.random((1 << 14, 3)) - 0.5
+        p[:, 3] = 40.0
+        reqs.append((n, K, frames, w, p))
+
+    sv = hip.Server(hip_ctx, window=32, max_active=64)
+    bid = {n: sv.add_basis(b["g"], b["h"], b["Phi"], b["lam"], b["gv"], b["C"]) for n, b in bases.items()}
+
+    def run(sel):
+        ids = [sv.submit(bid[reqs[j][0]], reqs[j][3], reqs[j][4], reqs[j][2], dt=0.01, lifetime=50.0) for j in sel]
+        guard = 0
+        while not all(sv.done(i) for i in ids):
+            sv.round()
+            guard += 1
+            assert guard < 100000
+        return [sv.result(i) for i in ids]
+
+    mix = run(range(len(reqs)))
+    for j in range(len(reqs)):       # each alone: nothing else on the server
+        alone = run([j])[0]
+        assert np.array_equal(alone[0].view(np.uint64), mix[j][0].view(np.uint64)), j
+        assert np.array_equal(alone[1].view(np.uint32), mix[j][1].view(np.uint32)), j
+    assert sv.stats()["failed"] == 0
+    sv.close()
+    for j in (0, 3):
+        n, K, frames, w0, p0 = reqs[j]
+        b = bases[n]
+        m = O.Mesh((n, n, n), b["h"], (-0.5, -0.5, -0.5), inside=box_mask(b["g"]))
+        C = np.ascontiguousarray(b["C"][:K, :K, :K])
+        Phi = np.ascontiguousarray(b["Phi"][:K])
+        w, p = w0.copy(), p0.copy()
+        for f in range(frames):
+            w = O.reduced_step(w, C, b["lam"][:K], b["gv"][:K], 0.01, 0.0, False, nthreads=NTHREADS)
+            u = O.reconstruct(Phi, w, nthreads=NTHREADS)
+            p = O.particles(m, u, p, resolution=b["h"], timestep=0.01, aabb_min=(-.5, -.5, -.5),
+                            aabb_max=(.5, .5, .5), lifetime=50.0, seed=0, step=f, nthreads=NTHREADS)
+        wg, pg, _ = mix[j]
+        rel = np.linalg.norm(wg - w) / np.linalg.norm(w)
+        err = np.abs(pg[:, :3] - p[:, :3]).max()
+        print(f"  request {j} ({n}^3, K={K}): omega rel err {rel:.2e}, particle max err {err:.2e}", flush=True)
+        assert rel <= 1e-6, (j, rel)
+        assert err <= 1e-5, (j, err)
+        assert np.array_equal(pg[:, 3], p[:, 3]), j
+
+
+def test_config5_large_scene(hip_ctx):
+    """128^3, K = 512 analytic modes built on the device (Phi 12.99 GB in
+    FP32), power-law omega, 2^24 particles, 500 frames at dt = 0.005 with a
+    frame window of 32.  omega within 1e-8 of the FP64 plain path and of unit
+    norm to 1e-12; u on 4096 sampled faces equal to a host FP64 sum over the
+    FP32-rounded Phi (to the float rounding of the result); one particle
+    pass of 4096 sampled particles bit for bit against the oracle with the
+    GPU's u."""
+    from parallelspectralfluids3d_amd import hip
+    n, K, N, frames, dt = 128, 512, 1 << 24, 500, 0.005
+    _say(f"config 5: {n}^3, K={K}, {N} particles, {frames} frames")
+    g = Grid(n, n, n)
+    h = 1.0 / n
+    tab = basis.box_mode_table_normalised(g, K)
+    lam = tab[:, 0].copy()
+    omega = basis.power_law_omega(K, 0)
+    rng = np.random.default_rng(1)
+    parts = np.empty((N, 4), np.float32)
+    parts[:, :3] = rng.random((N, 3), dtype=np.float32) - np.float32(0.5)
+    parts[:, 3] = np.inf
+    s = hip.Scene(hip_ctx, hip.make_desc(g, h, K, C_BF16X2, PHI_FP32, N))
+    s.generate_box_basis(tab)
+    s.build_box_advection(tab)
+    s.set_omega(omega)
+    s.set_params(dt, 0.0, False, lifetime=60.0)
+    s.set_particles(parts)
+    del parts
+    s.set_frame_window(32)
+    s.integrate(frames)
+    w_gpu = s.get_omega()
+    u_gpu = s.get_velocity()
+    ref = hip.Scene(hip_ctx, hip.make_desc(g, h, K, C_FP64, PHI_FP32, 0))
+    ref.upload_basis(None, lam)
+    ref.build_box_advection(tab)
+    ref.set_omega(omega)
+    ref.set_params(dt, 0.0, False)
+    ref.step(frames)
+    w_ref = ref.get_omega()
+    ref.close()
+    rel = np.linalg.norm(w_gpu - w_ref) / np.linalg.norm(w_ref)
+    print(f"  omega rel err vs FP64 path {rel:.2e}", flush=True)
+    assert rel <= 1e-8, rel
+    assert abs(np.linalg.norm(w_gpu) - 1.0) <= 1e-12
+    fs = rng.choice(g.nF, 4096, replace=False)
+    u_host = (basis.box_mode_values(g, tab, fs).astype(np.float32).astype(np.float64).T @ w_gpu)
+    assert np.all(np.abs(u_gpu[fs] - u_host.astype(np.float32)) <= np.spacing(np.abs(u_host).astype(np.float32)))
+    p_now = s.get_particles()
+    sample = p_now[rng.choice(N, 4096, replace=False)].copy()
+    del p_now
+    s.set_particles(sample)
+    s.trace()
+    p_gpu = s.get_particles()
+    s.close()
+    m = O.Mesh((n, n, n), h, (-0.5, -0.5, -0.5), inside=np.ones(g.nV, bool))
+    p_ora = O.particles(m, u_gpu, sample, resolution=h, timestep=dt, aabb_min=(-.5, -.5, -.5),
+                        aabb_max=(.5, .5, .5), lifetime=60.0, seed=0, step=frames)
+    assert np.array_equal(p_gpu.view(np.uint32), p_ora.view(np.uint32))

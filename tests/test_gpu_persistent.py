@@ -12,8 +12,9 @@ from scenes import make_scene
 pytestmark = pytest.mark.gpu
 
 C_BF16, C_FP16, C_BF16X2 = 2, 3, 4
-# relative omega error vs FP64 after 1000 steps at dt = 0.01
-BUDGET = {C_BF16: 2e-4, C_FP16: 2e-5, C_BF16X2: 2e-6}
+# relative omega error vs FP64 after 1000 steps at dt = 0.01; BF16 at the
+# north star's 1e-4 (the 64^3/K=256 case is in test_gpu_configs.py)
+BUDGET = {C_BF16: 1e-4, C_FP16: 2e-5, C_BF16X2: 2e-6}
 
 
 def _say(msg):
