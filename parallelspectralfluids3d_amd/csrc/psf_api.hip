@@ -572,7 +572,7 @@ int psf::scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOp
     chk(salloc(s, &s->dev.lambda, K));
     chk(salloc(s, &s->dev.gravity, K));
     chk(salloc(s, &s->dev.visc, K));
-    chk(salloc(s, &s->dev.y, K));
+    chk(salloc(s, &s->dev.y, K * static_cast<size_t>(plain_splits(static_cast<int>(K)))));   // plain-path partials
     chk(salloc(s, &s->dev.status, 64));
     chk(salloc(s, reinterpret_cast<unsigned char**>(&s->dev.C), s->dev.C_bytes));
     if (opt.borrowed_phi) {

@@ -20,7 +20,7 @@ struct DeviceScene {
     double* lambda = nullptr;    // K
     double* gravity = nullptr;   // K
     double* visc = nullptr;      // K, exp(nu * lambda_k * dt)
-    double* y = nullptr;         // K, contraction result of the last step
+    double* y = nullptr;         // K * plain_splits(K): the plain contraction's partials of y
     int* status = nullptr;       // device status word (1 = non-finite energy)
     // structure tensor
     void* C = nullptr;           // layout and type depend on c_precision

@@ -1,8 +1,8 @@
 // HIP kernels of the reduced-space fluid step for gfx950 (MI355X).
 //
-//   k_contract_plain   y_i = w^T C_i w      psfsolver.cpp:39-44
-//   k_update           FP64 energy renormalisation, viscosity, forcing
-//                                           psfsolver.cpp:34-38, 45-59
+//   k_contract_plain   partials of y_i = w^T C_i w     psfsolver.cpp:39-44
+//   k_update           their sums, then the FP64 energy renormalisation,
+//                      viscosity and forcing      psfsolver.cpp:34-38, 45-59
 //   k_reconstruct      u = float(Phi w)     psfsolver.cpp:60
 //   k_trace            particle pass        psfsolver.cpp:70-232
 // (paths relative to the reference repository root)
@@ -48,78 +48,130 @@ __device__ __forceinline__ double block_sum256(double v, double* red)
 
 // ---------------------------------------------------------------------------
 // Plain contraction.  C holds K matrices K x K, each column-major (element
-// (a, b) of C_i at (i*K + b)*K + a), in FP64 or FP32.  One block per mode i;
-// wave w reads the columns b = w, w+4, ... with coalesced VEC-wide loads along
-// a, and accumulates w_b * sum_a w_a C_i(a, b).
+// (a, b) of C_i at (i*K + b)*K + a), in FP64 or FP32.  The columns of each
+// mode are cut into chunks of kPlainCols: block (i, s) handles columns
+// b = 32 s .. 32 s + 31 of C_i, wave w the columns b = 32 s + w + 4 j, and
+// writes one FP64 partial sum_b w_b sum_a w_a C_i(a, b) to part[i * S + s];
+// k_update adds a mode's S partials in order.  With K * S blocks the launch
+// fills the chip (2048 blocks at K = 256), and every lane issues the loads
+// of all its columns before the first product, so each wave keeps 8 (FP32)
+// or 16 (FP64) vector loads in flight.  Products and the per-column sums
+// over a are in T; the column sums enter an FP64 accumulator.
+// Measured at K = 256 in FP32: 10.4-12.5 us per launch (5.4-6.5 TB/s) and
+// 14.7 us per step with k_update.  Folding the update into the same launch
+// (the last block to finish, found through a device-scope counter, sums the
+// partials) took 66 us per step: each block's release fence before counting
+// writes back its XCD's L2.
+constexpr int kPlainCols = 32;
+
+int plain_splits(int K) { return (K + kPlainCols - 1) / kPlainCols; }
+
 template <typename T, int VEC>
 __global__ __launch_bounds__(256) void k_contract_plain(const T* __restrict__ C, const double* __restrict__ omega,
-                                                        double* __restrict__ y, int K)
+                                                        double* __restrict__ part, int K, int S)
 {
+    typedef T vec_t __attribute__((ext_vector_type(VEC)));
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* sw = reinterpret_cast<T*>(smem_raw);
     __shared__ double red[4];
     for (int k = threadIdx.x; k < K; k += 256) sw[k] = static_cast<T>(omega[k]);
     __syncthreads();
-    const int i = blockIdx.x;
+    const int i = blockIdx.x / S, sc = blockIdx.x - i * S;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+    constexpr int NJ = kPlainCols / 4;            // columns per wave
+    const int b0 = sc * kPlainCols + wv;
     const T* Ci = C + static_cast<size_t>(i) * K * K;
-    T acc = 0;
-#pragma unroll 4
-    for (int b = wv; b < K; b += 4) {
-        const T* col = Ci + static_cast<size_t>(b) * K;
-        T part = 0;
-        for (int a0 = lane * VEC; a0 < K; a0 += kWave * VEC) {
-            if constexpr (VEC == 4) {
-                const float4 c = *reinterpret_cast<const float4*>(col + a0);
-                part += sw[a0] * c.x + sw[a0 + 1] * c.y + sw[a0 + 2] * c.z + sw[a0 + 3] * c.w;
-            } else if constexpr (VEC == 2) {
-                const double2 c = *reinterpret_cast<const double2*>(col + a0);
-                part += sw[a0] * c.x + sw[a0 + 1] * c.y;
-            } else {
-                part += sw[a0] * col[a0];
-            }
+    T colsum[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) colsum[j] = 0;
+    for (int a0 = lane * VEC; a0 < K; a0 += kWave * VEC) {
+        vec_t cv[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int b = b0 + 4 * j;
+            cv[j] = b < K ? *reinterpret_cast<const vec_t*>(Ci + static_cast<size_t>(b) * K + a0) : vec_t{};
         }
-        acc += sw[b] * part;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            T t = 0;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) t += sw[a0 + e] * cv[j][e];
+            colsum[j] += t;
+        }
     }
-    const double s = block_sum256(static_cast<double>(acc), red);
-    if (threadIdx.x == 0) y[i] = s;
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int b = b0 + 4 * j;
+        if (b < K) acc += static_cast<double>(sw[b]) * static_cast<double>(colsum[j]);
+    }
+    const double sum = block_sum256(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = sum;
 }
 
-// FP64 update, one block (psfsolver.cpp:34-59):
+// FP64 update, one block (psfsolver.cpp:34-59): y_k = sum of the S partials
+// part[k * S + s] in order, then
 //   e1 = w.w; w += dt*y; e2 = w.w; w *= sqrt(e1/e2); w_k *= exp(nu lambda_k dt);
 //   w += dt*g if forcing.
 // A zero or non-finite energy sets *status and leaves w unchanged.
 // With hist != nullptr the new w is also stored at hist[k * hist_ld + hist_t]
 // (one column of a frame window).
-__global__ __launch_bounds__(256) void k_update(double* __restrict__ omega, const double* __restrict__ y,
+// Thread t owns modes t, t + 256, ... (R of them, K <= 256 R) and loads all of
+// their inputs before the first reduction, so the kernel pays one memory
+// round trip; e1 and e2 are reduced together (one barrier).
+template <int R>
+__global__ __launch_bounds__(256) void k_update(double* __restrict__ omega, const double* __restrict__ part, int S,
                                                 const double* __restrict__ visc, const double* __restrict__ g,
                                                 int K, double dt, int gravity_active, int* __restrict__ status,
                                                 double* __restrict__ hist, int hist_ld, int hist_t)
 {
 #pragma clang fp contract(off)
-    extern __shared__ double swn[];
-    __shared__ double red[4];
-    double s1 = 0.0;
-    for (int k = threadIdx.x; k < K; k += 256) s1 += omega[k] * omega[k];
-    const double e1 = block_sum256(s1, red);
-    double s2 = 0.0;
-    for (int k = threadIdx.x; k < K; k += 256) {
-        const double wn = omega[k] + dt * y[k];
-        swn[k] = wn;
-        s2 += wn * wn;
+    __shared__ double red[8];
+    const int tid = threadIdx.x;
+    double w[R], wn[R], vi[R], gi[R];
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k = tid + 256 * r;
+        w[r] = 0.0; wn[r] = 0.0; vi[r] = 1.0; gi[r] = 0.0;
+        if (k < K) {
+            double yk = 0.0;
+            for (int q = 0; q < S; ++q) yk += part[static_cast<size_t>(k) * S + q];
+            w[r] = omega[k];
+            vi[r] = visc[k];
+            gi[r] = g[k];
+            wn[r] = w[r] + dt * yk;
+        }
     }
-    const double e2 = block_sum256(s2, red);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        s1 += w[r] * w[r];
+        s2 += wn[r] * wn[r];
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    const int lane = tid & (kWave - 1), wv = tid >> 6;
+    if (lane == 0) {
+        red[wv] = s1;
+        red[4 + wv] = s2;
+    }
+    __syncthreads();
+    const double e1 = ((red[0] + red[1]) + red[2]) + red[3];
+    const double e2 = ((red[4] + red[5]) + red[6]) + red[7];
     if (!(e2 > 0.0) || !__builtin_isfinite(e2) || !__builtin_isfinite(e1)) {
-        if (threadIdx.x == 0) *status = 1;
+        if (tid == 0) *status = 1;
         return;
     }
     const double fac = sqrt(e1 / e2);
-    for (int k = threadIdx.x; k < K; k += 256) {
-        double w = swn[k] * fac;
-        w = w * visc[k];
-        if (gravity_active) w = w + dt * g[k];
-        omega[k] = w;
-        if (hist) hist[static_cast<size_t>(k) * hist_ld + hist_t] = w;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k = tid + 256 * r;
+        if (k >= K) break;
+        double v = wn[r] * fac;
+        v = v * vi[r];
+        if (gravity_active) v = v + dt * gi[r];
+        omega[k] = v;
+        if (hist) hist[static_cast<size_t>(k) * hist_ld + hist_t] = v;
     }
 }
 
@@ -871,34 +923,43 @@ __global__ void k_zero_int(int* p, int n)
 }
 
 // ---------------------------------------------------------------------------
-hipError_t launch_contract_plain(int prec, const void* C, const double* omega, double* y, int K, hipStream_t s)
+hipError_t launch_contract_plain(int prec, const void* C, const double* omega, double* part, int K, hipStream_t s)
 {
+    const int S = plain_splits(K);
+    const unsigned blocks = static_cast<unsigned>(K) * S;
     if (prec == PSF_C_FP64) {
         const size_t sh = sizeof(double) * K;
-        if (K % 2 == 0)
-            k_contract_plain<double, 2><<<K, 256, sh, s>>>(static_cast<const double*>(C), omega, y, K);
+        if (K % 128 == 0)
+            k_contract_plain<double, 2><<<blocks, 256, sh, s>>>(static_cast<const double*>(C), omega, part, K, S);
         else
-            k_contract_plain<double, 1><<<K, 256, sh, s>>>(static_cast<const double*>(C), omega, y, K);
+            k_contract_plain<double, 1><<<blocks, 256, sh, s>>>(static_cast<const double*>(C), omega, part, K, S);
     } else {
         const size_t sh = sizeof(float) * K;
-        if (K % 4 == 0)
-            k_contract_plain<float, 4><<<K, 256, sh, s>>>(static_cast<const float*>(C), omega, y, K);
+        if (K % 256 == 0)
+            k_contract_plain<float, 4><<<blocks, 256, sh, s>>>(static_cast<const float*>(C), omega, part, K, S);
+        else if (K % 128 == 0)
+            k_contract_plain<float, 2><<<blocks, 256, sh, s>>>(static_cast<const float*>(C), omega, part, K, S);
         else
-            k_contract_plain<float, 1><<<K, 256, sh, s>>>(static_cast<const float*>(C), omega, y, K);
+            k_contract_plain<float, 1><<<blocks, 256, sh, s>>>(static_cast<const float*>(C), omega, part, K, S);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_update(double* omega, const double* y, const double* visc, const double* g, int K, double dt,
+hipError_t launch_update(double* omega, const double* part, const double* visc, const double* g, int K, double dt,
                          int gravity_active, int* status, double* hist, int hist_ld, int hist_t, hipStream_t s)
 {
-    k_update<<<1, 256, sizeof(double) * K, s>>>(omega, y, visc, g, K, dt, gravity_active, status, hist, hist_ld,
-                                                 hist_t);
+    const int S = plain_splits(K);
+    if (K <= 256)
+        k_update<1><<<1, 256, 0, s>>>(omega, part, S, visc, g, K, dt, gravity_active, status, hist, hist_ld, hist_t);
+    else if (K <= 512)
+        k_update<2><<<1, 256, 0, s>>>(omega, part, S, visc, g, K, dt, gravity_active, status, hist, hist_ld, hist_t);
+    else if (K <= 1024)
+        k_update<4><<<1, 256, 0, s>>>(omega, part, S, visc, g, K, dt, gravity_active, status, hist, hist_ld, hist_t);
+    else
+        k_update<16><<<1, 256, 0, s>>>(omega, part, S, visc, g, K, dt, gravity_active, status, hist, hist_ld, hist_t);
     return hipGetLastError();
 }
 
-// W = 1: the GEMV (HBM-bound; 0.83 of peak against 0.76 for one MFMA column
-// tile); the window kernels give the same bits
 hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F, int K, const double* omega,
                               float* u, hipStream_t s)
 {

@@ -72,12 +72,14 @@ struct TraceScene {
     uint32_t block0;
 };
 
-// reduced step, plain path: y_i = w^T C_i w (one block per mode i), then the
-// FP64 update of w (one block)
-hipError_t launch_contract_plain(int c_precision, const void* C, const double* omega, double* y, int K,
+// reduced step, plain path: partials of y_i = w^T C_i w over column chunks
+// (plain_splits(K) per mode, K * plain_splits(K) doubles in part), then the
+// FP64 update of w (one block), which sums them
+int plain_splits(int K);
+hipError_t launch_contract_plain(int c_precision, const void* C, const double* omega, double* part, int K,
                                  hipStream_t s);
 // hist != nullptr also stores the new w at hist[k * hist_ld + hist_t]
-hipError_t launch_update(double* omega, const double* y, const double* visc, const double* gravity, int K,
+hipError_t launch_update(double* omega, const double* part, const double* visc, const double* gravity, int K,
                          double dt, int gravity_active, int* status, double* hist, int hist_ld, int hist_t,
                          hipStream_t s);
 // u = float(Phi w)
