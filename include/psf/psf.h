@@ -150,6 +150,13 @@ PSF_API int psf_scene_build_box_advection(psf_scene* scene, const int32_t* freqs
    AbstractSolver::lifeTime (frames). */
 PSF_API int psf_set_params(psf_scene* scene, double dt, double viscosity, int gravity_active, float lifetime);
 PSF_API int psf_set_step_path(psf_scene* scene, int path);
+/* Watchdog of the persistent path (no counterpart in the reference): every
+   wait of a workgroup for the others' y in the device-wide exchange gives up
+   after `seconds` (default 2 s; 0 restores it), every workgroup then leaves
+   the launch, and the next call that reads results returns PSF_ERR_HIP
+   ("exchange timed out") instead of hanging.  omega keeps the last step all
+   workgroups completed.  PSF_ERR_UNSUPPORTED for plain-path scenes. */
+PSF_API int psf_set_wait_limit(psf_scene* scene, double seconds);
 
 /* basisCoeff (AbstractSolver::basisCoeff) */
 PSF_API int psf_set_omega(psf_scene* scene, const double* omega);

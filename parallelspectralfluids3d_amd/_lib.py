@@ -101,6 +101,7 @@ def load():
         "psf_density": (i32, [vp, vp, vp]),
         "psf_set_params": (i32, [vp, dbl, dbl, i32, flt]),
         "psf_set_step_path": (i32, [vp, i32]),
+        "psf_set_wait_limit": (i32, [vp, dbl]),
         "psf_set_omega": (i32, [vp, vp]),
         "psf_get_omega": (i32, [vp, vp]),
         "psf_set_particles": (i32, [vp, vp, u32]),

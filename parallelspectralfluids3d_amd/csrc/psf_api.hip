@@ -1,6 +1,7 @@
 // C ABI of the psf HIP library (declared in include/psf/psf.h).
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
@@ -9,6 +10,8 @@
 #include <vector>
 
 #include <map>
+#include <mutex>
+#include <set>
 #include <unordered_map>
 
 #include "psf_internal.hpp"
@@ -464,6 +467,39 @@ int ctx_set_error(psf_ctx* ctx, int code, const std::string& msg)
 
 
 
+namespace {
+
+// Contexts alive in the process.  The first psf_ctx_create registers, with
+// std::atexit, a handler that destroys the contexts a caller leaves alive at
+// exit.  It registers right after the HIP runtime has answered, i.e. after
+// the runtime's own initialisation registered its teardown, and exit runs
+// handlers and static destructors in reverse order of registration: so the
+// contexts' streams, events and memory are released while the runtime still
+// holds them.  (A Python caller's own atexit hook, which runs earlier still,
+// usually leaves nothing for it.)  Both containers are never destroyed, so
+// no static destructor of this library runs after the runtime's teardown.
+std::mutex& live_mutex()
+{
+    static std::mutex* m = new std::mutex();
+    return *m;
+}
+std::set<psf_ctx*>& live_contexts()
+{
+    static std::set<psf_ctx*>* s = new std::set<psf_ctx*>();
+    return *s;
+}
+void release_live_contexts()
+{
+    std::vector<psf_ctx*> left;
+    {
+        std::lock_guard<std::mutex> g(live_mutex());
+        left.assign(live_contexts().begin(), live_contexts().end());
+    }
+    for (psf_ctx* c : left) psf_ctx_destroy(c);
+}
+
+}  // namespace
+
 extern "C" {
 
 int psf_abi_version(void) { return PSF_ABI_VERSION; }
@@ -482,6 +518,12 @@ int psf_ctx_create(int device, psf_ctx** out)
         delete c;
         return PSF_ERR_HIP;
     }
+    static std::once_flag registered;
+    std::call_once(registered, [] { std::atexit(release_live_contexts); });
+    {
+        std::lock_guard<std::mutex> g(live_mutex());
+        live_contexts().insert(c);
+    }
     *out = c;
     return PSF_OK;
 }
@@ -489,6 +531,10 @@ int psf_ctx_create(int device, psf_ctx** out)
 int psf_ctx_destroy(psf_ctx* ctx)
 {
     if (!ctx) return PSF_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> g(live_mutex());
+        live_contexts().erase(ctx);
+    }
     (void)hipSetDevice(ctx->device);
     while (!ctx->servers.empty()) psf_server_destroy(*ctx->servers.begin());
     while (!ctx->scenes.empty()) psf_scene_destroy(*ctx->scenes.begin());
@@ -776,6 +822,16 @@ int psf_set_step_path(psf_scene* s, int path)
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
     if (int rc0 = window_flush(s)) return rc0;
     s->path = path;
+    return PSF_OK;
+}
+
+int psf_set_wait_limit(psf_scene* s, double seconds)
+{
+    if (!s) return PSF_ERR_ARG;
+    if (!(seconds >= 0.0) || seconds > 3600.0) return set_err(s->ctx, PSF_ERR_ARG, "wait limit must be in [0, 3600] s");
+    const uint64_t ticks = seconds == 0.0 ? 0ull : std::max<uint64_t>(1ull, static_cast<uint64_t>(seconds * 1e8));
+    if (int rc = persistent_set_wait_ticks(s, ticks))
+        return set_err(s->ctx, rc, "the scene has no persistent path (c_precision %d)", s->d.c_precision);
     return PSF_OK;
 }
 

@@ -72,6 +72,7 @@ struct PersistentState {
     double* inv_scale = nullptr; // [G * ipw]
     unsigned long long* xbuf = nullptr;   // [2][K][kXStride], words 0 and 1 used
     uint32_t tag = 0;
+    uint64_t wait_ticks = kWaitTicks;   // bound of each exchange wait (s_memrealtime ticks)
     const void* fn = nullptr;    // the k_persistent instance for this layout
 };
 
@@ -91,6 +92,7 @@ struct PArgs {
     float col1, wscale;
     double* hist;                // optional: w after step s at hist[k * hist_ld + s]
     int hist_ld;
+    uint64_t wait_ticks;         // bound of each exchange wait
 };
 
 static inline PersistentState* st(psf_scene* s) { return reinterpret_cast<PersistentState*>(s->persistent); }
@@ -395,7 +397,7 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
                         }
                     }
                     if (pend) {
-                        if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                        if (__builtin_amdgcn_s_memrealtime() - t0 > a.wait_ticks) {
                             s_flag = 2;
                             break;
                         }
@@ -666,6 +668,7 @@ static PArgs make_pargs(psf_scene* s, int nsteps, double* hist, int hist_ld)
     a.wscale = p->wscale;
     a.hist = hist;
     a.hist_ld = hist_ld;
+    a.wait_ticks = p->wait_ticks;
     return a;
 }
 
@@ -839,6 +842,14 @@ hipError_t launch_pack_c(psf_scene* s, const double* Cf, int K0, hipStream_t str
     k_pack_c<<<grid, 256, 0, stream>>>(Cf, K0, static_cast<int>(s->d.K), p->G, p->ipw, p->bpw,
                                        prec == PSF_C_BF16X2 ? 1 : 0, p->meta, p->blocks, p->inv_scale);
     return hipGetLastError();
+}
+
+int persistent_set_wait_ticks(psf_scene* s, uint64_t ticks)
+{
+    PersistentState* p = st(s);
+    if (!p) return PSF_ERR_UNSUPPORTED;
+    p->wait_ticks = ticks ? ticks : kWaitTicks;
+    return PSF_OK;
 }
 
 }  // namespace psf

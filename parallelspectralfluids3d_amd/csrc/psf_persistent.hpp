@@ -13,6 +13,9 @@ int persistent_auto_ipw(int c_precision, int K);
 hipError_t persistent_scene_init(psf_scene* s);
 void persistent_scene_free(psf_scene* s);
 int persistent_upload_c(psf_scene* s, const double* C);
+// bound of each wait of the persistent step's exchange, in s_memrealtime
+// ticks (100 MHz); 0 restores the default of 2 s
+int persistent_set_wait_ticks(psf_scene* s, uint64_t ticks);
 // hist != nullptr also stores w after step j at hist[k * hist_ld + j]
 hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t st, double* hist = nullptr,
                                    int hist_ld = 0);

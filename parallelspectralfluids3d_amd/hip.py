@@ -265,6 +265,10 @@ class Scene:
         self._c(self.ctx.lib.psf_set_params(self.h, float(dt), float(viscosity), int(bool(gravity_active)),
                                             float(lifetime)), "set_params")
 
+    def set_wait_limit(self, seconds):
+        """Watchdog of the persistent exchange (psf_set_wait_limit; 0 = default 2 s)."""
+        self._c(self.ctx.lib.psf_set_wait_limit(self.h, float(seconds)), "set_wait_limit")
+
     def set_path(self, path):
         self._c(self.ctx.lib.psf_set_step_path(self.h, int(path)), "set_step_path")
 

@@ -4,7 +4,7 @@
 // buildAdvection() uploads it, integrate() runs once per frame, getters read
 // the results back.
 //
-//   solver_check <dir> [extras | window <W>]
+//   solver_check <dir> [extras | window <W>] | eigen | leak
 // <dir>/meta.txt: nx ny nz resolution K N frames dt viscosity gravity lifetime c_precision phi_precision
 // <dir>/{phi,lambda,gravity,C,omega}.bin (float64), parts.bin (float32 N x 4)
 // writes <dir>/{omega_out.bin (float64), u_out.bin, parts_out.bin (float32)}
@@ -137,9 +137,39 @@ int eigen_check()
     return (err < 1e-12 && vec < 1e-9) ? 0 : 1;
 }
 
+// A context and a scene with queued work left alive at exit: the library
+// releases them itself, before the HIP runtime's teardown, and the process
+// exits normally.
+int leak_check()
+{
+    psf_ctx* ctx = nullptr;
+    if (psf_ctx_create(0, &ctx) != PSF_OK) return 1;
+    psf_scene_desc d{};
+    d.nx = d.ny = d.nz = 8;
+    d.resolution = 0.125;
+    for (int c = 0; c < 3; ++c) {
+        d.aabb_min[c] = -0.5f;
+        d.aabb_max[c] = 0.5f;
+    }
+    d.K = 4;
+    d.c_precision = PSF_C_BF16X2;
+    d.phi_precision = PSF_PHI_FP32;
+    d.max_particles = 0;
+    psf_scene* s = nullptr;
+    if (psf_scene_create(ctx, &d, &s) != PSF_OK) return 1;
+    std::vector<double> lam = {1.0, 2.0, 3.0, 4.0}, C(64, 0.0), w = {0.5, 0.5, 0.5, 0.5};
+    if (psf_scene_upload_basis(s, nullptr, lam.data(), nullptr) != PSF_OK) return 1;
+    if (psf_scene_upload_advection(s, C.data()) != PSF_OK) return 1;
+    if (psf_set_omega(s, w.data()) != PSF_OK) return 1;
+    if (psf_step(s, 1000) != PSF_OK) return 1;   // still running when main returns
+    std::printf("leak_check: returning with a live context\n");
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc == 2 && std::string(argv[1]) == "eigen") return eigen_check();
+    if (argc == 2 && std::string(argv[1]) == "leak") return leak_check();
     if (argc != 2 && argc != 3 && !(argc == 4 && std::string(argv[2]) == "window")) {
         std::fprintf(stderr, "usage: %s <dir> [extras | window <W>] | eigen\n", argv[0]);
         return 2;

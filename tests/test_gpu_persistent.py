@@ -96,3 +96,36 @@ def test_forcing_viscosity_and_nonfinite(hip_ctx):
     with pytest.raises(L.PsfError) as e:
         s.get_omega()
     assert e.value.status == L.PSF_ERR_NONFINITE
+
+
+def test_exchange_watchdog_gives_up(hip_ctx):
+    """A wait limit of 10 ns (one tick of the wait clock) makes every
+    workgroup's first wait for the others' y give up: the launch ends, the
+    status reports the timeout instead of a hang, omega keeps the last step
+    all workgroups completed, and after the default limit is restored and
+    omega is set again the scene steps like the oracle."""
+    from parallelspectralfluids3d_amd import _lib as L
+    _say("persistent watchdog")
+    sd = make_scene(n=8, K=16)
+    s = _scene(hip_ctx, sd, C_BF16X2)
+    s.set_params(0.01)
+    s.step(3)
+    w3 = s.get_omega()
+    s.set_wait_limit(1e-8)
+    s.step(50)
+    with pytest.raises(L.PsfError) as e:
+        s.get_omega()
+    assert e.value.status == L.PSF_ERR_HIP and "timed out" in str(e.value)
+    s.set_wait_limit(0.0)
+    s.set_omega(sd.omega)
+    s.step(100)
+    w = _oracle_traj(sd, 100)
+    assert np.linalg.norm(s.get_omega() - w) / np.linalg.norm(w) <= BUDGET[C_BF16X2]
+    assert np.all(np.isfinite(w3))
+    with pytest.raises(L.PsfError) as e:
+        s.set_wait_limit(-1.0)
+    assert e.value.status == L.PSF_ERR_ARG
+    p = _scene(hip_ctx, sd, 0)          # FP64: the plain path has no exchange
+    with pytest.raises(L.PsfError) as e:
+        p.set_wait_limit(1.0)
+    assert e.value.status == L.PSF_ERR_UNSUPPORTED

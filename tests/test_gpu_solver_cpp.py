@@ -123,3 +123,15 @@ def test_hip_solver_frame_window(tmp_path):
     assert c["windows_dropped"] == 0
     assert c["window_fills"] == frames // 32 + 1
     assert c["step_launches"] == c["window_fills"] and c["recon_launches"] == c["window_fills"]
+
+
+def test_live_context_at_exit():
+    """A C++ caller that returns from main with a context, a scene and a
+    running persistent launch still alive: the library's exit handler
+    releases them before the HIP runtime's teardown, and the process exits
+    with status 0 and nothing on stderr from the runtime's teardown."""
+    print("[gpu-case] solver_check leak", flush=True)
+    r = subprocess.run([EXE, "leak"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    assert "leak_check:" in r.stdout
+    assert "terminate" not in r.stderr and "Segmentation" not in r.stderr, r.stderr
