@@ -214,6 +214,30 @@ typedef struct psf_scene_counters {
 } psf_scene_counters;
 PSF_API int psf_scene_get_counters(psf_scene* scene, psf_scene_counters* out);
 
+/* Device addresses of a scene's resident buffers, for a caller's own GPU
+   code (a renderer drawing the particles or u in place, as the reference's
+   OpenCL solver shares its particle buffer with OpenGL,
+   psfsolvergpu.cpp:358-414).  Valid until the scene changes shape or is
+   destroyed; u and the particle records are those of the current frame once
+   psf_ctx_sync has returned (deferred particle passes run first).  Layouts
+   as for the uploads (C only for FP64/FP32 scenes, else NULL); particle
+   records are stored sorted by cell, parts[j] holding caller particle pid[j]. */
+typedef struct psf_scene_view {
+    double* omega;               /* K */
+    const double* lambda;        /* K */
+    const double* gravity;       /* K */
+    const double* visc;          /* K: exp(viscosity * lambda_k * dt) */
+    const void* C;               /* K matrices K x K, column-major, FP64 or FP32; NULL for packed precisions */
+    const void* Phi;             /* column-major, leading dimension ld_phi, FP64 or FP32 */
+    uint64_t ld_phi;
+    const float* u;              /* F */
+    float* particles;            /* n_particles x (x, y, z, w) */
+    const uint32_t* pid;         /* n_particles */
+    uint32_t n_particles;
+    int* status;                 /* 1 after a zero or non-finite energy */
+} psf_scene_view;
+PSF_API int psf_scene_device_view(psf_scene* scene, psf_scene_view* out);
+
 /* ---- scene cache -----------------------------------------------------------------
  * A versioned file of a scene's precomputed inputs: Phi (optional, FP64, F x K
  * column-major), lambda, g (optional) and C (FP64, K matrices K x K), i.e. what

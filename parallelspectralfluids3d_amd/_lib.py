@@ -52,6 +52,13 @@ class SceneCounters(ct.Structure):
                 ("particle_launches", ct.c_uint64)]
 
 
+class SceneView(ct.Structure):
+    _fields_ = [("omega", ct.c_void_p), ("lam", ct.c_void_p), ("gravity", ct.c_void_p), ("visc", ct.c_void_p),
+                ("C", ct.c_void_p), ("Phi", ct.c_void_p), ("ld_phi", ct.c_uint64), ("u", ct.c_void_p),
+                ("particles", ct.c_void_p), ("pid", ct.c_void_p), ("n_particles", ct.c_uint32),
+                ("status", ct.c_void_p)]
+
+
 class ServerStats(ct.Structure):
     _fields_ = [("submitted", ct.c_uint64), ("completed", ct.c_uint64), ("failed", ct.c_uint64),
                 ("rounds", ct.c_uint64), ("scene_frames", ct.c_uint64), ("active", ct.c_uint32),
@@ -113,6 +120,7 @@ def load():
         "psf_integrate": (i32, [vp, i32]),
         "psf_set_frame_window": (i32, [vp, ct.c_uint32]),
         "psf_scene_get_counters": (i32, [vp, ct.POINTER(SceneCounters)]),
+        "psf_scene_device_view": (i32, [vp, ct.POINTER(SceneView)]),
         "psf_time_phase": (i32, [vp, i32, i32, ct.POINTER(ct.c_double)]),
         "psf_server_create": (i32, [vp, ct.POINTER(ServerConfig), pp]),
         "psf_server_destroy": (i32, [vp]),

@@ -1025,6 +1025,26 @@ int psf_integrate(psf_scene* s, int nframes)
     return PSF_OK;
 }
 
+int psf_scene_device_view(psf_scene* s, psf_scene_view* out)
+{
+    if (!s || !out) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    psf_scene_view v{};
+    v.omega = s->dev.omega;
+    v.lambda = s->dev.lambda;
+    v.gravity = s->dev.gravity;
+    v.visc = s->dev.visc;
+    v.C = is_plain_precision(s->d.c_precision) ? s->dev.C : nullptr;
+    v.Phi = s->dev.Phi;
+    v.ld_phi = s->dev.ldF;
+    v.u = current_u(s);
+    v.particles = reinterpret_cast<float*>(s->dev.parts);
+    v.pid = s->dev.pid;
+    v.n_particles = s->n_particles;
+    v.status = s->dev.status;
+    *out = v;
+    return PSF_OK;
+}
+
 int psf_scene_get_counters(psf_scene* s, psf_scene_counters* out)
 {
     if (!s || !out) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");

@@ -49,6 +49,10 @@ class Context:
     def sync(self):
         self.check(self.lib.psf_ctx_sync(self.h), "sync")
 
+    def stream(self):
+        """The context's hipStream_t as an integer (psf_ctx_stream)."""
+        return self.lib.psf_ctx_stream(self.h)
+
     def close(self):
         if getattr(self, "h", None):
             for sc in list(self.scenes):
@@ -326,6 +330,12 @@ class Scene:
         c = L.SceneCounters()
         self._c(self.ctx.lib.psf_scene_get_counters(self.h, ct.byref(c)), "get_counters")
         return {name: int(getattr(c, name)) for name, _ in L.SceneCounters._fields_}
+
+    def device_view(self) -> dict:
+        """Device addresses of the scene's buffers (psf_scene_device_view)."""
+        v = L.SceneView()
+        self._c(self.ctx.lib.psf_scene_device_view(self.h, ct.byref(v)), "device_view")
+        return {name: getattr(v, name) for name, _ in L.SceneView._fields_}
 
     def set_density_grid(self, dims, count_in_trace=True):
         """Particle density grid of dims = (rx, ry, rz) cells over the AABB

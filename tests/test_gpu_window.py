@@ -240,3 +240,22 @@ def test_unchanged_params_keep_the_window(hip_ctx):
         s.integrate(3)
     _same(one, win)
     assert win.counters()["windows_dropped"] == 1
+
+
+def test_device_view(hip_ctx):
+    """psf_scene_device_view: the addresses of the resident buffers; u follows
+    the current frame of a window; packed-precision scenes expose no C."""
+    _say("device view")
+    sd = make_scene(n=8, K=16)
+    one, win = _pair(hip_ctx, sd, C_FP64, PHI_FP32, 256, 8)
+    v = one.device_view()
+    assert all(v[k] for k in ("omega", "lam", "gravity", "visc", "C", "Phi", "u", "particles", "pid", "status"))
+    assert v["n_particles"] == 256 and v["ld_phi"] % 256 == 0 and v["ld_phi"] >= sd.grid.nF
+    win.integrate(3)
+    u3 = win.device_view()["u"]
+    win.integrate(1)
+    assert win.device_view()["u"] - u3 == 4 * win.device_view()["ld_phi"]    # the next row of the window
+    from parallelspectralfluids3d_amd import hip
+    d = hip.make_desc(sd.grid, sd.resolution, sd.K, C_BF16X2, PHI_FP32, 0, sd.aabb_min, sd.aabb_max)
+    b = hip.Scene(hip_ctx, d)
+    assert b.device_view()["C"] is None
