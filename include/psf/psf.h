@@ -238,6 +238,26 @@ typedef struct psf_scene_view {
 } psf_scene_view;
 PSF_API int psf_scene_device_view(psf_scene* scene, psf_scene_view* out);
 
+/* Reduced steps on caller-owned device buffers, allocating nothing.  No
+   counterpart in the reference; it is for callers that cannot allocate
+   device memory (code under HIP graph capture, serving loops with
+   preallocated buffers).  nsteps steps of the plain path of psf_step
+   (psfsolver.cpp:34-59) on device arrays: C (K matrices K x K, column-major,
+   PSF_C_FP64 or PSF_C_FP32), omega (K, updated in place), visc (K:
+   exp(viscosity * lambda_k * dt)), gravity (K; may be NULL when
+   gravity_active is 0) and status (one int, set to 1 on a zero or
+   non-finite energy, never cleared here).  Runs on the current device, queued
+   on `stream` (a hipStream_t of that device; NULL: the null stream), and
+   returns without waiting.  The partial sums of y live in one library-owned
+   device array per GPU (256 KiB, K <= 1024), held until the process ends:
+   calls that overlap on one GPU race, so order them on one stream.
+   PSF_ERR_ARG for K outside [1, 1024], nsteps < 0 or a null pointer;
+   PSF_ERR_UNSUPPORTED for other precisions.  psf_scene_device_view gives a
+   scene's buffers in these layouts. */
+PSF_API int psf_step_device(int c_precision, const void* C, double* omega, const double* visc,
+                            const double* gravity, uint32_t K, double dt, int gravity_active, int nsteps,
+                            int* status, void* stream);
+
 /* ---- scene cache -----------------------------------------------------------------
  * A versioned file of a scene's precomputed inputs: Phi (optional, FP64, F x K
  * column-major), lambda, g (optional) and C (FP64, K matrices K x K), i.e. what

@@ -121,6 +121,7 @@ def load():
         "psf_set_frame_window": (i32, [vp, ct.c_uint32]),
         "psf_scene_get_counters": (i32, [vp, ct.POINTER(SceneCounters)]),
         "psf_scene_device_view": (i32, [vp, ct.POINTER(SceneView)]),
+        "psf_step_device": (i32, [i32, vp, vp, vp, vp, u32, dbl, i32, i32, vp, vp]),
         "psf_time_phase": (i32, [vp, i32, i32, ct.POINTER(ct.c_double)]),
         "psf_server_create": (i32, [vp, ct.POINTER(ServerConfig), pp]),
         "psf_server_destroy": (i32, [vp]),
