@@ -61,6 +61,12 @@ constexpr int kMaxCUs = 256;
 constexpr uint64_t kWaitTicks = 200000000ull;   // s_memrealtime runs at 100 MHz: 2 s
 constexpr int kXStride = 16;     // 64-bit words per exchange record: one 128-byte line per mode
 constexpr int kBChunk = 6;       // B-operand blocks loaded ahead of the MFMA chain
+// waves per workgroup of a single scene's launch (one workgroup per CU): two
+// per SIMD, so one wave's MFMA, LDS and FP64 latencies overlap the other's
+#ifndef PSF_SCENE_WAVES
+#define PSF_SCENE_WAVES 8
+#endif
+constexpr int kSceneWaves = PSF_SCENE_WAVES;
 
 struct PersistentState {
     int G = 0, ipw = 0, Kp = 0, nb = 0, nBi = 0, bpw = 0;
@@ -193,18 +199,20 @@ __device__ __forceinline__ f32x4 mfma16(const uint4& a, const uint4& b, const f3
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
 }
 
-// The body of one workgroup: local workgroup g of the scene described by a.
-template <int ELEM, int RB>
+// The body of one workgroup (NW waves): local workgroup g of the scene
+// described by a.
+template <int ELEM, int RB, int NW>
 __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
 {
+    constexpr int NT = 64 * NW;
     __shared__ double s_w[kMaxK];
     __shared__ double s_y[kMaxK];
     __shared__ __align__(16) uint16_t s_wb[3 * kMaxK];   // w_hi at [0, Kp), w_lo at [Kp, 2 Kp), zeros after
     __shared__ uint32_t s_meta[kMaxMeta];
     __shared__ double s_visc[kMaxK];
     __shared__ double s_grav[kMaxK];
-    __shared__ double s_red[8];
-    __shared__ double s_ypart[4][kMaxIpw];
+    __shared__ double s_red[2 * NW];
+    __shared__ double s_ypart[NW][kMaxIpw];
     __shared__ int s_flag;
 
     const int tid = threadIdx.x, lane = tid & 63;
@@ -212,19 +220,19 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
     const int K = a.K, Kp = a.Kp;
     const int col = lane & 15, kg = lane >> 4;
 
-    for (int b = tid; b < a.bpw; b += 256) s_meta[b] = a.meta[b];
-    for (int k = tid; k < Kp; k += 256) {
+    for (int b = tid; b < a.bpw; b += NT) s_meta[b] = a.meta[b];
+    for (int k = tid; k < Kp; k += NT) {
         s_w[k] = k < K ? a.omega[k] : 0.0;
         s_visc[k] = k < K ? a.visc[k] : 1.0;
         s_grav[k] = k < K ? a.gravity[k] : 0.0;
     }
-    for (int k = tid; k < Kp; k += 256) s_wb[2 * Kp + k] = 0;
+    for (int k = tid; k < Kp; k += NT) s_wb[2 * Kp + k] = 0;
     if (tid == 0) s_flag = 0;
 
     // this wave's block range; the first RB blocks stay in registers, and
     // register slots past the wave's last block hold zero blocks, so the
     // resident MFMA chain runs without guards
-    const int per = (a.bpw + 3) / 4;
+    const int per = (a.bpw + NW - 1) / NW;
     const int b0 = wv * per;
     const int nblk = max(0, min(a.bpw - b0, per));
     const int nres = min(nblk, RB);
@@ -281,8 +289,8 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
         s_wb[k] = hi;
         s_wb[Kp + k] = lo;
     };
-    for (int k = tid; k < Kp; k += 256) split_one(k, s_w[k]);
-    if (tid < 4 * kMaxIpw) s_ypart[tid / kMaxIpw][tid % kMaxIpw] = 0.0;
+    for (int k = tid; k < Kp; k += NT) split_one(k, s_w[k]);
+    if (tid < NW * kMaxIpw) s_ypart[tid / kMaxIpw][tid % kMaxIpw] = 0.0;
     __syncthreads();
 
     for (int s = 0; s < a.nsteps; ++s) {
@@ -356,8 +364,10 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
         if (tid < a.ipw) {
             const int i = g + a.G * tid;
             if (i < K) {
-                const double y = (((s_ypart[0][tid] + s_ypart[1][tid]) + s_ypart[2][tid]) + s_ypart[3][tid]) *
-                                 a.inv_scale[g * a.ipw + tid];
+                double ys = s_ypart[0][tid];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) ys += s_ypart[w][tid];   // waves in order
+                const double y = ys * a.inv_scale[g * a.ipw + tid];
                 const unsigned long long bits = __double_as_longlong(y);
                 const unsigned long long tg = static_cast<unsigned long long>(tag) << 32;
                 __hip_atomic_store(xb + kXStride * i, tg | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -412,7 +422,7 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
         {
 #pragma clang fp contract(off)
             double s1 = 0.0, s2 = 0.0;
-            for (int k = tid; k < K; k += 256) {
+            for (int k = tid; k < K; k += NT) {
                 const double w = s_w[k];
                 const double wn = w + a.dt * s_y[k];
                 s_y[k] = wn;
@@ -423,18 +433,22 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
             s2 = wave_total(s2);
             if (lane == 0) {
                 s_red[wv] = s1;
-                s_red[4 + wv] = s2;
+                s_red[NW + wv] = s2;
             }
             __syncthreads();
-            const double e1 = ((s_red[0] + s_red[1]) + s_red[2]) + s_red[3];
-            const double e2 = ((s_red[4] + s_red[5]) + s_red[6]) + s_red[7];
+            double e1 = s_red[0], e2 = s_red[NW];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) {   // waves in order
+                e1 += s_red[w];
+                e2 += s_red[NW + w];
+            }
             if (!(e2 > 0.0) || !__builtin_isfinite(e2) || !__builtin_isfinite(e1)) {
                 if (tid == 0) s_flag = 1;
                 __syncthreads();
                 break;
             }
             const double fac = sqrt(e1 / e2);
-            for (int k = tid; k < K; k += 256) {
+            for (int k = tid; k < K; k += NT) {
                 double w = s_y[k] * fac;
                 w = w * s_visc[k];
                 if (a.gravity_active) w = w + a.dt * s_grav[k];
@@ -442,22 +456,22 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
                 split_one(k, w);
                 if (a.hist && g == 0) a.hist[static_cast<size_t>(k) * a.hist_ld + s] = w;
             }
-            if (tid < 4 * kMaxIpw) s_ypart[tid / kMaxIpw][tid % kMaxIpw] = 0.0;
+            if (tid < NW * kMaxIpw) s_ypart[tid / kMaxIpw][tid % kMaxIpw] = 0.0;
         }
         __syncthreads();
     }
     __syncthreads();
     if (g == 0) {
-        for (int k = tid; k < K; k += 256) a.omega[k] = s_w[k];
+        for (int k = tid; k < K; k += NT) a.omega[k] = s_w[k];
         if (tid == 0 && s_flag) *a.status = s_flag;
     }
 }
 
-// one scene: workgroup g = blockIdx.x
-template <int ELEM, int RB>
-__global__ __launch_bounds__(256, 1) void k_persistent(PArgs a)
+// one scene: workgroup g = blockIdx.x, NW waves each
+template <int ELEM, int RB, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void k_persistent(PArgs a)
 {
-    persistent_body<ELEM, RB>(a, static_cast<int>(blockIdx.x));
+    persistent_body<ELEM, RB, NW>(a, static_cast<int>(blockIdx.x));
 }
 
 // several independent scenes in one launch: wg_map[blockIdx.x] = scene << 16 | g.
@@ -469,21 +483,25 @@ __global__ __launch_bounds__(256, 2) void k_persistent_multi(const PArgs* __rest
 {
     const uint32_t m = wg_map[blockIdx.x];
     const PArgs a = table[m >> 16];
-    persistent_body<ELEM, RB>(a, static_cast<int>(m & 0xFFFFu));
+    persistent_body<ELEM, RB, 4>(a, static_cast<int>(m & 0xFFFFu));
 }
 
 // ---------------------------------------------------------------------------
 // smallest register budget that holds a wave's blocks (kRB when they do not fit)
+template <int ELEM>
+static const void* pick_kernel_elem(int bpw)
+{
+    constexpr int NW = kSceneWaves;
+    const int per = (bpw + NW - 1) / NW;
+    return per <= 4    ? reinterpret_cast<const void*>(&k_persistent<ELEM, 4, NW>)
+           : per <= 12 ? reinterpret_cast<const void*>(&k_persistent<ELEM, 12, NW>)
+           : per <= 18 ? reinterpret_cast<const void*>(&k_persistent<ELEM, 18, NW>)
+                       : reinterpret_cast<const void*>(&k_persistent<ELEM, kRB, NW>);
+}
+
 static const void* pick_kernel(int elem, int bpw)
 {
-    const int per = (bpw + 3) / 4;
-    if (elem == 0)
-        return per <= 4 ? reinterpret_cast<const void*>(&k_persistent<0, 4>)
-             : per <= 12 ? reinterpret_cast<const void*>(&k_persistent<0, 12>)
-                         : reinterpret_cast<const void*>(&k_persistent<0, kRB>);
-    return per <= 4 ? reinterpret_cast<const void*>(&k_persistent<1, 4>)
-         : per <= 12 ? reinterpret_cast<const void*>(&k_persistent<1, 12>)
-                     : reinterpret_cast<const void*>(&k_persistent<1, kRB>);
+    return elem == 0 ? pick_kernel_elem<0>(bpw) : pick_kernel_elem<1>(bpw);
 }
 
 hipError_t persistent_scene_init(psf_scene* s)
@@ -503,7 +521,7 @@ hipError_t persistent_scene_init(psf_scene* s)
     p->fn = pick_kernel(p->elem, p->bpw);
     // every workgroup must be resident at once
     int per_cu = 0, cus = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->fn, 256, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->fn, 64 * kSceneWaves, 0);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->ctx->device);
     if (e != hipSuccess) return e;
     if (per_cu * cus < p->G) return hipErrorCooperativeLaunchTooLarge;
@@ -679,7 +697,7 @@ hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream,
     if (nsteps <= 0) return hipSuccess;
     PArgs a = make_pargs(s, nsteps, hist, hist_ld);
     void* args[] = {&a};
-    hipError_t e = hipLaunchKernel(p->fn, dim3(p->G), dim3(256), args, 0, stream);
+    hipError_t e = hipLaunchKernel(p->fn, dim3(p->G), dim3(64 * kSceneWaves), args, 0, stream);
     if (e == hipSuccess) p->tag += static_cast<uint32_t>(nsteps);
     return e;
 }
