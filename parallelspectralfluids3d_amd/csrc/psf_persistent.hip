@@ -103,6 +103,13 @@ struct PArgs {
 
 static inline PersistentState* st(psf_scene* s) { return reinterpret_cast<PersistentState*>(s->persistent); }
 
+// Row tiles of a mode in storage order: 0, nb-1, 1, nb-2, ...  Row p holds
+// nb - p tile blocks (about half as many for two-tile blocks), so a long row
+// and a short row alternate; the contiguous block ranges of the waves then
+// hold nearly equal numbers of row groups (two each at K = 256 with eight
+// waves), and each group costs a flush of its accumulator.
+static inline int row_at(int idx, int nb) { return (idx & 1) ? nb - 1 - (idx >> 1) : (idx >> 1); }
+
 int blocks_per_mode(int prec, int nb)
 {
     if (prec == PSF_C_BF16X2) return nb * (nb + 1) / 2;
@@ -534,8 +541,8 @@ hipError_t persistent_scene_init(psf_scene* s)
     std::vector<uint32_t> meta;
     meta.reserve(p->bpw);
     for (int t = 0; t < p->ipw; ++t)
-        for (int q0 = 0; q0 < p->nb; ++q0) {
-            const int pr = q0;
+        for (int ri = 0; ri < p->nb; ++ri) {
+            const int pr = row_at(ri, p->nb);
             if (prec == PSF_C_BF16X2) {
                 for (int q = pr; q < p->nb; ++q)
                     meta.push_back(uint32_t(t) << 24 | uint32_t(pr) << 16 | uint32_t(q) << 8 | uint32_t(q));
@@ -617,7 +624,8 @@ int persistent_upload_c(psf_scene* s, const double* C)
             inv[size_t(g) * p->ipw + t] = 1.0 / (scale * p->wscale);
             // blocks of mode i in metadata order
             int bi = 0;
-            for (int pr = 0; pr < nb; ++pr) {
+            for (int ri = 0; ri < nb; ++ri) {
+                const int pr = row_at(ri, nb);
                 auto emit = [&](int q1, int q2, bool hilo) {
                     const size_t base = ((size_t(g) * p->bpw) + size_t(t) * p->nBi + bi) * 64 * 8;
                     for (int l = 0; l < 64; ++l) {
