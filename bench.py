@@ -36,6 +36,7 @@ sys.path.insert(0, ROOT)
 METRIC = "reduced steps/s + particle-steps/s at 64³,K=256; p99 latency; err vs FP64"
 HBM_PEAK = 8.0e12
 FP64_PEAK = 78.6e12
+FP32_PEAK = 157.3e12
 # HBM bytes per launch at 64^3/K=256 with 2^20 particles, measured with
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --counter-pass`
 # (fetched KiB x 2, the factor a 1 GiB hipMemcpy reads back, plus written
@@ -44,6 +45,11 @@ RECON_TRAFFIC_64_256 = 821_147_648          # k_reconstruct<float, 1>: 399,391 K
 WINDOW_TRAFFIC_64_256_W32 = 929_118_528     # k_reconstruct_mfma<float, 2>: 403,749 / 99,845 KiB
 TRACE_TRAFFIC_64_1M = 37_685_248            # k_trace: 10,209 / 16,384 KiB
 TRACE_FRAMES_TRAFFIC_64_1M = 66_196_608     # k_trace_frames (8 frames per launch): 24,131 / 16,384 KiB
+
+
+# u = Phi w of the frame loop: FP32 Phi and w with FP32 sums on the FP32 matrix
+# cores (FP64 is kept for the renormalisation, Euler update and viscosity)
+PHI_PREC = 2
 
 
 def log(msg):
@@ -114,6 +120,7 @@ def cpu_baseline(sc, budget_s=12.0):
     timed = frames - 1
     tot = t_red + t_rec + t_par
     N = p.shape[0]
+    ref_state = dict(frames=frames, omega=w.copy(), parts=p.copy(), u=u.copy())   # the oracle after `frames` frames
     # one thread, a few frames of the same workload
     t1_frames, t1_tot = 3, 0.0
     for f in range(t1_frames):
@@ -132,7 +139,7 @@ def cpu_baseline(sc, budget_s=12.0):
                     break
     except OSError:
         pass
-    return {"value": N * timed / tot, "unit": "particle-steps/s", "cores": nthreads, "kind": "port",
+    return ref_state, {"value": N * timed / tot, "unit": "particle-steps/s", "cores": nthreads, "kind": "port",
             "sample": f"{timed} integrate() frames of the same {g.nx}^3/K={len(sc['lam'])}/{N}-particle workload "
                       f"after 1 warm-up frame (FP64 C and Phi as in psfsolver.cpp)",
             "ms_per_frame": 1e3 * tot / timed, "ms_reduced": 1e3 * t_red / timed,
@@ -661,7 +668,7 @@ def main():
     ctx = hip.Context(0)
 
     # ---- frame loop (the value) ---------------------------------------------------
-    s = make_hip_scene(ctx, sc, L.C_BF16X2, L.PHI_FP32, True, N)
+    s = make_hip_scene(ctx, sc, L.C_BF16X2, PHI_PREC, True, N)
     W = args.window
     s.set_frame_window(W)
     log(f"frame scene uploaded (frame window {W})")
@@ -762,16 +769,27 @@ def main():
                                            "particle-step, address unit busy 75-77% (TA_BUSY_avr), 69% of "
                                            "wave-cycles waiting on memory (SQ_WAIT_INST_ANY), L1 hit rate 94%")
     rec_bytes = F * K * 4 + F * 4 + K * 8
+    # (the calibrated traffic constants were measured for the FP64-sum kernels)
+    fp64_sums = PHI_PREC != 2
     roof["k_reconstruct"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
                                  achieved=rec_bytes / (ms_rec * 1e-3) / 1e9, algorithmic_bytes_per_launch=rec_bytes,
-                                 traffic=RECON_TRAFFIC_64_256 if (n, K) == (64, 256) else None)
+                                 traffic=RECON_TRAFFIC_64_256 if (n, K) == (64, 256) and fp64_sums else None)
     if W > 1:
         wflops = 2.0 * F * K * W
-        roof["k_reconstruct_window"] = dict(bound="fp64", unit="TFLOP/s", peak=FP64_PEAK / 1e12,
-                                            achieved=wflops / (ms_wrec * 1e-3) / 1e12,
-                                            algorithmic_flops_per_launch=wflops,
-                                            algorithmic_bytes_per_launch=F * K * 4 + W * (F * 4 + K * 8),
-                                            traffic=WINDOW_TRAFFIC_64_256_W32 if (n, K, W) == (64, 256, 32) else None)
+        wbytes = F * K * 4 + W * (F * 4 + K * 8)
+        if fp64_sums:
+            roof["k_reconstruct_window"] = dict(bound="fp64", unit="TFLOP/s", peak=FP64_PEAK / 1e12,
+                                                achieved=wflops / (ms_wrec * 1e-3) / 1e12,
+                                                algorithmic_flops_per_launch=wflops, algorithmic_bytes_per_launch=wbytes,
+                                                traffic=WINDOW_TRAFFIC_64_256_W32 if (n, K, W) == (64, 256, 32) else None)
+        else:
+            # FP32 matrix cores: 2 F K W flops take less time at their peak than
+            # the pass over Phi at the HBM peak, so HBM bounds the launch
+            roof["k_reconstruct_window"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
+                                                achieved=wbytes / (ms_wrec * 1e-3) / 1e9,
+                                                algorithmic_bytes_per_launch=wbytes, algorithmic_flops_per_launch=wflops,
+                                                fp32_mfma_tflops=wflops / (ms_wrec * 1e-3) / 1e12,
+                                                fp32_mfma_peak_tflops=FP32_PEAK / 1e12, traffic=None)
         cbytes = K * K * (K + 1) // 2 * 4 + 32 * K
         roof["k_persistent"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
                                     achieved=cbytes / (kern["k_persistent"] * 1e-3) / 1e9,
@@ -837,9 +855,25 @@ def main():
         serving = serving_bench(sctx, args.serving_requests, args.serving_rate)
         sctx.close()
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
-    if cpu:
+    cpu, accuracy = None, None
+    if not args.no_cpu_baseline:
+        ref_state, cpu = cpu_baseline(sc)
         log(f"cpu baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
+        # the frame loop's arithmetic against the FP64 oracle over the frames the CPU leg ran
+        actx = hip.Context(0)
+        a = make_hip_scene(actx, sc, L.C_BF16X2, PHI_PREC, True, N)
+        a.set_frame_window(W)
+        a.integrate(ref_state["frames"])
+        wg, ug, pg = a.get_omega(), a.get_velocity(), a.get_particles()
+        a.close()
+        actx.close()
+        accuracy = {"frames": ref_state["frames"],
+                    "omega_rel_err": float(np.linalg.norm(wg - ref_state["omega"]) / np.linalg.norm(ref_state["omega"])),
+                    "u_max_abs_err_over_max_u": float(np.abs(ug - ref_state["u"]).max() / np.abs(ref_state["u"]).max()),
+                    "particle_max_abs_err": float(np.abs(pg[:, :3] - ref_state["parts"][:, :3]).max()),
+                    "against": "the oracle (FP64 C, Phi and sums, FP32 particles) from the same initial state"}
+        log(f"frame loop vs FP64 oracle after {ref_state['frames']} frames: omega {accuracy['omega_rel_err']:.2e}, "
+            f"u {accuracy['u_max_abs_err_over_max_u']:.2e} of max|u|, particles {accuracy['particle_max_abs_err']:.2e}")
 
     out = {
         "metric": METRIC,
@@ -853,8 +887,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16",
-        "dtype_detail": "C in BF16 hi+lo on MFMA with FP32 accumulation; FP64 omega update; FP32 Phi with FP64 "
-                        "sums (FP64 FMA, one pass over Phi per frame window); FP32 particles",
+        "dtype_detail": "C in BF16 hi+lo on MFMA with FP32 accumulation; FP64 omega update (energy, Euler, "
+                        "renormalisation, viscosity); u = Phi w with FP32 Phi and w and FP32 sums on the FP32 matrix "
+                        "cores (one pass over Phi per frame window); FP32 particles",
         "data": "synthetic",
         "config": {"workload": f"{n}^3 closed box, K={K} analytic DEC eigenbasis, {N} particles, dt=0.01, "
                                f"one integrate() frame per step", "grid": n, "K": K, "particles": N,
@@ -876,6 +911,7 @@ def main():
         "density": dens,
         "serving": serving,
         "cpu_baseline": cpu,
+        "frame_loop_error_vs_fp64_oracle": accuracy,
     }
     print(json.dumps(out), flush=True)
 

@@ -61,8 +61,12 @@ typedef enum psf_c_precision {
     PSF_C_BF16X2 = 4   /* C = hi + lo, both BF16, on MFMA, FP32 accumulation */
 } psf_c_precision;
 
-/* storage of the basis Phi; the sum u_f = sum_k Phi[f][k] w_k is FP64 either way */
-typedef enum psf_phi_precision { PSF_PHI_FP64 = 0, PSF_PHI_FP32 = 1 } psf_phi_precision;
+/* storage of the basis Phi and arithmetic of u_f = sum_k Phi[f][k] w_k */
+typedef enum psf_phi_precision {
+    PSF_PHI_FP64 = 0,       /* FP64 Phi, FP64 sums (parity path) */
+    PSF_PHI_FP32 = 1,       /* FP32 Phi, FP64 sums */
+    PSF_PHI_FP32_ACC32 = 2  /* FP32 Phi and w, FP32 sums on the FP32 matrix cores */
+} psf_phi_precision;
 
 /* how the reduced step is launched */
 typedef enum psf_step_path {

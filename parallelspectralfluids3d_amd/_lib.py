@@ -15,7 +15,7 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "psf", "psf.h")
 PSF_OK, PSF_ERR_ARG, PSF_ERR_HIP, PSF_ERR_NOMEM, PSF_ERR_STATE, PSF_ERR_UNSUPPORTED, PSF_ERR_NONFINITE = \
     0, -1, -2, -3, -4, -5, -6
 C_FP64, C_FP32, C_BF16, C_FP16, C_BF16X2 = 0, 1, 2, 3, 4
-PHI_FP64, PHI_FP32 = 0, 1
+PHI_FP64, PHI_FP32, PHI_FP32_ACC32 = 0, 1, 2
 PATH_PLAIN, PATH_PERSISTENT = 0, 1
 PHASE_STEP, PHASE_RECON, PHASE_TRACE, PHASE_FRAME, PHASE_NSTEP, PHASE_WINDOW_RECON = 0, 1, 2, 3, 4, 5
 PHASE_DENSITY_COUNT, PHASE_DENSITY_VOLUME, PHASE_FRAME_DENSITY, PHASE_TRACE_FUSED = 6, 7, 8, 9

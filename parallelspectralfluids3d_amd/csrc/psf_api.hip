@@ -172,7 +172,7 @@ int validate_desc(psf_ctx* ctx, const psf_scene_desc* d)
     if (!(d->resolution > 0.0) || !std::isfinite(d->resolution)) return set_err(ctx, PSF_ERR_ARG, "resolution must be > 0");
     if (d->c_precision < PSF_C_FP64 || d->c_precision > PSF_C_BF16X2)
         return set_err(ctx, PSF_ERR_ARG, "unknown c_precision %d", d->c_precision);
-    if (d->phi_precision != PSF_PHI_FP64 && d->phi_precision != PSF_PHI_FP32)
+    if (d->phi_precision != PSF_PHI_FP64 && d->phi_precision != PSF_PHI_FP32 && d->phi_precision != PSF_PHI_FP32_ACC32)
         return set_err(ctx, PSF_ERR_ARG, "unknown phi_precision %d", d->phi_precision);
     for (int c = 0; c < 3; ++c)
         if (!(d->aabb_max[c] > d->aabb_min[c])) return set_err(ctx, PSF_ERR_ARG, "aabb_max must exceed aabb_min");
@@ -568,7 +568,7 @@ int psf_scene_footprint(const psf_scene_desc* d, uint64_t* bytes)
     uint32_t V, F, a, b, c;
     counts(*d, V, F, a, b, c);
     const size_t ldF = round_up(F, 256);
-    const size_t phi = ldF * d->K * (d->phi_precision == PSF_PHI_FP32 ? 4 : 8);
+    const size_t phi = ldF * d->K * (d->phi_precision == PSF_PHI_FP64 ? 8 : 4);
     *bytes = c_bytes(*d) + phi + size_t(F) * 4 + size_t(d->max_particles) * (16 + 48) + size_t(d->K) * 8 * 6 + 256;
     return PSF_OK;
 }
@@ -611,7 +611,7 @@ int psf::scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOp
     s->dev.C_bytes = is_plain_precision(desc->c_precision)
                          ? c_bytes(*desc)
                          : persistent_c_bytes(desc->c_precision, static_cast<int>(K), opt.modes_per_wg);
-    const size_t phi_bytes = s->dev.ldF * K * (desc->phi_precision == PSF_PHI_FP32 ? 4 : 8);
+    const size_t phi_bytes = s->dev.ldF * K * (desc->phi_precision == PSF_PHI_FP64 ? 8 : 4);
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
     chk(salloc(s, &s->dev.omega, K));
@@ -704,7 +704,7 @@ int psf_scene_upload_basis(psf_scene* s, const double* Phi, const double* lambda
     int rc = PSF_OK;
     if (Phi) {
         // column by column through a pinned staging buffer, converted to the storage type
-        const size_t esz = s->d.phi_precision == PSF_PHI_FP32 ? 4 : 8;
+        const size_t esz = s->d.phi_precision == PSF_PHI_FP64 ? 8 : 4;
         void* stage = nullptr;
         HIPCHK(ctx, hipHostMalloc(&stage, F * esz, hipHostMallocDefault));
         for (size_t k = 0; k < K && rc == PSF_OK; ++k) {

@@ -292,7 +292,7 @@ int psf_scene_generate_box_basis(psf_scene* s, const int32_t* freqs, const doubl
     BOX_HIPCHK(ctx, upload(damp, amp));
     BOX_HIPCHK(ctx, upload(dprof, prof));
     const unsigned grid = (s->F + 255) / 256;
-    if (s->d.phi_precision == PSF_PHI_FP32)
+    if (s->d.phi_precision != PSF_PHI_FP64)   // FP32 storage
         k_box_basis<float><<<grid, 256, 0, ctx->stream>>>(
             static_cast<float*>(s->dev.Phi), s->dev.ldF, s->F, s->nZF, s->nYF, m.dims[0], m.dims[1], m.dims[2], m.K,
             static_cast<const int*>(dpos.p), static_cast<const double*>(damp.p), static_cast<const double*>(dprof.p),

@@ -262,12 +262,24 @@ __device__ __forceinline__ int scene_of_block(const Tab* __restrict__ tab, int n
 // tiles as one aligned 16-byte load of 4 consecutive faces, and stores 4
 // consecutive faces of one frame at once (accumulator slot r of lane l holds
 // row l/16 + 4r, column l%16).  NT column tiles of 16 frames: W <= 16 * NT.
+//
+// ACC = float: the same product on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32, twice the FP64 rate on this part) for FP32 Phi
+// and w rounded to FP32, with FP32 accumulation: each column is a dot
+// product of its own, so a frame's u has the same bits in any window and
+// alone (single frames use this kernel with one column).  Its accumulator
+// slot r of lane l holds row 4 (l/16) + r (measured), so a lane's 16 values
+// of one frame are 16 consecutive faces.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-template <typename T, int NT>
+template <typename T, int NT, typename ACC = double>
 __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint32_t F, int K, const double* hist_,
                                                  int hist_ld, int W, float* uh_, size_t ldU, uint32_t blk)
 {
+    static_assert(std::is_same<ACC, double>::value || std::is_same<T, float>::value,
+                  "FP32 accumulation takes FP32 Phi");
+    constexpr bool kF32 = std::is_same<ACC, float>::value;
+    typedef typename std::conditional<kF32, f32x4, f64x4>::type accv;
     const gptr<const T> Phi = as_global(Phi_);
     const gptr<const double> hist = as_global(hist_);
     const gptr<float> uh = as_global(uh_);
@@ -277,11 +289,11 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
     if (f0 >= ldF) return;
     const int li = lane & 15, lk = lane >> 4;
     const gptr<const T> arow = Phi + f0 + 4 * li;
-    f64x4 acc[4][NT];
+    accv acc[4][NT];
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = accv{0, 0, 0, 0};
     // frames of the lane's B column: t = 16 ct + li; columns t >= W load a
     // valid element (t clamped) and multiply by 0 instead
     bool tv[NT];
@@ -310,25 +322,29 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
         for (int ct = 0; ct < NT; ++ct) b[ct] = hist[static_cast<size_t>(k) * hist_ld + tc[ct]];
     };
     auto step = [&](const RawA& r, const double (&braw)[NT]) {
-        double a[4];
+        ACC a[4];
         if constexpr (sizeof(T) == 4) {
             a[0] = r.v[0].x; a[1] = r.v[0].y; a[2] = r.v[0].z; a[3] = r.v[0].w;
         } else {
             a[0] = r.v[0].x; a[1] = r.v[0].y; a[2] = r.v[1].x; a[3] = r.v[1].y;
         }
-        double b[NT];
+        ACC b[NT];
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) b[ct] = tv[ct] ? braw[ct] : 0.0;
+        for (int ct = 0; ct < NT; ++ct) b[ct] = tv[ct] ? static_cast<ACC>(braw[ct]) : ACC(0);
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-            for (int ct = 0; ct < NT; ++ct)
-                acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+            for (int ct = 0; ct < NT; ++ct) {
+                if constexpr (kF32)
+                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+                else
+                    acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+            }
     };
     // full k-steps (every k < K), branch-free, loads kPipe steps ahead: the
     // queue slot of step k4 is refilled with step k4 + kPipe (clamped to the
     // last full step; the extra loads are in bounds and unused)
-    constexpr int kPipe = 4;
+    constexpr int kPipe = 4;   // (6, 8 and 12 measured no faster with FP32 accumulation)
     const int K4 = K >> 2;
     if (K4 > 0) {
         RawA qa[kPipe];
@@ -376,7 +392,8 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
         const gptr<float> ut = uh + static_cast<size_t>(16 * ct + li) * ldU;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const size_t f = f0 + 4 * (lk + 4 * r);
+            // face of slot r, tile rt: f0 + 4 * row + rt
+            const size_t f = kF32 ? f0 + 16 * lk + 4 * r : f0 + 4 * (lk + 4 * r);
             const f32x4 o = {static_cast<float>(acc[0][ct][r]), static_cast<float>(acc[1][ct][r]),
                              static_cast<float>(acc[2][ct][r]), static_cast<float>(acc[3][ct][r])};
             if (f + 3 < F) {
@@ -390,12 +407,12 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
     }
 }
 
-template <typename T, int NT>
+template <typename T, int NT, typename ACC = double>
 __global__ __launch_bounds__(256) void k_reconstruct_mfma(const T* __restrict__ Phi, size_t ldF, uint32_t F, int K,
                                                           const double* __restrict__ hist, int hist_ld, int W,
                                                           float* __restrict__ uh, size_t ldU)
 {
-    mfma_window_body<T, NT>(Phi, ldF, F, K, hist, hist_ld, W, uh, ldU, blockIdx.x);
+    mfma_window_body<T, NT, ACC>(Phi, ldF, F, K, hist, hist_ld, W, uh, ldU, blockIdx.x);
 }
 
 template <int NT>
@@ -963,6 +980,8 @@ hipError_t launch_update(double* omega, const double* part, const double* visc, 
 hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F, int K, const double* omega,
                               float* u, hipStream_t s)
 {
+    if (prec == PSF_PHI_FP32_ACC32)   // one column of the FP32-accumulating window kernel (the same bits)
+        return launch_reconstruct_window(prec, 1, 1, Phi, ldF, F, K, omega, u, s);
     if (prec == PSF_PHI_FP32) {
         constexpr int V = kGemvVecF32;
         const unsigned grid = static_cast<unsigned>((ldF + 256 * V - 1) / (256 * V));
@@ -980,7 +999,13 @@ hipError_t launch_reconstruct_window(int prec, int hist_ld, int ncols, const voi
     if (ldF % 256 != 0) return hipErrorInvalidValue;
     if (ncols < 1 || ncols > 32 || hist_ld < ncols) return hipErrorInvalidValue;
     const unsigned grid = static_cast<unsigned>(ldF / 256);
-    if (prec == PSF_PHI_FP32) {
+    if (prec == PSF_PHI_FP32_ACC32) {
+        const float* P = static_cast<const float*>(Phi);
+        if (ncols > 16)
+            k_reconstruct_mfma<float, 2, float><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
+        else
+            k_reconstruct_mfma<float, 1, float><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
+    } else if (prec == PSF_PHI_FP32) {
         const float* P = static_cast<const float*>(Phi);
         if (ncols > 16) k_reconstruct_mfma<float, 2><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
         else k_reconstruct_mfma<float, 1><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);

@@ -14,7 +14,7 @@ from scenes import make_scene, particles_uniform
 pytestmark = pytest.mark.gpu
 
 C_FP64, C_FP32, C_BF16X2 = 0, 1, 4
-PHI_FP64, PHI_FP32 = 0, 1
+PHI_FP64, PHI_FP32, PHI_FP32_ACC32 = 0, 1, 2
 
 
 def _say(msg):
@@ -49,7 +49,8 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("window", [2, 8, 32])
-@pytest.mark.parametrize("c_prec,phi_prec", [(C_FP64, PHI_FP64), (C_FP64, PHI_FP32), (C_BF16X2, PHI_FP32)])
+@pytest.mark.parametrize("c_prec,phi_prec", [(C_FP64, PHI_FP64), (C_FP64, PHI_FP32), (C_BF16X2, PHI_FP32),
+                                             (C_BF16X2, PHI_FP32_ACC32)])
 def test_window_bitwise_equal_single_frames(hip_ctx, window, c_prec, phi_prec):
     """Read back mid-window and at window ends: omega, u and particles equal."""
     _say(f"window={window} c={c_prec} phi={phi_prec}")
@@ -259,3 +260,59 @@ def test_device_view(hip_ctx):
     d = hip.make_desc(sd.grid, sd.resolution, sd.K, C_BF16X2, PHI_FP32, 0, sd.aabb_min, sd.aabb_max)
     b = hip.Scene(hip_ctx, d)
     assert b.device_view()["C"] is None
+
+
+@pytest.mark.parametrize("window", [1, 16, 32])
+def test_fp32_accumulation_within_bound(hip_ctx, window):
+    """PSF_PHI_FP32_ACC32 (FP32 Phi and w, FP32 sums on the matrix cores)
+    against the oracle's FP64 u = float(Phi w) (psfsolver.cpp:60), every face
+    within the a-priori bound of the three roundings -- Phi and w to FP32
+    (2^-24 each) and K FP32 additions (K 2^-24) -- times sum_k |Phi_fk w_k|,
+    plus the float output's half ulp; particles after 40 frames within 1e-6."""
+    _say(f"fp32 accumulation bound, window={window}")
+    from parallelspectralfluids3d_amd import hip
+    sd = make_scene(n=8, K=16)
+    N = 2048
+    d = hip.make_desc(sd.grid, sd.resolution, sd.K, C_FP64, PHI_FP32_ACC32, N, sd.aabb_min, sd.aabb_max)
+    s = hip.Scene(hip_ctx, d)
+    s.upload_basis(sd.Phi, sd.lam, sd.g)
+    s.upload_advection(sd.C)
+    s.set_omega(sd.omega)
+    s.set_params(0.01, 0.0, False, lifetime=60.0)
+    p = particles_uniform(sd, N, seed=31)
+    s.set_particles(p)
+    s.set_frame_window(window)
+    m = O.Mesh((sd.grid.nx, sd.grid.ny, sd.grid.nz), sd.resolution, sd.aabb_min, inside=sd.inside)
+    w = sd.omega.copy()
+    worst = 0.0
+    for f in range(40):
+        w = O.reduced_step(w, sd.C, sd.lam, sd.g, 0.01, 0.0, False)
+        u = O.reconstruct(sd.Phi, w)
+        p = O.particles(m, u, p, resolution=sd.resolution, timestep=0.01, aabb_min=sd.aabb_min,
+                        aabb_max=sd.aabb_max, lifetime=60.0, seed=0, step=f)
+        s.integrate(1)
+        ug = s.get_velocity().astype(np.float64)
+        mag = np.abs(sd.Phi * w[:, None]).sum(axis=0)
+        bound = (sd.K + 2) * 2.0 ** -24 * mag + 2.0 ** -24 * np.abs(u.astype(np.float64)) + 1e-300
+        ratio = np.abs(ug - u) / bound
+        worst = max(worst, float(ratio.max()))
+        assert np.all(ratio <= 1.0), (f, float(ratio.max()))
+    print(f"  worst error / bound {worst:.3f}", flush=True)
+    assert np.abs(s.get_particles()[:, :3] - p[:, :3]).max() <= 1e-6
+
+
+def test_fp32_accumulation_exact_columns(hip_ctx):
+    """u = Phi e_k in FP32 accumulation is column k of the FP32 Phi exactly."""
+    _say("fp32 accumulation columns")
+    from parallelspectralfluids3d_amd import hip
+    sd = make_scene(n=8, K=16)
+    d = hip.make_desc(sd.grid, sd.resolution, sd.K, C_FP64, PHI_FP32_ACC32, 0, sd.aabb_min, sd.aabb_max)
+    s = hip.Scene(hip_ctx, d)
+    s.upload_basis(sd.Phi, sd.lam, sd.g)
+    s.upload_advection(sd.C)
+    for k in (0, 7, sd.K - 1):
+        e = np.zeros(sd.K)
+        e[k] = 1.0
+        s.set_omega(e)
+        s.reconstruct()
+        assert np.array_equal(s.get_velocity(), sd.Phi[k].astype(np.float32))
