@@ -771,7 +771,15 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
 // the windowed frame loop, where the next window's reconstruction and steps
 // share the CUs, ran 5% faster than at 70 VGPRs.  Loading the next frame's
 // fluxes one frame early (with the current cell) measured 8% slower: it
-// needs 96 VGPRs (5 waves per SIMD).
+// needs 96 VGPRs (5 waves per SIMD).  Two LDS-staging variants were
+// measured and gave no gain, with the same bits: per frame, each block
+// reducing its particles' box of cells and staging the faces in it (two
+// barriers per frame) took 15.5 us per frame against 12; staging the box's
+// faces of all 8 frames once per launch (36 KiB per block, 4 blocks per CU)
+// took 12 us.  With the address unit relieved the time stays: the pass is
+// bound by its ~220 vector instructions per particle-frame (about half the
+// issue cycles) and the dependent chain from one frame's position to the
+// next frame's gathers.
 template <bool Pow2>
 __global__ __launch_bounds__(256, 8) void k_trace_frames(TraceParams P, const float* __restrict__ u0, size_t ldU,
                                                       uint32_t nframes, float4* __restrict__ parts,
