@@ -386,24 +386,45 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
         }
         step(ra, rb);
     }
+    // The stores go through a wave-private LDS tile of 16 frames x 64 faces:
+    // a lane's values of one frame are 16-byte pieces 64 bytes apart, and
+    // stores of that shape ran at about 1 TB/s (a third of the launch at
+    // W = 16).  Read back by rows, each store instruction writes 256
+    // contiguous bytes (a wave's 64 faces) of each of four frames.
+    constexpr int kTileLd = 68;   // floats per tile row (padded)
+    __shared__ __attribute__((aligned(16))) float tiles[4][16 * kTileLd];
+    float* const tile = tiles[wv];
 #pragma unroll
     for (int ct = 0; ct < NT; ++ct) {
-        if (!tv[ct]) continue;
-        const gptr<float> ut = uh + static_cast<size_t>(16 * ct + li) * ldU;
+        if (16 * ct >= W) break;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            // face of slot r, tile rt: f0 + 4 * row + rt
-            const size_t f = kF32 ? f0 + 16 * lk + 4 * r : f0 + 4 * (lk + 4 * r);
+            // local face of slot r in tile 0 (tile rt adds rt)
+            const int fl = kF32 ? 16 * lk + 4 * r : 4 * (lk + 4 * r);
             const f32x4 o = {static_cast<float>(acc[0][ct][r]), static_cast<float>(acc[1][ct][r]),
                              static_cast<float>(acc[2][ct][r]), static_cast<float>(acc[3][ct][r])};
+            *reinterpret_cast<f32x4*>(tile + li * kTileLd + fl) = o;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int row = lk + 4 * s4, t = 16 * ct + row;
+            const f32x4 o = *reinterpret_cast<const f32x4*>(tile + row * kTileLd + 4 * li);
+            if (t >= W) continue;
+            const gptr<float> ut = uh + static_cast<size_t>(t) * ldU;
+            const size_t f = f0 + 4 * li;
             if (f + 3 < F) {
-                *reinterpret_cast<gptr<f32x4>>(ut + f) = o;
+                __builtin_nontemporal_store(o, reinterpret_cast<gptr<f32x4>>(ut + f));
             } else {
                 if (f < F) ut[f] = o.x;
                 if (f + 1 < F) ut[f + 1] = o.y;
                 if (f + 2 < F) ut[f + 2] = o.z;
             }
         }
+        // the next tile's writes follow these reads
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
