@@ -643,16 +643,19 @@ ORA_EXPORT void ora_reconstruct(int F, int K, const double* Phi, const double* o
 
 /* ---- particles (psfsolver.cpp:70-232) ------------------------------------ */
 
-/* counter-based draw in [0,1024): stands in for rand()%1024 */
+/* counter-based draws in [0,1024), standing in for rand()%1024: one
+ * SplitMix64 output (Steele, Lea and Flood 2014: a Weyl increment, then the
+ * variant-13 finaliser) of a key mixing (seed, step, particle); draw k
+ * (0 <= k < 6) is bits 10k .. 10k+9 of it.  The particle step uses k = 0, 1, 2. */
 ORA_EXPORT uint32_t ora_respawn_draw(uint32_t seed, uint32_t step, uint32_t particle, uint32_t k)
 {
     uint64_t z = ((uint64_t)seed << 32) ^ ((uint64_t)step * 0x9E3779B97F4A7C15ull)
-               ^ ((uint64_t)particle * 0xD1B54A32D192ED03ull) ^ ((uint64_t)k * 0x94D049BB133111EBull);
+               ^ ((uint64_t)particle * 0xD1B54A32D192ED03ull);
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z = z ^ (z >> 31);
-    return (uint32_t)(z & 1023u);
+    return k < 6 ? (uint32_t)((z >> (10 * k)) & 1023u) : 0u;
 }
 
 static float fmix(float x, float y, float a) { return x * (1.0f - a) + y * a; }   /* glm::mix */

@@ -522,16 +522,22 @@ __device__ __forceinline__ void cell_indices_pow2(float a, float rb, int& O, int
 // always holds for the faces paired below)
 typedef float FluxPair __attribute__((ext_vector_type(2))) __attribute__((aligned(4)));
 
-__device__ __forceinline__ uint32_t respawn_draw(uint32_t seed, uint32_t step, uint32_t particle, uint32_t k)
+// The three respawn draws in [0, 1024) of (seed, step, particle): one
+// SplitMix64 output (Steele, Lea and Flood 2014: a Weyl increment, then the
+// variant-13 finaliser) of a key mixing the three counters, cut into 10-bit
+// fields, draw k = bits 10k .. 10k + 9.
+struct RespawnDraws {
+    uint32_t r0, r1, r2;
+};
+__device__ __forceinline__ RespawnDraws respawn_draws(uint32_t seed, uint32_t step, uint32_t particle)
 {
     uint64_t z = (static_cast<uint64_t>(seed) << 32) ^ (static_cast<uint64_t>(step) * 0x9E3779B97F4A7C15ull) ^
-                 (static_cast<uint64_t>(particle) * 0xD1B54A32D192ED03ull) ^
-                 (static_cast<uint64_t>(k) * 0x94D049BB133111EBull);
+                 (static_cast<uint64_t>(particle) * 0xD1B54A32D192ED03ull);
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z = z ^ (z >> 31);
-    return static_cast<uint32_t>(z & 1023u);
+    const uint32_t lo = static_cast<uint32_t>(z ^ (z >> 31));
+    return {lo & 1023u, (lo >> 10) & 1023u, (lo >> 20) & 1023u};
 }
 
 // A particle's cell indices on each axis (O: its voxel, M/P: the voxels
@@ -703,13 +709,11 @@ __device__ __forceinline__ float4 trace_advance(const TraceParams& P, const gptr
 }
 
 // respawn of an expired particle (psfsolver.cpp:219-222, with the build's
-// counter-based draws); the draws follow the caller's particle index
-__device__ __forceinline__ float4 trace_respawn(const TraceParams& P, const gptr<const uint32_t> pid, uint32_t id)
+// counter-based draws); the draws follow the caller's particle index who
+__device__ __forceinline__ float4 trace_respawn(const TraceParams& P, uint32_t who)
 {
-    const uint32_t who = pid[id];
-    const uint32_t r0 = respawn_draw(P.seed, P.frame, who, 0);
-    const uint32_t r1 = respawn_draw(P.seed, P.frame, who, 1);
-    const uint32_t r2 = respawn_draw(P.seed, P.frame, who, 2);
+    const RespawnDraws d = respawn_draws(P.seed, P.frame, who);
+    const uint32_t r0 = d.r0, r1 = d.r1, r2 = d.r2;
     float4 p;
     p.x = static_cast<float>((r0 / 1024.0 - 0.5) * 0.25);
     p.y = P.aabb_min[1] + 0.2f;
@@ -718,11 +722,11 @@ __device__ __forceinline__ float4 trace_respawn(const TraceParams& P, const gptr
     return p;
 }
 
-// one particle record through one frame; id is its storage slot (cells,
-// caller index)
-template <bool Pow2>
-__device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<const float> u, float4 p,
-                                            const gptr<const uint32_t> pid, uint32_t id, int32_t* __restrict__ cells)
+// one particle record through one frame; id is its storage slot (cells),
+// who() its caller index (called only for a respawn)
+template <bool Pow2, typename WhoFn>
+__device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<const float> u, float4 p, WhoFn who,
+                                            uint32_t id, int32_t* __restrict__ cells)
 {
     if (p.w > 0.0f) {
         const TraceCell c = trace_cell<Pow2>(P, p);
@@ -738,7 +742,7 @@ __device__ __forceinline__ float4 trace_one(const TraceParams& P, const gptr<con
 #pragma unroll
         for (int k = 0; k < 12; ++k) o[k] = -1;
     }
-    return trace_respawn(P, pid, id);
+    return trace_respawn(P, who());
 }
 
 // One counter increment per particle, with equal cells of neighbouring lanes
@@ -777,7 +781,9 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P, const float* __res
     // particle records stream through once: nontemporal, so u stays in L2
     f32x4* rec = reinterpret_cast<f32x4*>(parts) + id;
     const f32x4 v = __builtin_nontemporal_load(rec);
-    const float4 p = trace_one<Pow2>(P, as_global(u), make_float4(v.x, v.y, v.z, v.w), as_global(pid), id, cells);
+    const gptr<const uint32_t> gp = as_global(pid);
+    const float4 p = trace_one<Pow2>(P, as_global(u), make_float4(v.x, v.y, v.z, v.w), [&] { return gp[id]; }, id,
+                                     cells);
     const f32x4 o = {p.x, p.y, p.z, p.w};
     __builtin_nontemporal_store(o, rec);
     if (D.counts) density_add(D, p.x, p.y, p.z);
@@ -813,9 +819,10 @@ __global__ __launch_bounds__(256, 8) void k_trace_frames(TraceParams P, const fl
     const f32x4 v = __builtin_nontemporal_load(rec);
     float4 p = make_float4(v.x, v.y, v.z, v.w);
     const gptr<const float> u = as_global(u0);
-    const gptr<const uint32_t> gp = as_global(pid);
+    // the caller index, loaded once: a respawn then waits for no load
+    const uint32_t who = as_global(pid)[id];
     for (uint32_t t = 0; t < nframes; ++t) {
-        p = trace_one<Pow2>(P, u + static_cast<size_t>(t) * ldU, p, gp, id, nullptr);
+        p = trace_one<Pow2>(P, u + static_cast<size_t>(t) * ldU, p, [who] { return who; }, id, nullptr);
         P.frame++;
     }
     const f32x4 o = {p.x, p.y, p.z, p.w};
@@ -944,17 +951,18 @@ __global__ __launch_bounds__(256) void k_trace_window_multi(const TraceScene* __
     TraceParams P = ts.P;
     const gptr<f32x4> parts = as_global(reinterpret_cast<f32x4*>(ts.parts));
     const gptr<const float> uh = as_global(ts.u_hist);
-    const gptr<const uint32_t> pid = as_global(ts.pid);
     const f32x4 v = __builtin_nontemporal_load(&parts[id]);
     float4 p = make_float4(v.x, v.y, v.z, v.w);
+    const uint32_t who = as_global(ts.pid)[id];   // loaded once (see k_trace_frames)
+    const auto whofn = [who] { return who; };
     if (P.pow2) {
         for (uint32_t t = 0; t < ts.nframes; ++t) {
-            p = trace_one<true>(P, uh + static_cast<size_t>(t) * ts.ldU, p, pid, id, nullptr);
+            p = trace_one<true>(P, uh + static_cast<size_t>(t) * ts.ldU, p, whofn, id, nullptr);
             P.frame++;
         }
     } else {
         for (uint32_t t = 0; t < ts.nframes; ++t) {
-            p = trace_one<false>(P, uh + static_cast<size_t>(t) * ts.ldU, p, pid, id, nullptr);
+            p = trace_one<false>(P, uh + static_cast<size_t>(t) * ts.ldU, p, whofn, id, nullptr);
             P.frame++;
         }
     }

@@ -123,6 +123,18 @@ def test_respawn_formula():
     assert draws.min() >= 0 and draws.max() < 1024 and 400 < draws.mean() < 624
 
 
+def test_respawn_draws_are_splitmix64_fields():
+    """With seed = particle = 0 the key is step * 0x9E3779B97F4A7C15, so the
+    hash is SplitMix64's output number step + 1 from state 0; the first three
+    published outputs pin it, and draw k is bits 10k..10k+9 of the output."""
+    outputs = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    for step, z in enumerate(outputs):
+        assert [O.respawn_draw(0, step, 0, k) for k in range(6)] == [(z >> (10 * k)) & 1023 for k in range(6)]
+    # the seed and the particle index enter the key
+    assert O.respawn_draw(1, 0, 0, 0) != O.respawn_draw(0, 0, 0, 0) or O.respawn_draw(1, 0, 0, 1) != O.respawn_draw(0, 0, 0, 1)
+    assert [O.respawn_draw(0, 0, 1, k) for k in range(3)] != [O.respawn_draw(0, 0, 0, k) for k in range(3)]
+
+
 @pytest.mark.parametrize("nthreads", [1, 4])
 def test_deterministic_across_threads(nthreads):
     g = Grid(N_GRID, N_GRID, N_GRID)
