@@ -158,6 +158,9 @@ SERVE_K0 = 192
 # {default, 2, 4, 8} measured on the 512-request stream (0 = per-request default)
 SERVE_MPW_BATCHED = 0
 SERVE_MPW_ALONE = 2
+# velocity fields with FP32 sums (PSF_PHI_FP32_ACC32), as in the frame loop, in
+# every serving run
+SERVE_U_FP32_SUMS = True
 
 
 def serving_workload(n_requests, rate, seed=0, particles=1 << 14):
@@ -197,13 +200,13 @@ def serving_bases():
     return out
 
 
-def run_stream(ctx, bases, reqs, max_active, window=32, saturate=False, modes_per_wg=0):
+def run_stream(ctx, bases, reqs, max_active, window=32, saturate=False, modes_per_wg=0, u_fp32_sums=False):
     """Feeds the requests to a server at their arrival times (all at once when
     saturate) and runs rounds until every request has finished.  Returns the
     throughput over the makespan and the latency of each request from its
     arrival."""
     from parallelspectralfluids3d_amd import hip
-    sv = hip.Server(ctx, window=window, max_active=max_active, modes_per_wg=modes_per_wg)
+    sv = hip.Server(ctx, window=window, max_active=max_active, modes_per_wg=modes_per_wg, u_fp32_sums=u_fp32_sums)
     bid = {n: sv.add_basis(b["grid"], b["h"], b["Phi"], b["lam"], b["g"], b["C"]) for n, b in bases.items()}
     pending = sorted(range(len(reqs)), key=lambda j: reqs[j]["arrival"])
     outstanding = {}
@@ -256,7 +259,8 @@ def dropin_sequential(ctx, bases, reqs):
     for r in reqs:
         b = bases[r["grid"]]
         K = r["K"]
-        d = hip.make_desc(b["grid"], b["h"], K, L.C_BF16X2, L.PHI_FP32, r["parts"].shape[0])
+        d = hip.make_desc(b["grid"], b["h"], K, L.C_BF16X2, L.PHI_FP32_ACC32 if SERVE_U_FP32_SUMS else L.PHI_FP32,
+                          r["parts"].shape[0])
         s = hip.Scene(ctx, d)
         s.upload_basis(b["Phi"][:K], b["lam"][:K], b["g"][:K])
         s.upload_advection(np.ascontiguousarray(b["C"][:K, :K, :K]))
@@ -280,13 +284,15 @@ def serving_bench(ctx, n_requests, rate):
     # per-request default (the most modes whose C blocks fit in registers:
     # fewer workgroups per scene, more scenes per persistent launch); one at a
     # time, two (more workgroups share a lone scene's contraction).
-    batched = run_stream(ctx, bases, reqs, max_active=64, modes_per_wg=SERVE_MPW_BATCHED)
+    batched = run_stream(ctx, bases, reqs, max_active=64, modes_per_wg=SERVE_MPW_BATCHED,
+                         u_fp32_sums=SERVE_U_FP32_SUMS)
     log(f"serving batched: {batched['scene_frames_per_s']:.0f} scene-frames/s, "
         f"p50 {batched['latency_ms_p50']:.1f} ms, p99 {batched['latency_ms_p99']:.1f} ms")
-    alone = run_stream(ctx, bases, reqs, max_active=1, modes_per_wg=SERVE_MPW_ALONE)
+    alone = run_stream(ctx, bases, reqs, max_active=1, modes_per_wg=SERVE_MPW_ALONE, u_fp32_sums=SERVE_U_FP32_SUMS)
     log(f"serving one at a time: {alone['scene_frames_per_s']:.0f} scene-frames/s, "
         f"p50 {alone['latency_ms_p50']:.1f} ms, p99 {alone['latency_ms_p99']:.1f} ms")
-    sat = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED)
+    sat = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED,
+                     u_fp32_sums=SERVE_U_FP32_SUMS)
     log(f"serving batched, all queued at once: {sat['scene_frames_per_s']:.0f} scene-frames/s")
     nd = min(16, n_requests)
     dropin = dropin_sequential(ctx, bases, reqs[:nd])
@@ -294,7 +300,8 @@ def serving_bench(ctx, n_requests, rate):
         f"scene-frames/s")
     return {
         "workload": f"{n_requests} requests, Poisson arrivals at {rate:g}/s, grids {SERVE_GRIDS}^3, K in [16, 192], "
-                    f"frames in [100, 2000], 2^14 particles each; one K0={SERVE_K0} basis per grid",
+                    f"frames in [100, 2000], 2^14 particles each; one K0={SERVE_K0} basis per grid; BF16 hi+lo C, "
+                    f"u with {'FP32' if SERVE_U_FP32_SUMS else 'FP64'} sums",
         "modes_per_wg": {"batched": SERVE_MPW_BATCHED or "per-request default", "one_at_a_time": SERVE_MPW_ALONE},
         "batched": batched,
         "one_at_a_time": alone,
@@ -657,7 +664,8 @@ def main():
         ctx = hip.Context(0)
         bases = serving_bases()
         reqs = serving_workload(args.serving_profile, args.serving_rate)
-        r = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED)
+        r = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED,
+                       u_fp32_sums=SERVE_U_FP32_SUMS)
         ctx.close()
         print(json.dumps({"serving_profile": args.serving_profile, **r}), flush=True)
         return

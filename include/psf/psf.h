@@ -385,6 +385,8 @@ typedef struct psf_server_config {
     uint64_t mem_budget;    /* device bytes for admitted requests (0 = 64 GiB) */
     uint32_t modes_per_wg;  /* modes per workgroup of the persistent layout, 1..8 (0 = per request, the
                                most that keep all of its C blocks in registers) */
+    uint32_t u_fp32_sums;   /* 0: velocity fields u = Phi w with FP64 sums (PSF_PHI_FP32); 1: FP32 sums
+                               on the FP32 matrix cores (PSF_PHI_FP32_ACC32) */
 } psf_server_config;
 
 typedef struct psf_server_stats {

@@ -436,12 +436,12 @@ __global__ __launch_bounds__(256) void k_reconstruct_mfma(const T* __restrict__ 
     mfma_window_body<T, NT, ACC>(Phi, ldF, F, K, hist, hist_ld, W, uh, ldU, blockIdx.x);
 }
 
-template <int NT>
+template <int NT, typename ACC>
 __global__ __launch_bounds__(256) void k_reconstruct_mfma_multi(const WindowScene* __restrict__ tab, int n, int W)
 {
     const WindowScene& w = tab[scene_of_block(tab, n, blockIdx.x)];
-    mfma_window_body<float, NT>(static_cast<const float*>(w.Phi), w.ldF, w.F, w.K, w.hist, W, W, w.uh, w.ldF,
-                                blockIdx.x - w.block0);
+    mfma_window_body<float, NT, ACC>(static_cast<const float*>(w.Phi), w.ldF, w.F, w.K, w.hist, W, W, w.uh, w.ldF,
+                                     blockIdx.x - w.block0);
 }
 
 // out[k] = hist[k * ld + t]: one column of a window history
@@ -1058,10 +1058,15 @@ hipError_t launch_reconstruct_window_multi(int prec, int window, const WindowSce
                                            hipStream_t s)
 {
     if (n <= 0 || blocks == 0) return hipSuccess;
-    if (prec != PSF_PHI_FP32) return hipErrorInvalidValue;
+    if (prec != PSF_PHI_FP32 && prec != PSF_PHI_FP32_ACC32) return hipErrorInvalidValue;
     if (window != 2 && window != 4 && window != 8 && window != 16 && window != 32) return hipErrorInvalidValue;
-    if (window == 32) k_reconstruct_mfma_multi<2><<<blocks, 256, 0, s>>>(d_tab, n, window);
-    else k_reconstruct_mfma_multi<1><<<blocks, 256, 0, s>>>(d_tab, n, window);
+    if (prec == PSF_PHI_FP32_ACC32) {
+        if (window == 32) k_reconstruct_mfma_multi<2, float><<<blocks, 256, 0, s>>>(d_tab, n, window);
+        else k_reconstruct_mfma_multi<1, float><<<blocks, 256, 0, s>>>(d_tab, n, window);
+    } else {
+        if (window == 32) k_reconstruct_mfma_multi<2, double><<<blocks, 256, 0, s>>>(d_tab, n, window);
+        else k_reconstruct_mfma_multi<1, double><<<blocks, 256, 0, s>>>(d_tab, n, window);
+    }
     return hipGetLastError();
 }
 

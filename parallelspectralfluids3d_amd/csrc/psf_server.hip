@@ -148,7 +148,7 @@ int admit(psf_server* sv, Request* r)
     psf_scene_desc d = b.desc;
     d.K = r->K;
     d.c_precision = PSF_C_BF16X2;
-    d.phi_precision = PSF_PHI_FP32;
+    d.phi_precision = sv->cfg.u_fp32_sums ? PSF_PHI_FP32_ACC32 : PSF_PHI_FP32;
     d.max_particles = r->n;
     SceneOptions o;
     o.modes_per_wg = r->ipw;
@@ -250,6 +250,7 @@ int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** o
         return fail(ctx, PSF_ERR_ARG, "server window must be 2, 4, 8, 16 or 32");
     if (c.max_active > kMaxActive) return fail(ctx, PSF_ERR_ARG, "max_active must be at most 64");
     if (c.modes_per_wg > 8) return fail(ctx, PSF_ERR_ARG, "modes_per_wg must be at most 8");
+    if (c.u_fp32_sums > 1) return fail(ctx, PSF_ERR_ARG, "u_fp32_sums must be 0 or 1");
     SRV_HIPCHK(ctx, hipSetDevice(ctx->device));
     psf_server* sv = new (std::nothrow) psf_server();
     if (!sv) return PSF_ERR_NOMEM;
@@ -482,7 +483,8 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
         SRV_HIPCHK(ctx, hipEventRecord(sv->ev_step[q], sa));
         SRV_HIPCHK(ctx, hipStreamWaitEvent(sr, sv->ev_step[q], 0));
         SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_wtab[q], sv->h_wtab[q], sizeof(WindowScene) * n, hipMemcpyHostToDevice, sr));
-        SRV_HIPCHK(ctx, launch_reconstruct_window_multi(PSF_PHI_FP32, W, sv->d_wtab[q], n, wblocks, sr));
+        SRV_HIPCHK(ctx, launch_reconstruct_window_multi(sv->cfg.u_fp32_sums ? PSF_PHI_FP32_ACC32 : PSF_PHI_FP32, W,
+                                                        sv->d_wtab[q], n, wblocks, sr));
         SRV_HIPCHK(ctx, hipEventRecord(sv->ev_rec[q], sr));
         SRV_HIPCHK(ctx, hipStreamWaitEvent(sb, sv->ev_rec[q], 0));
         SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_ttab[q], sv->h_ttab[q], sizeof(TraceScene) * n, hipMemcpyHostToDevice, sb));

@@ -48,11 +48,12 @@ def _run(sv, bid, reqs, dt=0.01):
     return [sv.result(i) for i in ids]
 
 
-def test_alone_equals_mix_bitwise(hip_ctx):
-    _say("server alone vs mix")
+@pytest.mark.parametrize("u_fp32_sums", [False, True])
+def test_alone_equals_mix_bitwise(hip_ctx, u_fp32_sums):
+    _say(f"server alone vs mix, u_fp32_sums={u_fp32_sums}")
     sd = make_scene(n=8, K=K0)
     reqs = _requests(sd)
-    sv, bid = _server(hip_ctx, sd, window=16)
+    sv, bid = _server(hip_ctx, sd, window=16, u_fp32_sums=u_fp32_sums)
     mix = _run(sv, bid, reqs)
     st = sv.stats()
     assert st["completed"] == len(reqs) and st["failed"] == 0
@@ -63,13 +64,15 @@ def test_alone_equals_mix_bitwise(hip_ctx):
     sv.close()
 
 
-def test_server_matches_oracle(hip_ctx):
+@pytest.mark.parametrize("u_fp32_sums", [False, True])
+def test_server_matches_oracle(hip_ctx, u_fp32_sums):
     """omega within the BF16 hi+lo budget of the FP64 oracle; particles within
-    1e-5 (they see u from that omega) after up to 70 frames with respawns."""
-    _say("server vs oracle")
+    1e-5 (they see u from that omega, with FP64 or FP32 sums) after up to 70
+    frames with respawns."""
+    _say(f"server vs oracle, u_fp32_sums={u_fp32_sums}")
     sd = make_scene(n=8, K=K0)
     reqs = _requests(sd, seed=1)
-    sv, bid = _server(hip_ctx, sd, window=8)
+    sv, bid = _server(hip_ctx, sd, window=8, u_fp32_sums=u_fp32_sums)
     res = _run(sv, bid, reqs)
     m = O.Mesh((sd.grid.nx, sd.grid.ny, sd.grid.nz), sd.resolution, sd.aabb_min, inside=sd.inside)
     for (K, frames, w0, p0), (wg, pg, lat) in zip(reqs, res):
