@@ -161,6 +161,9 @@ SERVE_MPW_ALONE = 2
 # velocity fields with FP32 sums (PSF_PHI_FP32_ACC32), as in the frame loop, in
 # every serving run
 SERVE_U_FP32_SUMS = True
+# batched runs: the scenes of a grid read one register-resident copy of its
+# C; one at a time: each request's own C (K modes), the fastest lone-scene form
+SERVE_SHARED_BATCHED = True
 
 
 def serving_workload(n_requests, rate, seed=0, particles=1 << 14):
@@ -200,13 +203,15 @@ def serving_bases():
     return out
 
 
-def run_stream(ctx, bases, reqs, max_active, window=32, saturate=False, modes_per_wg=0, u_fp32_sums=False):
+def run_stream(ctx, bases, reqs, max_active, window=32, saturate=False, modes_per_wg=0, u_fp32_sums=False,
+               shared_basis=False):
     """Feeds the requests to a server at their arrival times (all at once when
     saturate) and runs rounds until every request has finished.  Returns the
     throughput over the makespan and the latency of each request from its
     arrival."""
     from parallelspectralfluids3d_amd import hip
-    sv = hip.Server(ctx, window=window, max_active=max_active, modes_per_wg=modes_per_wg, u_fp32_sums=u_fp32_sums)
+    sv = hip.Server(ctx, window=window, max_active=max_active, modes_per_wg=modes_per_wg, u_fp32_sums=u_fp32_sums,
+                    shared_basis=shared_basis)
     bid = {n: sv.add_basis(b["grid"], b["h"], b["Phi"], b["lam"], b["g"], b["C"]) for n, b in bases.items()}
     pending = sorted(range(len(reqs)), key=lambda j: reqs[j]["arrival"])
     outstanding = {}
@@ -285,15 +290,18 @@ def serving_bench(ctx, n_requests, rate):
     # fewer workgroups per scene, more scenes per persistent launch); one at a
     # time, two (more workgroups share a lone scene's contraction).
     batched = run_stream(ctx, bases, reqs, max_active=64, modes_per_wg=SERVE_MPW_BATCHED,
-                         u_fp32_sums=SERVE_U_FP32_SUMS)
+                         u_fp32_sums=SERVE_U_FP32_SUMS, shared_basis=SERVE_SHARED_BATCHED)
     log(f"serving batched: {batched['scene_frames_per_s']:.0f} scene-frames/s, "
         f"p50 {batched['latency_ms_p50']:.1f} ms, p99 {batched['latency_ms_p99']:.1f} ms")
     alone = run_stream(ctx, bases, reqs, max_active=1, modes_per_wg=SERVE_MPW_ALONE, u_fp32_sums=SERVE_U_FP32_SUMS)
     log(f"serving one at a time: {alone['scene_frames_per_s']:.0f} scene-frames/s, "
         f"p50 {alone['latency_ms_p50']:.1f} ms, p99 {alone['latency_ms_p99']:.1f} ms")
     sat = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED,
-                     u_fp32_sums=SERVE_U_FP32_SUMS)
+                     u_fp32_sums=SERVE_U_FP32_SUMS, shared_basis=SERVE_SHARED_BATCHED)
     log(f"serving batched, all queued at once: {sat['scene_frames_per_s']:.0f} scene-frames/s")
+    sat_own = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED,
+                         u_fp32_sums=SERVE_U_FP32_SUMS, shared_basis=False)
+    log(f"serving batched with per-request C, all queued at once: {sat_own['scene_frames_per_s']:.0f} scene-frames/s")
     nd = min(16, n_requests)
     dropin = dropin_sequential(ctx, bases, reqs[:nd])
     log(f"serving: first {nd} requests one after another through the drop-in, frame by frame: {dropin:.0f} "
@@ -306,6 +314,8 @@ def serving_bench(ctx, n_requests, rate):
         "batched": batched,
         "one_at_a_time": alone,
         "batched_all_queued": sat,
+        "batched_all_queued_per_request_c": sat_own,
+        "shared_basis": {"batched": SERVE_SHARED_BATCHED, "one_at_a_time": False},
         "throughput_vs_one_at_a_time": batched["scene_frames_per_s"] / alone["scene_frames_per_s"],
         "capacity_vs_one_at_a_time": sat["scene_frames_per_s"] / alone["scene_frames_per_s"],
         "dropin_frame_by_frame": {"requests": nd, "scene_frames_per_s": dropin,
@@ -665,7 +675,7 @@ def main():
         bases = serving_bases()
         reqs = serving_workload(args.serving_profile, args.serving_rate)
         r = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED,
-                       u_fp32_sums=SERVE_U_FP32_SUMS)
+                       u_fp32_sums=SERVE_U_FP32_SUMS, shared_basis=SERVE_SHARED_BATCHED)
         ctx.close()
         print(json.dumps({"serving_profile": args.serving_profile, **r}), flush=True)
         return

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define PSF_ABI_VERSION 2
+#define PSF_ABI_VERSION 3
 
 #if defined(__GNUC__)
 #define PSF_API __attribute__((visibility("default")))
@@ -376,8 +376,12 @@ PSF_API int psf_time_phase(psf_scene* scene, int phase, int count, double* ms);
  * the persistent launch and device memory (the request's footprint against
  * mem_budget) are free; finished ones leave at the end of their round.
  * Scenes share one basis per grid (Phi, C of K0 modes); a request uses its
- * first K modes, packed on the device.  A request's result does not depend
- * on what else runs in its rounds.  Frames are integrate() frames
+ * first K modes.  With shared_basis = 0 each request's C (K modes) is packed
+ * on the device and held in registers by its own workgroups; with
+ * shared_basis = 1 a basis's C is packed once (K0 <= 256) and one
+ * register-resident copy serves up to 16 of its scenes, each a pair of B
+ * columns of the same MFMA blocks.  A request's result does not depend on
+ * what else runs in its rounds.  Frames are integrate() frames
  * (psfsolver.cpp:21-234) with C in BF16 hi+lo, FP32 Phi and FP64 omega. */
 typedef struct psf_server_config {
     uint32_t window;        /* frames per round: 2, 4, 8, 16 or 32 (0 = 32) */
@@ -387,6 +391,8 @@ typedef struct psf_server_config {
                                most that keep all of its C blocks in registers) */
     uint32_t u_fp32_sums;   /* 0: velocity fields u = Phi w with FP64 sums (PSF_PHI_FP32); 1: FP32 sums
                                on the FP32 matrix cores (PSF_PHI_FP32_ACC32) */
+    uint32_t shared_basis;  /* 0: each request's reduced steps read its own packed C (modes_per_wg applies);
+                               1: the scenes of a basis read one shared copy of its C (K0 <= 256) */
 } psf_server_config;
 
 typedef struct psf_server_stats {

@@ -22,7 +22,7 @@ ARCH = os.environ.get("PSF_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["psf_kernels.hip", "psf_persistent.hip", "psf_api.hip", "psf_server.hip", "psf_box.hip",
            "psf_density.hip", "psf_cache.hip", "psf_eigen.hip",
-           "psf_advection.hip", "psf_blas.hip", "psf_noalloc.hip"]
+           "psf_advection.hip", "psf_blas.hip", "psf_noalloc.hip", "psf_group.hip"]
 
 CXXFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -608,7 +608,8 @@ int psf::scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOp
     s->path = is_plain_precision(desc->c_precision) ? PSF_PATH_PLAIN : PSF_PATH_PERSISTENT;
     const size_t K = desc->K;
     s->dev.ldF = round_up(s->F, 256);
-    s->dev.C_bytes = is_plain_precision(desc->c_precision)
+    s->dev.C_bytes = opt.no_c ? 0
+                     : is_plain_precision(desc->c_precision)
                          ? c_bytes(*desc)
                          : persistent_c_bytes(desc->c_precision, static_cast<int>(K), opt.modes_per_wg);
     const size_t phi_bytes = s->dev.ldF * K * (desc->phi_precision == PSF_PHI_FP64 ? 8 : 4);
@@ -632,7 +633,7 @@ int psf::scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOp
     chk(salloc(s, &s->dev.parts, desc->max_particles));
     if (opt.cells) chk(salloc(s, &s->dev.cells, size_t(desc->max_particles) * 12));
     chk(salloc(s, &s->dev.pid, desc->max_particles));
-    if (e == hipSuccess) e = persistent_scene_init(s);
+    if (e == hipSuccess && !opt.no_c) e = persistent_scene_init(s);
     if (e != hipSuccess) {
         psf_scene_destroy(s);
         return set_err(ctx, e == hipErrorOutOfMemory ? PSF_ERR_NOMEM : PSF_ERR_HIP, "scene allocation: %s",

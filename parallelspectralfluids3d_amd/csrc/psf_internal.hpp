@@ -15,6 +15,7 @@ struct SceneOptions {
     bool pooled = false;           // take device memory from the context's pool
     bool frame_pipeline = true;    // frame windows keep a second buffer pair and a side stream
     bool async_uploads = false;    // set-up calls do not wait for their copies (host sources must outlive them)
+    bool no_c = false;             // no structure tensor of its own: its steps run from a shared basis's C
 };
 
 int scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOptions& opt, psf_scene** out);

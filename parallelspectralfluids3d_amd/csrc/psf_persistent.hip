@@ -45,6 +45,7 @@
 #include <vector>
 
 #include "psf_persistent.hpp"
+#include "psf_wave.hpp"
 
 namespace psf {
 
@@ -161,40 +162,6 @@ int persistent_workgroups(int prec, int K, int ipw_req)
     int G, ipw, nb, nBi, bpw;
     layout(prec, K, ipw_req, G, ipw, nb, nBi, bpw);
     return G;
-}
-
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
-{
-    uint32_t b = __float_as_uint(f);
-    b += 0x7FFFu + ((b >> 16) & 1u);
-    return static_cast<uint16_t>(b >> 16);
-}
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
-
-// Wave-wide FP64 sum on DPP lane moves (no LDS traffic): quad butterflies,
-// half-row and row mirrors, then the row broadcasts of lanes 15 and 31; lane
-// 63 ends with the total, which is broadcast.  The order is fixed, so every
-// wave of every workgroup rounds identically.
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ double dpp_move(double v)
-{
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double wave_total(double v)
-{
-    v += dpp_move<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
-    v += dpp_move<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
-    v += dpp_move<0x141, 0xF>(v);   // row_half_mirror
-    v += dpp_move<0x140, 0xF>(v);   // row_mirror
-    v += dpp_move<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
-    v += dpp_move<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
-    return __hiloint2double(hi, lo);
 }
 
 template <int ELEM>
@@ -539,21 +506,30 @@ hipError_t persistent_scene_init(psf_scene* s)
     if (e != hipSuccess) return e;
     // block metadata (identical for every workgroup)
     std::vector<uint32_t> meta;
-    meta.reserve(p->bpw);
-    for (int t = 0; t < p->ipw; ++t)
-        for (int ri = 0; ri < p->nb; ++ri) {
-            const int pr = row_at(ri, p->nb);
+    int G = 0, ipw = 0, bpw = 0;
+    persistent_layout_meta(prec, K, s->modes_per_wg, G, ipw, bpw, meta);
+    return hipMemcpy(p->meta, meta.data(), sizeof(uint32_t) * meta.size(), hipMemcpyHostToDevice);
+}
+
+void persistent_layout_meta(int prec, int K, int ipw_req, int& G, int& ipw, int& bpw, std::vector<uint32_t>& meta)
+{
+    int nb = 0, nBi = 0;
+    layout(prec, K, ipw_req, G, ipw, nb, nBi, bpw);
+    meta.clear();
+    meta.reserve(bpw);
+    for (int t = 0; t < ipw; ++t)
+        for (int ri = 0; ri < nb; ++ri) {
+            const int pr = row_at(ri, nb);
             if (prec == PSF_C_BF16X2) {
-                for (int q = pr; q < p->nb; ++q)
+                for (int q = pr; q < nb; ++q)
                     meta.push_back(uint32_t(t) << 24 | uint32_t(pr) << 16 | uint32_t(q) << 8 | uint32_t(q));
             } else {
-                for (int q = pr; q < p->nb; q += 2) {
-                    const int q2 = q + 1 < p->nb ? q + 1 : q;
+                for (int q = pr; q < nb; q += 2) {
+                    const int q2 = q + 1 < nb ? q + 1 : q;
                     meta.push_back(uint32_t(t) << 24 | uint32_t(pr) << 16 | uint32_t(q) << 8 | uint32_t(q2));
                 }
             }
         }
-    return hipMemcpy(p->meta, meta.data(), sizeof(uint32_t) * meta.size(), hipMemcpyHostToDevice);
 }
 
 void persistent_scene_free(psf_scene* s)
@@ -863,10 +839,16 @@ hipError_t launch_pack_c(psf_scene* s, const double* Cf, int K0, hipStream_t str
     PersistentState* p = st(s);
     const int prec = s->d.c_precision;
     if (!p || (prec != PSF_C_BF16 && prec != PSF_C_BF16X2) || K0 < static_cast<int>(s->d.K)) return hipErrorInvalidValue;
-    const size_t total = std::max(static_cast<size_t>(p->G) * p->bpw * 64, static_cast<size_t>(p->G) * p->ipw);
+    return launch_pack_c_layout(Cf, K0, static_cast<int>(s->d.K), p->G, p->ipw, p->bpw, prec == PSF_C_BF16X2 ? 1 : 0,
+                                p->meta, p->blocks, p->inv_scale, stream);
+}
+
+hipError_t launch_pack_c_layout(const double* Cf, int K0, int K, int G, int ipw, int bpw, int hilo,
+                                const uint32_t* meta, uint4* blocks, double* inv_scale, hipStream_t stream)
+{
+    const size_t total = std::max(static_cast<size_t>(G) * bpw * 64, static_cast<size_t>(G) * ipw);
     const unsigned grid = static_cast<unsigned>((total + 255) / 256);
-    k_pack_c<<<grid, 256, 0, stream>>>(Cf, K0, static_cast<int>(s->d.K), p->G, p->ipw, p->bpw,
-                                       prec == PSF_C_BF16X2 ? 1 : 0, p->meta, p->blocks, p->inv_scale);
+    k_pack_c<<<grid, 256, 0, stream>>>(Cf, K0, K, G, ipw, bpw, hilo, meta, blocks, inv_scale);
     return hipGetLastError();
 }
 

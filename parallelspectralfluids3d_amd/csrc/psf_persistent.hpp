@@ -1,5 +1,7 @@
 // Persistent reduced-step path (MFMA precisions): one launch runs n steps.
 #pragma once
+#include <vector>
+
 #include "psf_common.hpp"
 
 namespace psf {
@@ -33,5 +35,15 @@ hipError_t launch_persistent_batch(PersistentBatch* b, psf_scene* const* sc, con
 // Packs C of the scene's K modes (BF16 or BF16 hi+lo) on the device from an
 // FP64 tensor of K0 >= K modes, C_i(a, b) at Cf[(i * K0 + b) * K0 + a].
 hipError_t launch_pack_c(psf_scene* s, const double* Cf, int K0, hipStream_t st);
+
+// The block layout of K modes at ipw_req modes per workgroup (0: one per
+// workgroup): G workgroups of bpw blocks, ipw modes each, and the metadata
+// word of each block (t << 24 | row tile << 16 | q1 << 8 | q2).
+void persistent_layout_meta(int c_precision, int K, int ipw_req, int& G, int& ipw, int& bpw,
+                            std::vector<uint32_t>& meta);
+// Packs the first K modes of the FP64 tensor Cf (K0 modes) into that layout
+// (hilo: BF16 hi+lo blocks, else BF16); inv_scale receives G * ipw ones.
+hipError_t launch_pack_c_layout(const double* Cf, int K0, int K, int G, int ipw, int bpw, int hilo,
+                                const uint32_t* meta, uint4* blocks, double* inv_scale, hipStream_t st);
 
 }  // namespace psf

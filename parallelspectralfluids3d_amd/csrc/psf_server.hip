@@ -33,6 +33,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "psf_group.hpp"
 #include "psf_internal.hpp"
 #include "psf_kernels.hpp"
 #include "psf_persistent.hpp"
@@ -43,6 +44,7 @@ using Clock = std::chrono::steady_clock;
 namespace {
 
 constexpr uint32_t kMaxActive = 64;
+constexpr int kMaxGroupInstances = 16;   // shared-basis instances per launch
 
 struct Basis {
     psf_scene_desc desc{};       // grid; K0 = desc.K
@@ -50,6 +52,8 @@ struct Basis {
     size_t ldF = 0;
     float* Phi = nullptr;        // ldF x K0, FP32, rows F..ldF-1 zero
     double* C = nullptr;         // K0^3, FP64
+    double* gravity_dev = nullptr;   // K0 (shared-basis steps)
+    GroupBasis* group = nullptr;     // C's register blocks (shared-basis steps)
     std::vector<double> lambda, gravity;
 };
 
@@ -86,6 +90,7 @@ struct psf_server {
     uint64_t next_id = 1;
     // per round parity: launch tables (pinned staging and device copies)
     PersistentBatch* batch[2] = {nullptr, nullptr};
+    GroupBatch* gbatch[2] = {nullptr, nullptr};   // shared-basis steps
     WindowScene* d_wtab[2] = {nullptr, nullptr};
     WindowScene* h_wtab[2] = {nullptr, nullptr};
     TraceScene* d_ttab[2] = {nullptr, nullptr};
@@ -125,15 +130,18 @@ int request_ipw(const psf_server* sv, uint32_t K)
 // Device bytes of a request, as the pool holds them.
 uint64_t request_bytes(const psf_server* sv, const Basis& b, uint32_t K, uint32_t n)
 {
+    // a shared-basis request holds no C of its own (the basis holds it once)
+    const bool own_c = !sv->cfg.shared_basis;
     const int ipw = request_ipw(sv, K);
-    const size_t cb = persistent_c_bytes(PSF_C_BF16X2, static_cast<int>(K), ipw);
-    const int G = persistent_workgroups(PSF_C_BF16X2, static_cast<int>(K), ipw);
+    const size_t cb = own_c ? persistent_c_bytes(PSF_C_BF16X2, static_cast<int>(K), ipw) : 0;
+    const int G = own_c ? persistent_workgroups(PSF_C_BF16X2, static_cast<int>(K), ipw) : 0;
     const size_t bpw = G ? cb / (size_t(G) * 1024) : 0;
     const size_t Wn = sv->cfg.window;
     const size_t sizes[] = {
         K * 8ull, K * 8ull, K * 8ull, K * 8ull, K * 8ull, 64 * 4ull,   // omega lambda gravity visc y status
         cb, b.F * 4ull, n * 16ull, n * 4ull,                            // C u parts pid
-        bpw * 4, size_t(G) * ((K + G - 1) / G) * 8, 2ull * 16 * K * 8,   // meta inv_scale xbuf
+        bpw * 4, G ? size_t(G) * ((K + G - 1) / G) * 8 : 0,              // meta inv_scale
+        own_c ? 2ull * 16 * K * 8 : 0,                                  // xbuf
         K * Wn * 8, b.ldF * Wn * 4, K * 8ull, K * Wn * 8,               // hist u_hist omega_view hist2
         b.ldF * Wn * 4,                                                  // u_hist2
     };
@@ -157,6 +165,7 @@ int admit(psf_server* sv, Request* r)
     o.pooled = true;
     o.frame_pipeline = false;   // the server pipelines rounds itself
     o.async_uploads = true;     // sources below live in the request and the basis; P waits on ev_upload
+    o.no_c = sv->cfg.shared_basis != 0;   // steps read the basis's C
     psf_scene* s = nullptr;
     int rc = scene_create_ex(sv->ctx, &d, o, &s);
     if (rc) return rc;
@@ -167,7 +176,7 @@ int admit(psf_server* sv, Request* r)
     SRV_HIPCHK(sv->ctx, scene_alloc(s, reinterpret_cast<void**>(&r->hist2), sizeof(double) * r->K * sv->cfg.window));
     SRV_HIPCHK(sv->ctx, scene_alloc(s, reinterpret_cast<void**>(&r->u_hist2), sizeof(float) * s->dev.ldF * sv->cfg.window));
     if ((rc = psf_scene_upload_basis(s, nullptr, b.lambda.data(), b.gravity.data()))) return rc;
-    SRV_HIPCHK(sv->ctx, launch_pack_c(s, b.C, static_cast<int>(b.desc.K), sv->ctx->stream));
+    if (!sv->cfg.shared_basis) SRV_HIPCHK(sv->ctx, launch_pack_c(s, b.C, static_cast<int>(b.desc.K), sv->ctx->stream));
     s->has_C = true;
     if ((rc = psf_set_params(s, r->dt, r->nu, r->grav, r->lifetime))) return rc;
     if ((rc = psf_set_omega(s, r->omega.data()))) return rc;
@@ -200,6 +209,25 @@ struct UploadStream {
     ~UploadStream() { ctx->stream = saved; }
 };
 
+// Shared-basis steps: whether the next round's launch holds every active
+// scene with frames left plus r (instances of up to kGroupMaxScenes scenes
+// per basis, their workgroups resident at once).
+bool group_fits(const psf_server* sv, const Request* r)
+{
+    std::vector<int> cnt(sv->bases.size(), 0);
+    for (const Request* a : sv->active)
+        if (a->sched < a->nframes) cnt[a->basis]++;
+    cnt[r->basis]++;
+    int wgs = 0, inst = 0;
+    for (size_t b = 0; b < cnt.size(); ++b) {
+        if (!cnt[b]) continue;
+        const int ni = (cnt[b] + kGroupMaxScenes - 1) / kGroupMaxScenes;
+        inst += ni;
+        wgs += ni * group_basis_workgroups(sv->bases[b].group);
+    }
+    return wgs <= sv->wg_capacity && inst <= group_batch_max_instances(sv->gbatch[0]);
+}
+
 // Host time one round call may spend admitting while a round runs: longer
 // admission would leave the device idle once the round in flight ends.
 constexpr double kAdmitBudgetMs = 0.25;
@@ -213,6 +241,7 @@ void admit_queued(psf_server* sv, bool device_idle)
             break;
         Request* r = sv->queue.front();
         if (sv->wgs_in_use + r->wgs > sv->wg_capacity) break;
+        if (sv->cfg.shared_basis && !group_fits(sv, r)) break;
         if (sv->mem_in_use + r->footprint > sv->cfg.mem_budget) break;
         sv->queue.pop_front();
         sv->wgs_in_use += r->wgs;
@@ -251,6 +280,7 @@ int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** o
     if (c.max_active > kMaxActive) return fail(ctx, PSF_ERR_ARG, "max_active must be at most 64");
     if (c.modes_per_wg > 8) return fail(ctx, PSF_ERR_ARG, "modes_per_wg must be at most 8");
     if (c.u_fp32_sums > 1) return fail(ctx, PSF_ERR_ARG, "u_fp32_sums must be 0 or 1");
+    if (c.shared_basis > 1) return fail(ctx, PSF_ERR_ARG, "shared_basis must be 0 or 1");
     SRV_HIPCHK(ctx, hipSetDevice(ctx->device));
     psf_server* sv = new (std::nothrow) psf_server();
     if (!sv) return PSF_ERR_NOMEM;
@@ -261,8 +291,13 @@ int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** o
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->ev_upload, hipEventDisableTiming);
     for (int q = 0; q < 2 && e == hipSuccess; ++q) {
-        sv->batch[q] = persistent_batch_create(ctx->device, static_cast<int>(kMaxActive), 8192);
-        if (!sv->batch[q]) e = hipErrorOutOfMemory;
+        if (c.shared_basis) {
+            sv->gbatch[q] = group_batch_create(ctx->device, kMaxGroupInstances);
+            if (!sv->gbatch[q]) e = hipErrorOutOfMemory;
+        } else {
+            sv->batch[q] = persistent_batch_create(ctx->device, static_cast<int>(kMaxActive), 8192);
+            if (!sv->batch[q]) e = hipErrorOutOfMemory;
+        }
         if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sv->d_wtab[q]), sizeof(WindowScene) * kMaxActive);
         if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sv->d_ttab[q]), sizeof(TraceScene) * kMaxActive);
         if (e == hipSuccess)
@@ -280,7 +315,8 @@ int psf_server_create(psf_ctx* ctx, const psf_server_config* cfg, psf_server** o
         psf_server_destroy(sv);
         return fail(ctx, PSF_ERR_HIP, std::string("server set-up: ") + hipGetErrorString(e));
     }
-    sv->wg_capacity = persistent_batch_wg_capacity(sv->batch[0], PSF_C_BF16X2);
+    sv->wg_capacity = c.shared_basis ? group_batch_capacity(sv->gbatch[0])
+                                     : persistent_batch_wg_capacity(sv->batch[0], PSF_C_BF16X2);
     ctx->servers.insert(sv);
     *out = sv;
     return PSF_OK;
@@ -302,11 +338,14 @@ int psf_server_destroy(psf_server* sv)
             scene_release(kv.second->scene);
         }
     for (Basis& b : sv->bases) {
+        group_basis_destroy(b.group);
         if (b.Phi) (void)hipFree(b.Phi);
         if (b.C) (void)hipFree(b.C);
+        if (b.gravity_dev) (void)hipFree(b.gravity_dev);
     }
     for (int q = 0; q < 2; ++q) {
         persistent_batch_destroy(sv->batch[q]);
+        group_batch_destroy(sv->gbatch[q]);
         if (sv->d_wtab[q]) (void)hipFree(sv->d_wtab[q]);
         if (sv->d_ttab[q]) (void)hipFree(sv->d_ttab[q]);
         if (sv->h_wtab[q]) (void)hipHostFree(sv->h_wtab[q]);
@@ -336,6 +375,8 @@ int psf_server_add_basis(psf_server* sv, const psf_scene_desc* desc, const doubl
     int rc = psf_scene_footprint(&d, &probe);   // validates the grid and K0
     if (rc) return fail(ctx, rc, "basis descriptor is invalid");
     if (d.K > 1024) return fail(ctx, PSF_ERR_UNSUPPORTED, "basis K0 must be at most 1024");
+    if (sv->cfg.shared_basis && !group_supported(static_cast<int>(d.K)))
+        return fail(ctx, PSF_ERR_UNSUPPORTED, "shared-basis steps need K0 <= 256");
     SRV_HIPCHK(ctx, hipSetDevice(ctx->device));
     Basis b;
     b.desc = d;
@@ -353,17 +394,29 @@ int psf_server_add_basis(psf_server* sv, const psf_scene_desc* desc, const doubl
         }
     }
     if (e == hipSuccess) e = hipMemcpy(b.C, C, K0 * K0 * K0 * 8, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        if (b.Phi) (void)hipFree(b.Phi);
-        if (b.C) (void)hipFree(b.C);
-        return fail(ctx, e == hipErrorOutOfMemory ? PSF_ERR_NOMEM : PSF_ERR_HIP,
-                    std::string("basis upload: ") + hipGetErrorString(e));
-    }
     b.lambda.assign(lambda, lambda + K0);
     if (gravity)
         b.gravity.assign(gravity, gravity + K0);
     else
         b.gravity.assign(K0, 0.0);
+    if (sv->cfg.shared_basis) {
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b.gravity_dev), K0 * 8);
+        if (e == hipSuccess) e = hipMemcpy(b.gravity_dev, b.gravity.data(), K0 * 8, hipMemcpyHostToDevice);
+    }
+    std::string gerr;
+    if (e == hipSuccess && sv->cfg.shared_basis) {
+        rc = group_basis_create(b.C, static_cast<int>(K0), b.gravity_dev, ctx->stream, &b.group, &gerr);
+        if (rc != PSF_OK && gerr.empty()) gerr = "shared-basis blocks: out of memory";
+    }
+    if (e != hipSuccess || rc != PSF_OK) {
+        if (b.Phi) (void)hipFree(b.Phi);
+        if (b.C) (void)hipFree(b.C);
+        if (b.gravity_dev) (void)hipFree(b.gravity_dev);
+        if (e != hipSuccess)
+            return fail(ctx, e == hipErrorOutOfMemory ? PSF_ERR_NOMEM : PSF_ERR_HIP,
+                        std::string("basis upload: ") + hipGetErrorString(e));
+        return fail(ctx, rc, gerr);
+    }
     sv->bases.push_back(std::move(b));
     *basis_id = static_cast<uint32_t>(sv->bases.size() - 1);
     return PSF_OK;
@@ -391,7 +444,7 @@ int psf_server_submit(psf_server* sv, uint32_t basis_id, uint32_t K, uint32_t nf
     if (nframes == 0) return fail(ctx, PSF_ERR_ARG, "nframes must be positive");
     if (!std::isfinite(dt) || !std::isfinite(viscosity)) return fail(ctx, PSF_ERR_ARG, "non-finite parameter");
     const int ipw = request_ipw(sv, K);
-    const int wgs = persistent_workgroups(PSF_C_BF16X2, static_cast<int>(K), ipw);
+    const int wgs = sv->cfg.shared_basis ? 0 : persistent_workgroups(PSF_C_BF16X2, static_cast<int>(K), ipw);
     const uint64_t fp = request_bytes(sv, b, K, n);
     if (wgs > sv->wg_capacity) return fail(ctx, PSF_ERR_UNSUPPORTED, "request needs more workgroups than one launch holds");
     if (fp > sv->cfg.mem_budget) return fail(ctx, PSF_ERR_NOMEM, "request footprint exceeds the server's memory budget");
@@ -479,7 +532,33 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
             SRV_HIPCHK(ctx, hipStreamWaitEvent(sa, sv->ev_upload, 0));
             sv->upload_pending = false;
         }
-        SRV_HIPCHK(ctx, launch_persistent_batch(sv->batch[q], sc.data(), nsteps.data(), hist.data(), ld.data(), n, sa));
+        if (sv->cfg.shared_basis) {
+            // instances: each basis's scenes in runs of at most kGroupMaxScenes, as even as possible
+            std::vector<std::vector<int>> by_basis(sv->bases.size());
+            for (int j = 0; j < n; ++j) by_basis[run[j]->basis].push_back(j);
+            std::vector<GroupScene> gs(n);
+            std::vector<GroupInstance> inst;
+            int o = 0;
+            for (size_t bi = 0; bi < by_basis.size(); ++bi) {
+                const std::vector<int>& js = by_basis[bi];
+                const int m = static_cast<int>(js.size());
+                if (!m) continue;
+                const int ni = (m + kGroupMaxScenes - 1) / kGroupMaxScenes;
+                for (int c = 0, x = 0; c < ni; ++c) {
+                    const int cnt = m / ni + (c < m % ni ? 1 : 0);
+                    inst.push_back(GroupInstance{sv->bases[bi].group, cnt, gs.data() + o});
+                    for (int y = 0; y < cnt; ++y, ++x, ++o) {
+                        const int j = js[x];
+                        psf_scene* s = sc[j];
+                        gs[o] = GroupScene{s->dev.omega, s->dev.visc, hist[j], s->dev.status, s->dt,
+                                           static_cast<int>(s->d.K), nsteps[j], s->gravity_active, W};
+                    }
+                }
+            }
+            SRV_HIPCHK(ctx, launch_group_steps(sv->gbatch[q], inst.data(), static_cast<int>(inst.size()), 0, sa));
+        } else {
+            SRV_HIPCHK(ctx, launch_persistent_batch(sv->batch[q], sc.data(), nsteps.data(), hist.data(), ld.data(), n, sa));
+        }
         SRV_HIPCHK(ctx, hipEventRecord(sv->ev_step[q], sa));
         SRV_HIPCHK(ctx, hipStreamWaitEvent(sr, sv->ev_step[q], 0));
         SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_wtab[q], sv->h_wtab[q], sizeof(WindowScene) * n, hipMemcpyHostToDevice, sr));
@@ -532,6 +611,12 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
             } else {
                 r->state = kFailed;
                 r->error = ctx->err;
+                if (sv->cfg.shared_basis) {
+                    for (int q2 = 0; q2 < 2; ++q2) {
+                        const std::string d = group_batch_diag(sv->gbatch[q2]);
+                        if (!d.empty()) r->error += " (" + d + ")";
+                    }
+                }
                 sv->st.failed++;
             }
             release(sv, r);

@@ -200,13 +200,15 @@ SERVE_GRIDS = (16, 24, 32, 48)
 SERVE_K0 = 192
 
 
-def test_config4_serving_real_grids(hip_ctx):
+@pytest.mark.parametrize("shared_basis", [False, True])
+def test_config4_serving_real_grids(hip_ctx, shared_basis):
     """Requests on the serving stream's four grids with K from 16 to 192: each
     gives the same bits alone and in the mix; two sampled requests (16^3/K=16
     and 48^3/K=192) agree with the oracle run frame by frame on the first K
-    modes of the basis (BF16 hi+lo C and FP32 Phi against FP64)."""
+    modes of the basis (BF16 hi+lo C and FP32 Phi against FP64).  With
+    shared_basis the scenes of a grid read one copy of its C."""
     from parallelspectralfluids3d_amd import hip
-    _say("config 4: serving on {16,24,32,48}^3, K up to 192")
+    _say(f"config 4: serving on {{16,24,32,48}}^3, K up to 192, shared_basis={shared_basis}")
     bases = {}
     for n in SERVE_GRIDS:
         g = Grid(n, n, n)
@@ -227,7 +229,7 @@ def test_config4_serving_real_grids(hip_ctx):
         p[:, 3] = 40.0
         reqs.append((n, K, frames, w, p))
 
-    sv = hip.Server(hip_ctx, window=32, max_active=64)
+    sv = hip.Server(hip_ctx, window=32, max_active=64, shared_basis=shared_basis)
     bid = {n: sv.add_basis(b["g"], b["h"], b["Phi"], b["lam"], b["gv"], b["C"]) for n, b in bases.items()}
 
     def run(sel):
