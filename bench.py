@@ -91,6 +91,26 @@ def make_hip_scene(ctx, sc, c_prec, phi_prec, with_phi, N):
     return s
 
 
+def oracle_precision_errors(sc, traj):
+    """The FP64 oracle's reduced steps (psfsolver.cpp:34-59) from the bench's
+    initial omega, every 100 steps for 1000 steps, and the relative omega
+    error of each GPU trajectory against it (the checker of the precision
+    section; run in the CPU leg)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    nthreads = min(16, os.cpu_count() or 1)
+    K = sc["lam"].size
+    w = sc["omega"].copy()
+    ora = []
+    for _ in range(10):
+        for _ in range(100):
+            w = O.reduced_step(w, sc["C"], sc["lam"], np.zeros(K), 0.01, 0.0, False, nthreads=nthreads)
+        ora.append(w.copy())
+    return {name: [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(t, ora)]
+            for name, t in traj.items()}
+
+
 def cpu_baseline(sc, budget_s=12.0):
     """The oracle (a C restatement of psfsolver.cpp:21-234, OpenMP as in the
     reference) on this host: whole integrate() frames until budget_s elapse."""
@@ -778,14 +798,21 @@ def main():
                            algorithmic_bytes_per_launch=trace_bytes, traffic=TRACE_TRAFFIC_64_1M if
                            (n, N) == (64, 1 << 20) else None)
     if ms_trf is not None:
-        fb = N * 32 + L.TRACE_FUSE * F * 4       # records once, each frame's u once
+        # the per-unit bytes of a particle-step (32 B record in + out, plus 4 F of
+        # u per frame) times the units of one launch (N particles x TRACE_FUSE
+        # frames); the fused launch itself moves the records once
+        fb = L.TRACE_FUSE * (N * 32 + F * 4)
+        fb_moved = N * 32 + L.TRACE_FUSE * F * 4
         roof["k_trace_frames"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
                                       achieved=fb / (ms_trf * 1e-3) / 1e9, algorithmic_bytes_per_launch=fb,
-                                      frames_per_launch=L.TRACE_FUSE,
+                                      compulsory_bytes_per_launch=fb_moved, frames_per_launch=L.TRACE_FUSE,
                                       traffic=TRACE_FRAMES_TRAFFIC_64_1M if (n, N) == (64, 1 << 20) else None,
-                                      note="latency-bound gathers, not HBM: 12 dependent flux-pair loads per "
-                                           "particle-step, address unit busy 75-77% (TA_BUSY_avr), 69% of "
-                                           "wave-cycles waiting on memory (SQ_WAIT_INST_ANY), L1 hit rate 94%")
+                                      note="algorithmic bytes = particle-steps x 32 B + frames x 4F; the fused "
+                                           "launch keeps records in registers across its frames and moves "
+                                           "compulsory_bytes_per_launch. Bound by issue, not HBM: about 220 "
+                                           "vector instructions per particle-step and 12 dependent flux-pair "
+                                           "gathers; address unit busy 75-77% (TA_BUSY_avr), 69% of wave-cycles "
+                                           "waiting on memory (SQ_WAIT_INST_ANY)")
     rec_bytes = F * K * 4 + F * 4 + K * 8
     # (the calibrated traffic constants were measured for the FP64-sum kernels)
     fp64_sums = PHI_PREC != 2
@@ -846,25 +873,21 @@ def main():
         r.close()
     log("reduced steps/s " + ", ".join(f"{k}={v:.0f}" for k, v in res.items()))
 
-    # ---- omega error against the FP64 path (which matches the oracle to 1e-12) -----------
-    errors = {}
+    # ---- omega trajectories of each precision, every 100 steps (compared with the
+    # FP64 oracle in the CPU leg below, and with the GPU's FP64 path) -----------------------
+    errors, traj = {}, {}
     if not args.no_precision:
-        ref = make_hip_scene(ctx, sc, L.C_FP64, L.PHI_FP32, False, 0)
-        traj = []
-        for _ in range(10):
-            ref.step(100)
-            traj.append(ref.get_omega())
-        ref.close()
-        for name, prec in (("bf16x2", L.C_BF16X2), ("bf16", L.C_BF16), ("fp16", L.C_FP16)):
+        for name, prec in (("fp64", L.C_FP64), ("bf16x2", L.C_BF16X2), ("bf16", L.C_BF16), ("fp16", L.C_FP16)):
             r = make_hip_scene(ctx, sc, prec, L.PHI_FP32, False, 0)
-            e = []
-            for b in range(10):
+            traj[name] = []
+            for _ in range(10):
                 r.step(100)
-                w = r.get_omega()
-                e.append(float(np.linalg.norm(w - traj[b]) / np.linalg.norm(traj[b])))
-            errors[name] = e
+                traj[name].append(r.get_omega())
             r.close()
-        log("omega rel err at 1000 steps: " + ", ".join(f"{k}={v[-1]:.2e}" for k, v in errors.items()))
+        for name in ("bf16x2", "bf16", "fp16"):
+            errors[name] = [float(np.linalg.norm(w - v) / np.linalg.norm(v)) for w, v in zip(traj[name], traj["fp64"])]
+        log("omega rel err at 1000 steps vs the GPU FP64 path: " +
+            ", ".join(f"{k}={v[-1]:.2e}" for k, v in errors.items()))
     ctx.close()
     serving = None
     if not args.no_serving:
@@ -873,9 +896,13 @@ def main():
         serving = serving_bench(sctx, args.serving_requests, args.serving_rate)
         sctx.close()
 
-    cpu, accuracy = None, None
+    cpu, accuracy, ora_err = None, None, None
     if not args.no_cpu_baseline:
         ref_state, cpu = cpu_baseline(sc)
+        if traj:
+            ora_err = oracle_precision_errors(sc, traj)
+            log("omega rel err at 1000 steps vs the FP64 oracle: " +
+                ", ".join(f"{k}={v[-1]:.2e}" for k, v in ora_err.items()))
         log(f"cpu baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
         # the frame loop's arithmetic against the FP64 oracle over the frames the CPU leg ran
         actx = hip.Context(0)
@@ -917,7 +944,8 @@ def main():
         "reduced_steps_per_s": res.get("bf16x2"),
         "reduced_steps_per_s_by_path": res,
         "reduced_speedup_vs_plain_fp32": res["bf16x2"] / res["plain_fp32"],
-        "omega_rel_err_vs_fp64_every_100_steps": errors,
+        "omega_rel_err_vs_fp64_oracle_every_100_steps": ora_err,
+        "omega_rel_err_vs_gpu_fp64_every_100_steps": errors,
         "timed_region": timed_region,
         "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                              "samples": int(lat.size),
