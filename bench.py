@@ -238,6 +238,17 @@ def run_stream(ctx, bases, reqs, max_active, window=32, saturate=False, modes_pe
     lat = np.zeros(len(reqs))
     frames_done = 0
     nxt = 0
+    enqueue_s = None
+    if saturate:
+        # capacity: the clock starts with every request in the queue (each
+        # submit copies the request and sorts its particles on the host)
+        te = time.perf_counter()
+        for j in pending:
+            r = reqs[j]
+            rid = sv.submit(bid[r["grid"]], r["omega"], r["parts"], r["frames"])
+            outstanding[rid] = (j, 0.0)
+        nxt = len(pending)
+        enqueue_s = time.perf_counter() - te
     t0 = time.perf_counter()
     while nxt < len(pending) or outstanding:
         now = time.perf_counter() - t0
@@ -252,19 +263,19 @@ def run_stream(ctx, bases, reqs, max_active, window=32, saturate=False, modes_pe
             if nxt < len(pending):
                 time.sleep(max(0.0, reqs[pending[nxt]]["arrival"] - (time.perf_counter() - t0)))
             continue
-        if sv.round():
-            for rid in [k for k in outstanding if sv.done(k)]:
-                j, t_sub = outstanding.pop(rid)
-                _, _, ms = sv.result(rid)
-                arrival = 0.0 if saturate else reqs[j]["arrival"]
-                lat[j] = t_sub + ms * 1e-3 - arrival
-                frames_done += reqs[j]["frames"]
+        sv.round()
+        for rid in sv.completed():
+            j, t_sub = outstanding.pop(rid)
+            _, _, ms = sv.result(rid)
+            arrival = 0.0 if saturate else reqs[j]["arrival"]
+            lat[j] = t_sub + ms * 1e-3 - arrival
+            frames_done += reqs[j]["frames"]
     makespan = time.perf_counter() - t0
     st = sv.stats()
     sv.close()
     first = 0.0 if saturate else min(r["arrival"] for r in reqs)
     span = makespan - first
-    return dict(makespan_s=makespan, scene_frames=frames_done, scene_frames_per_s=frames_done / span,
+    return dict(makespan_s=makespan, enqueue_s=enqueue_s, scene_frames=frames_done, scene_frames_per_s=frames_done / span,
                 particle_steps_per_s=frames_done * reqs[0]["parts"].shape[0] / span,
                 latency_ms_p50=float(np.percentile(lat, 50) * 1e3), latency_ms_p99=float(np.percentile(lat, 99) * 1e3),
                 rounds=int(st["rounds"]), busy_s=st["busy_ms"] * 1e-3, admit_s=st["admit_ms"] * 1e-3,
@@ -318,7 +329,11 @@ def serving_bench(ctx, n_requests, rate):
         f"p50 {alone['latency_ms_p50']:.1f} ms, p99 {alone['latency_ms_p99']:.1f} ms")
     sat = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED,
                      u_fp32_sums=SERVE_U_FP32_SUMS, shared_basis=SERVE_SHARED_BATCHED)
-    log(f"serving batched, all queued at once: {sat['scene_frames_per_s']:.0f} scene-frames/s")
+    log(f"serving batched, all queued at once: {sat['scene_frames_per_s']:.0f} scene-frames/s "
+        f"(enqueue {sat['enqueue_s']:.3f} s before the clock)")
+    sat_alone = run_stream(ctx, bases, reqs, max_active=1, saturate=True, modes_per_wg=SERVE_MPW_ALONE,
+                           u_fp32_sums=SERVE_U_FP32_SUMS)
+    log(f"serving one at a time, all queued at once: {sat_alone['scene_frames_per_s']:.0f} scene-frames/s")
     sat_own = run_stream(ctx, bases, reqs, max_active=64, saturate=True, modes_per_wg=SERVE_MPW_BATCHED,
                          u_fp32_sums=SERVE_U_FP32_SUMS, shared_basis=False)
     log(f"serving batched with per-request C, all queued at once: {sat_own['scene_frames_per_s']:.0f} scene-frames/s")
@@ -334,10 +349,13 @@ def serving_bench(ctx, n_requests, rate):
         "batched": batched,
         "one_at_a_time": alone,
         "batched_all_queued": sat,
+        "one_at_a_time_all_queued": sat_alone,
         "batched_all_queued_per_request_c": sat_own,
         "shared_basis": {"batched": SERVE_SHARED_BATCHED, "one_at_a_time": False},
         "throughput_vs_one_at_a_time": batched["scene_frames_per_s"] / alone["scene_frames_per_s"],
-        "capacity_vs_one_at_a_time": sat["scene_frames_per_s"] / alone["scene_frames_per_s"],
+        "capacity_vs_one_at_a_time": sat["scene_frames_per_s"] / sat_alone["scene_frames_per_s"],
+        "capacity_note": "both with all 512 requests queued before the clock starts; one at a time is each request "
+                         "on its own C (K modes) and the frame window, the fastest lone-scene form",
         "dropin_frame_by_frame": {"requests": nd, "scene_frames_per_s": dropin,
                                   "note": "the first requests of the stream, each through psf_integrate with no "
                                           "frame window, uploads included"},

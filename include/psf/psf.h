@@ -424,6 +424,10 @@ PSF_API int psf_server_submit(psf_server* server, uint32_t basis_id, uint32_t K,
 PSF_API int psf_server_round(psf_server* server, uint32_t* completed);
 /* 1 if the request finished (its result is ready), 0 if queued or running. */
 PSF_API int psf_server_done(psf_server* server, uint64_t request_id);
+/* The ids of requests that finished (done or failed) since the last call, in
+   the order they finished: up to `capacity` of them into ids, their number
+   into *count; the rest stay for the next call. */
+PSF_API int psf_server_completed(psf_server* server, uint64_t* ids, uint32_t capacity, uint32_t* count);
 /* Result of a finished request: omega (K), particles (n x 4, caller's order),
    latency from submit to the end of its last round in milliseconds.  The
    request is forgotten afterwards. */

@@ -131,6 +131,7 @@ def load():
                                     ct.POINTER(ct.c_uint64)]),
         "psf_server_round": (i32, [vp, ct.POINTER(ct.c_uint32)]),
         "psf_server_done": (i32, [vp, ct.c_uint64]),
+        "psf_server_completed": (i32, [vp, ct.POINTER(ct.c_uint64), ct.c_uint32, ct.POINTER(ct.c_uint32)]),
         "psf_server_result": (i32, [vp, ct.c_uint64, vp, vp, ct.POINTER(ct.c_double)]),
         "psf_server_get_stats": (i32, [vp, ct.POINTER(ServerStats)]),
     }

@@ -642,12 +642,14 @@ int psf::scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOp
     // zero padding of Phi (rows F..ldF-1 of each column are read by the
     // vectorised reconstruction) and the status word
     if (!s->phi_borrowed) chk(hipMemsetAsync(s->dev.Phi, 0, phi_bytes, ctx->stream));
-    chk(launch_zero_int(s->dev.status, 64, ctx->stream));
-    // constants by fill kernels: host-to-device copies from pageable memory
-    // measured 0.5 ms per scene in a long-running process (scene admission)
-    chk(launch_fill_f64(s->dev.visc, static_cast<int>(K), 1.0, ctx->stream));
-    chk(launch_fill_f64(s->dev.gravity, static_cast<int>(K), 0.0, ctx->stream));
-    chk(launch_fill_f64(s->dev.lambda, static_cast<int>(K), 0.0, ctx->stream));
+    if (!opt.no_init) {
+        chk(launch_zero_int(s->dev.status, 64, ctx->stream));
+        // constants by fill kernels: host-to-device copies from pageable memory
+        // measured 0.5 ms per scene in a long-running process (scene admission)
+        chk(launch_fill_f64(s->dev.visc, static_cast<int>(K), 1.0, ctx->stream));
+        chk(launch_fill_f64(s->dev.gravity, static_cast<int>(K), 0.0, ctx->stream));
+        chk(launch_fill_f64(s->dev.lambda, static_cast<int>(K), 0.0, ctx->stream));
+    }
     if (!s->async_uploads) chk(hipStreamSynchronize(ctx->stream));
     if (e != hipSuccess) {
         psf_scene_destroy(s);
@@ -914,6 +916,62 @@ int psf_get_omega(psf_scene* s, double* omega)
 // for the x component the lanes of a wave share the y and z voxel rows (and
 // likewise for y and z up to one row), and each gather touches few cache
 // lines.  Every interface below speaks the caller's order.
+}  // extern "C"
+
+// The cell-sorted storage order of n particles in a scene of descriptor d:
+// sorted[j] = xyzw[pid[j]] (a stable counting sort by the cell of the
+// half-voxel-shifted grid, z-major like the voxels).
+void psf::sort_particles(const psf_scene_desc& d, const float* xyzw, uint32_t n, float* sorted, uint32_t* pid)
+{
+    std::iota(pid, pid + n, 0u);
+    std::vector<uint32_t> key(n);
+    const double res = d.resolution;
+    const size_t V = size_t(d.nx) * d.ny * d.nz;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t c[3];
+        const uint32_t dims[3] = {d.nx, d.ny, d.nz};
+        for (int a = 0; a < 3; ++a) {
+            const float sa = (d.aabb_min[a] + (d.aabb_max[a] - d.aabb_min[a])) + xyzw[4 * size_t(i) + a];
+            const double q = static_cast<double>(sa) / res - 0.5;
+            long v = std::isfinite(q) ? static_cast<long>(std::max(-1.0, std::min(q, 1.0e9))) : 0;
+            c[a] = static_cast<uint32_t>(std::max(0L, std::min(v, static_cast<long>(dims[a]) - 1)));
+        }
+        key[i] = (c[2] * d.ny + c[1]) * d.nx + c[0];
+    }
+    if (V <= std::max<size_t>(size_t(4) * n, size_t(1) << 16)) {
+        std::vector<uint32_t> start(V + 1, 0u);
+        for (uint32_t i = 0; i < n; ++i) start[key[i] + 1]++;
+        for (size_t v = 0; v < V; ++v) start[v + 1] += start[v];
+        for (uint32_t i = 0; i < n; ++i) pid[start[key[i]]++] = i;
+    } else {
+        std::stable_sort(pid, pid + n, [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    }
+    for (uint32_t j = 0; j < n; ++j) std::memcpy(sorted + 4 * size_t(j), xyzw + 4 * size_t(pid[j]), 16);
+}
+
+const std::vector<uint32_t>& psf::scene_pid(const psf_scene* s)
+{
+    return reinterpret_cast<const psf_scene_host*>(s->host)->pid;
+}
+
+void psf::scene_adopt_state(psf_scene* s, std::vector<uint32_t>&& pid, const double* lambda, double dt, double nu,
+                            int gravity_active, float lifetime)
+{
+    psf_scene_host* h = host_of(s);
+    s->n_particles = static_cast<uint32_t>(pid.size());
+    h->pid = std::move(pid);
+    h->lambda.assign(lambda, lambda + s->d.K);
+    s->dt = dt;
+    s->nu = nu;
+    s->gravity_active = gravity_active ? 1 : 0;
+    s->lifetime = lifetime;
+    s->has_basis = true;
+    s->has_omega = true;
+    density_particles_changed(s);
+}
+
+extern "C" {
+
 int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
 {
     if (!s || (!xyzw && n)) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
@@ -922,31 +980,8 @@ int psf_set_particles(psf_scene* s, const float* xyzw, uint32_t n)
     if (int rc = flush_traces(s)) return rc;   // deferred passes belong to the particles being replaced
     std::vector<uint32_t>& pid = host_of(s)->pid;
     pid.resize(n);
-    std::iota(pid.begin(), pid.end(), 0u);
-    std::vector<uint32_t> key(n);
-    const double res = s->d.resolution;
-    for (uint32_t i = 0; i < n; ++i) {
-        uint32_t c[3];
-        const uint32_t dims[3] = {s->d.nx, s->d.ny, s->d.nz};
-        for (int a = 0; a < 3; ++a) {
-            const float sa = (s->d.aabb_min[a] + (s->d.aabb_max[a] - s->d.aabb_min[a])) + xyzw[4 * size_t(i) + a];
-            const double q = static_cast<double>(sa) / res - 0.5;
-            long v = std::isfinite(q) ? static_cast<long>(std::max(-1.0, std::min(q, 1.0e9))) : 0;
-            c[a] = static_cast<uint32_t>(std::max(0L, std::min(v, static_cast<long>(dims[a]) - 1)));
-        }
-        key[i] = (c[2] * s->d.ny + c[1]) * s->d.nx + c[0];
-    }
-    if (size_t(s->V) <= std::max<size_t>(size_t(4) * n, size_t(1) << 16)) {
-        // stable counting sort by cell
-        std::vector<uint32_t> start(size_t(s->V) + 1, 0u);
-        for (uint32_t i = 0; i < n; ++i) start[key[i] + 1]++;
-        for (size_t v = 0; v < s->V; ++v) start[v + 1] += start[v];
-        for (uint32_t i = 0; i < n; ++i) pid[start[key[i]]++] = i;
-    } else {
-        std::stable_sort(pid.begin(), pid.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
-    }
     std::vector<float> sorted(size_t(n) * 4);
-    for (uint32_t j = 0; j < n; ++j) std::memcpy(&sorted[4 * size_t(j)], xyzw + 4 * size_t(pid[j]), 16);
+    sort_particles(s->d, xyzw, n, sorted.data(), pid.data());
     if (n) {
         // after the passes just launched (and any earlier ones) on the context stream
         if (int rc = upload_sync(s, s->dev.parts, sorted.data(), size_t(n) * 16)) return rc;

@@ -2,6 +2,7 @@
 // the scene server (psf_server.hip).
 #pragma once
 #include <string>
+#include <vector>
 
 #include "psf_common.hpp"
 #include "psf_kernels.hpp"
@@ -16,9 +17,19 @@ struct SceneOptions {
     bool frame_pipeline = true;    // frame windows keep a second buffer pair and a side stream
     bool async_uploads = false;    // set-up calls do not wait for their copies (host sources must outlive them)
     bool no_c = false;             // no structure tensor of its own: its steps run from a shared basis's C
+    bool no_init = false;          // no fill kernels: the caller writes omega, lambda, gravity, visc and status
 };
 
 int scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOptions& opt, psf_scene** out);
+// the cell-sorted storage order of n particles (psf_set_particles): sorted[j] = xyzw[pid[j]]
+void sort_particles(const psf_scene_desc& d, const float* xyzw, uint32_t n, float* sorted, uint32_t* pid);
+// Host-side state of a scene whose device arrays the caller fills with its
+// own copies: particles in storage order with caller indices pid, lambda,
+// the step parameters, omega present.  Nothing is uploaded.
+// caller index of each stored particle of a scene
+const std::vector<uint32_t>& scene_pid(const psf_scene* s);
+void scene_adopt_state(psf_scene* s, std::vector<uint32_t>&& pid, const double* lambda, double dt, double nu,
+                       int gravity_active, float lifetime);
 // destroy a scene whose device work is complete (no wait on the context stream)
 void scene_release(psf_scene* s);
 // omega and particles (caller's order) by synchronous null-stream copies

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <new>
 #include <string>
@@ -69,7 +70,11 @@ struct Request {
     int grav = 0;
     float lifetime = 0.0f;
     std::vector<double> omega;   // input, then result
-    std::vector<float> parts;    // input, then result
+    std::vector<float> parts;    // result (caller's order)
+    std::vector<float> sorted;   // input particles in storage order (sorted at submit)
+    std::vector<uint32_t> pid;   // caller index of each stored particle
+    void* pin = nullptr;         // pinned staging: the admission's uploads, then the results read back
+    size_t pin_class = 0;
     psf_scene* scene = nullptr;
     uint64_t footprint = 0;
     int wgs = 0, ipw = 0;
@@ -80,13 +85,51 @@ struct Request {
 
 }  // namespace
 
+// Recycled pinned host buffers in power-of-two size classes (admission
+// staging): hipHostMalloc is slow, and host-to-device copies from pageable
+// memory can block the host.
+struct PinnedPool {
+    std::multimap<size_t, void*> idle;
+    size_t held = 0;
+    ~PinnedPool()
+    {
+        for (auto& kv : idle) (void)hipHostFree(kv.second);
+    }
+    static size_t cls(size_t bytes)
+    {
+        size_t c = 4096;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    void* get(size_t bytes, size_t& c)
+    {
+        c = cls(bytes);
+        auto it = idle.find(c);
+        if (it != idle.end()) {
+            void* p = it->second;
+            idle.erase(it);
+            return p;
+        }
+        void* p = nullptr;
+        if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess) return nullptr;
+        held += c;
+        return p;
+    }
+    void put(void* p, size_t c)
+    {
+        if (p) idle.emplace(c, p);
+    }
+};
+
 struct psf_server {
     psf_ctx* ctx = nullptr;
+    PinnedPool pinned;
     psf_server_config cfg{};
     std::vector<Basis> bases;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     std::deque<Request*> queue;
     std::vector<Request*> active;
+    std::deque<uint64_t> finished;   // ids of requests done or failed, in that order (psf_server_completed)
     uint64_t next_id = 1;
     // per round parity: launch tables (pinned staging and device copies)
     PersistentBatch* batch[2] = {nullptr, nullptr};
@@ -164,22 +207,66 @@ int admit(psf_server* sv, Request* r)
     o.cells = false;
     o.pooled = true;
     o.frame_pipeline = false;   // the server pipelines rounds itself
-    o.async_uploads = true;     // sources below live in the request and the basis; P waits on ev_upload
+    o.async_uploads = true;     // P waits for the admission's copies on ev_upload
     o.no_c = sv->cfg.shared_basis != 0;   // steps read the basis's C
+    o.no_init = true;           // every constant comes with the copies below
     psf_scene* s = nullptr;
     int rc = scene_create_ex(sv->ctx, &d, o, &s);
     if (rc) return rc;
     r->scene = s;
-    // particles first: host sorting and null-stream copies overlap the GPU's work
-    if ((rc = psf_set_particles(s, r->parts.data(), r->n))) return rc;
     if ((rc = psf_set_frame_window(s, sv->cfg.window))) return rc;
     SRV_HIPCHK(sv->ctx, scene_alloc(s, reinterpret_cast<void**>(&r->hist2), sizeof(double) * r->K * sv->cfg.window));
     SRV_HIPCHK(sv->ctx, scene_alloc(s, reinterpret_cast<void**>(&r->u_hist2), sizeof(float) * s->dev.ldF * sv->cfg.window));
-    if ((rc = psf_scene_upload_basis(s, nullptr, b.lambda.data(), b.gravity.data()))) return rc;
-    if (!sv->cfg.shared_basis) SRV_HIPCHK(sv->ctx, launch_pack_c(s, b.C, static_cast<int>(b.desc.K), sv->ctx->stream));
+    // Every upload from one pinned staging buffer, in copies that never wait
+    // on the host: the particles in storage order and their caller indices,
+    // omega, lambda, gravity, viscosity factors exp(nu lambda_k dt)
+    // (psfsolver.cpp:51-55) and a cleared status word.
+    const size_t K = r->K, n = r->n;
+    const size_t o_pid = n * 16, o_w = o_pid + ((n * 4 + 15) & ~size_t(15)), o_l = o_w + K * 8, o_g = o_l + K * 8,
+                 o_v = o_g + K * 8, o_st = o_v + K * 8, total = o_st + 256;
+    r->pin = sv->pinned.get(total, r->pin_class);
+    if (!r->pin) return fail(sv->ctx, PSF_ERR_NOMEM, "pinned staging of an admission");
+    unsigned char* pin = static_cast<unsigned char*>(r->pin);
+    std::memcpy(pin, r->sorted.data(), n * 16);
+    std::memcpy(pin + o_pid, r->pid.data(), n * 4);
+    std::memcpy(pin + o_w, r->omega.data(), K * 8);
+    std::memcpy(pin + o_l, b.lambda.data(), K * 8);
+    std::memcpy(pin + o_g, b.gravity.data(), K * 8);
+    double* visc = reinterpret_cast<double*>(pin + o_v);
+    for (size_t k = 0; k < K; ++k) visc[k] = std::exp(r->nu * b.lambda[k] * r->dt);
+    std::memset(pin + o_st, 0, 256);
+    hipStream_t up = sv->ctx->stream;   // the upload stream during admission
+    auto h2d = [&](void* dst, size_t off, size_t bytes) {
+        return bytes ? hipMemcpyAsync(dst, pin + off, bytes, hipMemcpyHostToDevice, up) : hipSuccess;
+    };
+    SRV_HIPCHK(sv->ctx, h2d(s->dev.parts, 0, n * 16));
+    SRV_HIPCHK(sv->ctx, h2d(s->dev.pid, o_pid, n * 4));
+    SRV_HIPCHK(sv->ctx, h2d(s->dev.omega, o_w, K * 8));
+    SRV_HIPCHK(sv->ctx, h2d(s->dev.lambda, o_l, K * 8));
+    SRV_HIPCHK(sv->ctx, h2d(s->dev.gravity, o_g, K * 8));
+    SRV_HIPCHK(sv->ctx, h2d(s->dev.visc, o_v, K * 8));
+    SRV_HIPCHK(sv->ctx, h2d(s->dev.status, o_st, 256));
+    if (!sv->cfg.shared_basis) SRV_HIPCHK(sv->ctx, launch_pack_c(s, b.C, static_cast<int>(b.desc.K), up));
     s->has_C = true;
-    if ((rc = psf_set_params(s, r->dt, r->nu, r->grav, r->lifetime))) return rc;
-    if ((rc = psf_set_omega(s, r->omega.data()))) return rc;
+    scene_adopt_state(s, std::move(r->pid), b.lambda.data(), r->dt, r->nu, r->grav, r->lifetime);
+    std::vector<float>().swap(r->sorted);
+    return PSF_OK;
+}
+
+// omega, particles (caller's order) and status of a request whose last round
+// has completed, from the copies its last round queued into its pinned buffer
+int read_results(psf_server* sv, Request* r)
+{
+    const unsigned char* pin = static_cast<const unsigned char*>(r->pin);
+    const size_t np = size_t(r->n) * 16, nw = size_t(r->K) * 8;
+    int st = 0;
+    std::memcpy(&st, pin + np + nw, sizeof(int));
+    if (st == 1) return fail(sv->ctx, PSF_ERR_NONFINITE, "energy renormalisation saw a zero or non-finite energy");
+    if (st) return fail(sv->ctx, PSF_ERR_HIP, "persistent step: the exchange of y between workgroups timed out");
+    std::memcpy(r->omega.data(), pin + np, nw);
+    const std::vector<uint32_t>& pid = scene_pid(r->scene);
+    const float* sorted = reinterpret_cast<const float*>(pin);
+    for (uint32_t j = 0; j < r->n; ++j) std::memcpy(r->parts.data() + 4 * size_t(pid[j]), sorted + 4 * size_t(j), 16);
     return PSF_OK;
 }
 
@@ -194,6 +281,8 @@ void release(psf_server* sv, Request* r)
     r->scene = nullptr;
     r->hist2 = nullptr;
     r->u_hist2 = nullptr;
+    sv->pinned.put(r->pin, r->pin_class);   // its copies completed with the request's rounds
+    r->pin = nullptr;
     if (r->last_round < 0) sv->wgs_in_use -= r->wgs;   // otherwise freed when its last round launched
     sv->mem_in_use -= r->footprint;
 }
@@ -252,6 +341,7 @@ void admit_queued(psf_server* sv, bool device_idle)
             (void)hipStreamSynchronize(sv->upload);   // uploads already queued for the scene
             r->state = kFailed;
             r->t_done = Clock::now();
+            sv->finished.push_back(r->id);
             release(sv, r);
             sv->st.failed++;
             continue;
@@ -331,12 +421,14 @@ int psf_server_destroy(psf_server* sv)
     if (sv->second) (void)hipStreamSynchronize(sv->second);
     if (sv->third) (void)hipStreamSynchronize(sv->third);
     if (sv->upload) (void)hipStreamSynchronize(sv->upload);
-    for (auto& kv : sv->reqs)
+    for (auto& kv : sv->reqs) {
         if (kv.second->scene) {
             scene_free(kv.second->scene, kv.second->hist2);
             scene_free(kv.second->scene, kv.second->u_hist2);
             scene_release(kv.second->scene);
         }
+        sv->pinned.put(kv.second->pin, kv.second->pin_class);   // freed with the pool
+    }
     for (Basis& b : sv->bases) {
         group_basis_destroy(b.group);
         if (b.Phi) (void)hipFree(b.Phi);
@@ -459,7 +551,12 @@ int psf_server_submit(psf_server* sv, uint32_t basis_id, uint32_t K, uint32_t nf
     r->grav = gravity_active ? 1 : 0;
     r->lifetime = lifetime;
     r->omega.assign(omega, omega + K);
-    r->parts.assign(particles, particles + size_t(n) * 4);
+    r->parts.assign(size_t(n) * 4, 0.0f);
+    // storage order now, on the submitting thread: admission only copies
+    r->sorted.resize(size_t(n) * 4);
+    r->pid.resize(n);
+    psf_scene_desc sd = b.desc;
+    sort_particles(sd, particles, n, r->sorted.data(), r->pid.data());
     r->footprint = fp;
     r->wgs = wgs;
     r->ipw = ipw;
@@ -568,7 +665,6 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
         SRV_HIPCHK(ctx, hipStreamWaitEvent(sb, sv->ev_rec[q], 0));
         SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_ttab[q], sv->h_ttab[q], sizeof(TraceScene) * n, hipMemcpyHostToDevice, sb));
         SRV_HIPCHK(ctx, launch_trace_window_multi(sv->d_ttab[q], n, tblocks, sb));
-        SRV_HIPCHK(ctx, hipEventRecord(sv->ev_done[q], sb));
         for (int j = 0; j < n; ++j) {
             Request* r = run[j];
             r->scene->frame += static_cast<uint32_t>(nsteps[j]);
@@ -577,8 +673,18 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
             if (r->sched == r->nframes) {
                 r->last_round = sv->launched;
                 sv->wgs_in_use -= r->wgs;
+                // its results into its pinned buffer behind this round's particle
+                // launch (omega and status are final since the round's steps):
+                // retiring it then waits for nothing on the device
+                psf_scene* s = r->scene;
+                unsigned char* pin = static_cast<unsigned char*>(r->pin);
+                const size_t np = size_t(r->n) * 16, nw = size_t(r->K) * 8;
+                if (np) SRV_HIPCHK(ctx, hipMemcpyAsync(pin, s->dev.parts, np, hipMemcpyDeviceToHost, sb));
+                SRV_HIPCHK(ctx, hipMemcpyAsync(pin + np, s->dev.omega, nw, hipMemcpyDeviceToHost, sb));
+                SRV_HIPCHK(ctx, hipMemcpyAsync(pin + np + nw, s->dev.status, sizeof(int), hipMemcpyDeviceToHost, sb));
             }
         }
+        SRV_HIPCHK(ctx, hipEventRecord(sv->ev_done[q], sb));   // after the particles and the copies
         sv->launched++;
         sv->st.rounds++;
     }
@@ -602,8 +708,9 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
                 keep.push_back(r);
                 continue;
             }
-            const int rc = scene_read_results(r->scene, r->omega.data(), r->parts.data());
+            const int rc = read_results(sv, r);
             r->t_done = Clock::now();
+            sv->finished.push_back(r->id);
             if (rc == PSF_OK) {
                 r->state = kDone;
                 sv->st.completed++;
@@ -653,6 +760,18 @@ int psf_server_result(psf_server* sv, uint64_t id, double* omega, float* particl
     if (particles) std::memcpy(particles, r->parts.data(), r->parts.size() * 4);
     if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
     sv->reqs.erase(it);
+    return PSF_OK;
+}
+
+int psf_server_completed(psf_server* sv, uint64_t* ids, uint32_t capacity, uint32_t* count)
+{
+    if (!sv || !count || (!ids && capacity)) return PSF_ERR_ARG;
+    uint32_t c = 0;
+    while (c < capacity && !sv->finished.empty()) {
+        ids[c++] = sv->finished.front();
+        sv->finished.pop_front();
+    }
+    *count = c;
     return PSF_OK;
 }
 

@@ -442,6 +442,18 @@ class Server:
             self.ctx.check(rc, "psf_server_done")
         return rc == 1
 
+    def completed(self, capacity=1024) -> list:
+        """Ids of the requests that finished since the last call (done or
+        failed), in the order they finished."""
+        ids = (ct.c_uint64 * capacity)()
+        n = ct.c_uint32()
+        out = []
+        while True:
+            self.ctx.check(self.lib.psf_server_completed(self.h, ids, capacity, ct.byref(n)), "psf_server_completed")
+            out.extend(int(ids[j]) for j in range(n.value))
+            if n.value < capacity:
+                return out
+
     def result(self, rid):
         """(omega, particles, latency_ms) of a finished request."""
         w = np.empty(self._K[rid], np.float64)

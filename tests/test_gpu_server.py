@@ -43,10 +43,14 @@ def _requests(sd, seed=0):
 def _run(sv, bid, reqs, dt=0.01):
     ids = [sv.submit(bid, w, p, frames, dt=dt, lifetime=30.0) for (K, frames, w, p) in reqs]
     guard = 0
+    finished = []
     while not all(sv.done(i) for i in ids):
         sv.round()
+        finished += sv.completed()
         guard += 1
         assert guard < 10000
+    finished += sv.completed()
+    assert sorted(finished) == sorted(ids)      # psf_server_completed names each exactly once
     return [sv.result(i) for i in ids]
 
 
