@@ -393,6 +393,8 @@ typedef struct psf_server_config {
                                on the FP32 matrix cores (PSF_PHI_FP32_ACC32) */
     uint32_t shared_basis;  /* 0: each request's reduced steps read its own packed C (modes_per_wg applies);
                                1: the scenes of a basis read one shared copy of its C (K0 <= 256) */
+    uint32_t wait_limit_ns; /* bound of each wait of the reduced steps' exchange between workgroups, in ns
+                               (0 = 2 s); a request whose launch gave up fails with PSF_ERR_HIP */
 } psf_server_config;
 
 typedef struct psf_server_stats {

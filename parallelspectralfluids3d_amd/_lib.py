@@ -42,7 +42,8 @@ class SceneDesc(ct.Structure):
 
 class ServerConfig(ct.Structure):
     _fields_ = [("window", ct.c_uint32), ("max_active", ct.c_uint32), ("mem_budget", ct.c_uint64),
-                ("modes_per_wg", ct.c_uint32), ("u_fp32_sums", ct.c_uint32), ("shared_basis", ct.c_uint32)]
+                ("modes_per_wg", ct.c_uint32), ("u_fp32_sums", ct.c_uint32), ("shared_basis", ct.c_uint32),
+                ("wait_limit_ns", ct.c_uint32)]
 
 
 class SceneCounters(ct.Structure):

@@ -130,6 +130,7 @@ struct psf_server {
     std::deque<Request*> queue;
     std::vector<Request*> active;
     std::deque<uint64_t> finished;   // ids of requests done or failed, in that order (psf_server_completed)
+    std::string last_diag;           // the last exchange wait that gave up (basis-shared steps)
     uint64_t next_id = 1;
     // per round parity: launch tables (pinned staging and device copies)
     PersistentBatch* batch[2] = {nullptr, nullptr};
@@ -162,6 +163,13 @@ int fail(psf_ctx* ctx, int code, const std::string& msg) { return ctx_set_error(
         hipError_t e_ = (expr);                                                                              \
         if (e_ != hipSuccess) return fail((ctx), PSF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// bound of each exchange wait in s_memrealtime ticks (100 MHz; 0: the kernels' default)
+uint64_t wait_ticks(const psf_server* sv)
+{
+    const uint64_t ns = sv->cfg.wait_limit_ns;
+    return ns ? std::max<uint64_t>(1, ns / 10) : 0;
+}
 
 // modes per workgroup of a request's persistent layout
 int request_ipw(const psf_server* sv, uint32_t K)
@@ -246,7 +254,10 @@ int admit(psf_server* sv, Request* r)
     SRV_HIPCHK(sv->ctx, h2d(s->dev.gravity, o_g, K * 8));
     SRV_HIPCHK(sv->ctx, h2d(s->dev.visc, o_v, K * 8));
     SRV_HIPCHK(sv->ctx, h2d(s->dev.status, o_st, 256));
-    if (!sv->cfg.shared_basis) SRV_HIPCHK(sv->ctx, launch_pack_c(s, b.C, static_cast<int>(b.desc.K), up));
+    if (!sv->cfg.shared_basis) {
+        SRV_HIPCHK(sv->ctx, launch_pack_c(s, b.C, static_cast<int>(b.desc.K), up));
+        if (sv->cfg.wait_limit_ns) (void)persistent_set_wait_ticks(s, wait_ticks(sv));
+    }
     s->has_C = true;
     scene_adopt_state(s, std::move(r->pid), b.lambda.data(), r->dt, r->nu, r->grav, r->lifetime);
     std::vector<float>().swap(r->sorted);
@@ -652,7 +663,8 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
                     }
                 }
             }
-            SRV_HIPCHK(ctx, launch_group_steps(sv->gbatch[q], inst.data(), static_cast<int>(inst.size()), 0, sa));
+            SRV_HIPCHK(ctx, launch_group_steps(sv->gbatch[q], inst.data(), static_cast<int>(inst.size()), wait_ticks(sv),
+                                               sa));
         } else {
             SRV_HIPCHK(ctx, launch_persistent_batch(sv->batch[q], sc.data(), nsteps.data(), hist.data(), ld.data(), n, sa));
         }
@@ -719,10 +731,12 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
                 r->state = kFailed;
                 r->error = ctx->err;
                 if (sv->cfg.shared_basis) {
+                    // what the launch's first wait that gave up waited for
                     for (int q2 = 0; q2 < 2; ++q2) {
                         const std::string d = group_batch_diag(sv->gbatch[q2]);
-                        if (!d.empty()) r->error += " (" + d + ")";
+                        if (!d.empty()) sv->last_diag = d;
                     }
+                    if (!sv->last_diag.empty()) r->error += " (" + sv->last_diag + ")";
                 }
                 sv->st.failed++;
             }

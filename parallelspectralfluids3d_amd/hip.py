@@ -372,13 +372,14 @@ class Server:
     request."""
 
     def __init__(self, ctx: Context, window=32, max_active=32, mem_budget=0, modes_per_wg=0, u_fp32_sums=False,
-                 shared_basis=False):
+                 shared_basis=False, wait_limit_ns=0):
         self.ctx = ctx
         self.lib = ctx.lib
         cfg = L.ServerConfig()
         cfg.window, cfg.max_active, cfg.mem_budget, cfg.modes_per_wg = window, max_active, mem_budget, modes_per_wg
         cfg.u_fp32_sums = 1 if u_fp32_sums else 0
         cfg.shared_basis = 1 if shared_basis else 0
+        cfg.wait_limit_ns = int(wait_limit_ns)
         h = ct.c_void_p()
         ctx.check(self.lib.psf_server_create(ctx.h, ct.byref(cfg), ct.byref(h)), "psf_server_create")
         self.h = h

@@ -218,3 +218,40 @@ def test_shared_basis_rejects_large_basis(hip_ctx):
         sv.add_basis(sd.grid, sd.resolution, Phi, np.ones(K), np.zeros(K), np.zeros(K ** 3))
     assert e.value.status == L.PSF_ERR_UNSUPPORTED
     sv.close()
+
+
+@pytest.mark.parametrize("shared_basis", [False, True])
+def test_exchange_wait_limit(hip_ctx, shared_basis):
+    """A wait limit of 10 ns (one tick of the wait clock) makes the reduced
+    steps' exchange give up instead of waiting: the launch ends, and each
+    request fails with the timeout (with the basis-shared step, the error
+    names the wait that gave up: instance, workgroup, step, mode and scene);
+    a server with the default limit then runs the same requests."""
+    from parallelspectralfluids3d_amd import _lib as L
+    _say(f"server exchange wait limit, shared_basis={shared_basis}")
+    sd = make_scene(n=8, K=96)          # 12 workgroups per scene or instance
+    rng = np.random.default_rng(11)
+    reqs = []
+    for K in (96, 80):
+        w = rng.standard_normal(K)
+        w /= np.linalg.norm(w)
+        reqs.append((K, 64, w, particles_uniform(sd, 128, seed=5, lifetime=25.0)))
+    sv, bid = _server(hip_ctx, sd, window=32, shared_basis=shared_basis, wait_limit_ns=10)
+    ids = [sv.submit(bid, w, p, frames, lifetime=30.0) for (K, frames, w, p) in reqs]
+    guard = 0
+    while not all(sv.done(i) for i in ids):
+        sv.round()
+        guard += 1
+        assert guard < 1000
+    assert sv.stats()["failed"] == len(ids)
+    for i in ids:
+        with pytest.raises(L.PsfError) as e:
+            sv.result(i)
+        assert e.value.status == L.PSF_ERR_HIP and "timed out" in str(e.value)
+        if shared_basis:
+            assert "workgroup" in str(e.value) and "scene" in str(e.value), str(e.value)
+    sv.close()
+    sv, bid = _server(hip_ctx, sd, window=32, shared_basis=shared_basis)
+    res = _run(sv, bid, reqs)
+    assert all(np.all(np.isfinite(r[0])) for r in res)
+    sv.close()
