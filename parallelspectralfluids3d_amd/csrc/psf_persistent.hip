@@ -11,25 +11,32 @@
 // stored pre-swizzled in the per-lane register layout (16 bytes per lane, a
 // coalesced 1 KiB load per wave) and are loaded into VGPRs once per launch.
 //
-// w enters as two B columns: column 0 holds w_hi, column 1 w_lo (w - w_hi,
+// w enters as a pair of B columns: one holds w_hi, the next w_lo (w - w_hi,
 // scaled for FP16), so one MFMA gives T (w_hi + w_lo) with FP32 accumulation.
-// The row contraction with w_p and all reductions over rows, blocks, waves
-// and workgroups are FP64.
+// A wave's resident blocks cover a few row groups (runs of blocks with the
+// same mode and row tile); row group s of the wave takes column pair
+// (2s, 2s + 1) and every other column of its B operand is zero, so the whole
+// chain accumulates without a break and the row groups are separated only
+// once, after the last MFMA.  The row contraction with w_p and all
+// reductions over rows, blocks, waves and workgroups are FP64.
 //
 // Exchange.  Workgroup g owns modes i = g, g+G, ...  After the contraction it
-// publishes y_i as two 64-bit words (step tag << 32 | high or low half of the
-// FP64 bits) with relaxed agent-scope atomic stores, and one wave of every
-// workgroup polls all K pairs until both tags match.  Each mode's record sits
-// on its own 128-byte line: records that share a line written from several
-// CUs cost more than reading one line per mode.  Each word carries its own
-// validity, so no release/acquire fences (L2 write-back/invalidate) are
-// needed, and the y values arrive exactly.  Buffers alternate by tag parity:
-// a workgroup can only overwrite a buffer two steps later, after every
-// workgroup has consumed it.  Every workgroup then applies the same FP64
-// update (energy renormalisation, viscosity, forcing) to its own copy of w,
-// in a fixed order, so all copies stay bitwise identical.  Waits are bounded
-// (2 s of device time); on expiry every workgroup leaves the loop and the
-// status word reports it.
+// publishes y_i as a 16-byte record of two 64-bit words (step tag << 32 |
+// high or low half of the FP64 bits), stored and polled with single 16-byte
+// buffer accesses at agent coherence (sc1), and one wave of every workgroup
+// polls all K records until both tags match.  Each word carries its own tag
+// in the same aligned 8-byte unit, so a record seen half old and half new
+// fails the check and is read again.  Each mode's record sits on its own
+// 128-byte line: records that share a line written from several CUs cost
+// more than reading one line per mode.  No release/acquire fences (L2
+// write-back/invalidate) are needed, and the y values arrive exactly.
+// Buffers alternate by tag parity: a workgroup can only overwrite a buffer
+// two steps later, after every workgroup has consumed it.  Every workgroup
+// then applies the same FP64 update (energy renormalisation, viscosity,
+// forcing) to its own copy of w, in a fixed order, so all copies stay
+// bitwise identical; the energy of w before the step is summed by a second
+// wave while the first one polls.  Waits are bounded (2 s of device time);
+// on expiry every workgroup leaves the loop and the status word reports it.
 //
 // Residency.  The exchange needs all G workgroups resident at once.  The
 // launch is an ordinary one on the scene's stream: kernels on one stream run
@@ -52,6 +59,7 @@ namespace psf {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 16;
 constexpr int kRB = 36;          // largest register budget, in MFMA blocks per wave (K = 256 hi+lo needs 34)
@@ -108,7 +116,7 @@ static inline PersistentState* st(psf_scene* s) { return reinterpret_cast<Persis
 // nb - p tile blocks (about half as many for two-tile blocks), so a long row
 // and a short row alternate; the contiguous block ranges of the waves then
 // hold nearly equal numbers of row groups (two each at K = 256 with eight
-// waves), and each group costs a flush of its accumulator.
+// waves), and each group takes a column pair of the wave's accumulator.
 static inline int row_at(int idx, int nb) { return (idx & 1) ? nb - 1 - (idx >> 1) : (idx >> 1); }
 
 int blocks_per_mode(int prec, int nb)
@@ -173,6 +181,21 @@ __device__ __forceinline__ f32x4 mfma16(const uint4& a, const uint4& b, const f3
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
 }
 
+// The value of the lane 2 S columns to the right in the same row of 16
+// (DPP row_shl); S is a compile-time constant after unrolling.
+__device__ __forceinline__ double from_pair(double v, int S)
+{
+    switch (S) {
+        case 1: return dpp_move<0x102, 0xF>(v);
+        case 2: return dpp_move<0x104, 0xF>(v);
+        case 3: return dpp_move<0x106, 0xF>(v);
+        case 4: return dpp_move<0x108, 0xF>(v);
+        case 5: return dpp_move<0x10A, 0xF>(v);
+        case 6: return dpp_move<0x10C, 0xF>(v);
+        default: return dpp_move<0x10E, 0xF>(v);
+    }
+}
+
 // The body of one workgroup (NW waves): local workgroup g of the scene
 // described by a.
 template <int ELEM, int RB, int NW>
@@ -185,7 +208,7 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
     __shared__ uint32_t s_meta[kMaxMeta];
     __shared__ double s_visc[kMaxK];
     __shared__ double s_grav[kMaxK];
-    __shared__ double s_red[2 * NW];
+    __shared__ double s_red[NW + 1];   // per-wave sums of the new energy, then the old energy
     __shared__ double s_ypart[NW][kMaxIpw];
     __shared__ int s_flag;
 
@@ -219,7 +242,7 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
     }
     __syncthreads();
     // metadata of the resident blocks, one word per lane, read back with readlane;
-    // fmask bit j: block j ends a row group (flush), tmask bit j: it ends a mode
+    // fmask bit j: block j ends a row group, tmask bit j: it ends a mode
     const uint32_t mreg = lane < nres ? s_meta[b0 + lane] : 0u;
     uint64_t fmask = 0, tmask = 0;
     for (int j = 0; j < nres; ++j) {
@@ -228,24 +251,57 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
         if (kn != kj) fmask |= 1ull << j;
         if (kn < 0 || (kn >> 8) != (kj >> 8)) tmask |= 1ull << j;
     }
-    // B operand: lanes of column 0 read w_hi, column 1 w_lo, the others a
-    // zeroed region, so every lane issues the same unpredicated load
-    const int bsel = (col == 0 ? 0 : (col == 1 ? Kp : 2 * Kp)) + (kg & 1) * 8;
+    // Column pairs: with at most 8 row groups and every block resident, row
+    // group s accumulates into columns (2s, 2s + 1) over the whole chain.
+    // Otherwise the chain flushes columns 0 and 1 at the end of each group;
+    // so do the chains of the largest register budget, where the pair form's
+    // addresses would spill.
+    const int nseg = __builtin_popcountll(fmask);
+    const bool pairs = RB <= 18 && nseg <= 8 && nblk <= RB;
+    const int lane_seg = col >> 1;
+    const int bsel_on = ((col & 1) ? Kp : 0) + (kg & 1) * 8;   // this lane's half tile of w_hi or w_lo
+    const int bsel_off = 2 * Kp + (kg & 1) * 8;                // the zeroed region
+    // per lane: row tile of its column pair's group; per wave: the mode of
+    // each group (4 bits each) and which groups end a mode
+    int lane_p = 0;
+    uint32_t seg_t = 0, seg_end = 0;
+    {
+        int k = 0, jend = 0;
+        for (int j = 0; j < nres; ++j)
+            if ((fmask >> j) & 1ull) {
+                if (k == lane_seg) jend = j;
+                const uint32_t mj = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mreg), j));
+                if (k < 8) {
+                    seg_t |= ((mj >> 24) & 15u) << (4 * k);
+                    if ((tmask >> j) & 1ull) seg_end |= 1u << k;
+                }
+                ++k;
+            }
+        lane_p = static_cast<int>((static_cast<uint32_t>(__shfl(static_cast<int>(mreg), jend)) >> 16) & 255u);
+    }
+    // B operand of block j (metadata m): in pair form the lanes of the block's
+    // column pair read w_hi / w_lo and the others the zeroed region, so every
+    // lane issues the same unpredicated load
+    auto b_pair = [&](int j) {
+        const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mreg), j));
+        const int sj = __builtin_popcountll(fmask & ((1ull << j) - 1ull));
+        const int q = (kg < 2) ? static_cast<int>((m >> 8) & 255u) : static_cast<int>(m & 255u);
+        return *reinterpret_cast<const uint4*>(s_wb + (lane_seg == sj ? bsel_on : bsel_off) + q * kTile);
+    };
+    const int bsel = col < 2 ? bsel_on : bsel_off;
     auto b_operand = [&](uint32_t m) {
         const int q = (kg < 2) ? static_cast<int>((m >> 8) & 255u) : static_cast<int>(m & 255u);
         return *reinterpret_cast<const uint4*>(s_wb + bsel + q * kTile);
     };
-    // FP64 contraction of a 16 x 16 accumulator with w over its row tile p
-    auto row_contract = [&](const f32x4& acc, int p) {
-        double c = 0.0;
-        if (col < 2) {
-            const double* wr = s_w + p * kTile + 4 * kg;
-            c = static_cast<double>(acc[0]) * wr[0] + static_cast<double>(acc[1]) * wr[1] +
-                static_cast<double>(acc[2]) * wr[2] + static_cast<double>(acc[3]) * wr[3];
-            if (col == 1) c *= static_cast<double>(a.col1);
-        }
+    // FP64 contraction of a 16 x 16 accumulator's column with w over row tile p
+    auto col_contract = [&](const f32x4& acc, int p) {
+        const double* wr = s_w + p * kTile + 4 * kg;
+        double c = static_cast<double>(acc[0]) * wr[0] + static_cast<double>(acc[1]) * wr[1] +
+                   static_cast<double>(acc[2]) * wr[2] + static_cast<double>(acc[3]) * wr[3];
+        if (col & 1) c *= static_cast<double>(a.col1);
         return c;
     };
+    auto row_contract = [&](const f32x4& acc, int p) { return col < 2 ? col_contract(acc, p) : 0.0; };
 
     // hi/lo split of one coefficient into the B-operand table
     auto split_one = [&](int k, double w) {
@@ -266,18 +322,64 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
     for (int k = tid; k < Kp; k += NT) split_one(k, s_w[k]);
     if (tid < NW * kMaxIpw) s_ypart[tid / kMaxIpw][tid % kMaxIpw] = 0.0;
     __syncthreads();
+    // the exchange records as a buffer: 16-byte accesses with agent coherence
+    // (sc1, the cache policy of relaxed agent-scope atomics); xbuf holds
+    // 2 K records of kXStride words
+    constexpr int kSc1 = 16;
+    const __amdgpu_buffer_rsrc_t xrec = __builtin_amdgcn_make_buffer_rsrc(
+        a.xbuf, 0, static_cast<int>(sizeof(unsigned long long) * 2 * kXStride * K), 0x00020000);
 
     for (int s = 0; s < a.nsteps; ++s) {
         const uint32_t tag = a.tag0 + static_cast<uint32_t>(s) + 1u;
         // ---- contraction of this wave's blocks -------------------------------
-        {
-            // B operands come from LDS in chunks of kBChunk blocks, the next
-            // chunk loading while the current one feeds the MFMAs
+        // B operands come from LDS in chunks of kBChunk blocks, the next chunk
+        // loading while the current one feeds the MFMAs
+        const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
+        if (RB <= 18 && pairs) {
+            uint4 bq[2][kBChunk];
+#pragma unroll
+            for (int jj = 0; jj < kBChunk && jj < RB; ++jj) bq[0][jj] = b_pair(jj);
+            f32x4 acc0 = zero4, acc1 = zero4;
+#pragma unroll
+            for (int c = 0; c < RB; c += kBChunk) {
+                const int cb = (c / kBChunk) & 1;
+#pragma unroll
+                for (int jj = 0; jj < kBChunk; ++jj)
+                    if (c + kBChunk + jj < RB) bq[cb ^ 1][jj] = b_pair(c + kBChunk + jj);
+#pragma unroll
+                for (int jj = 0; jj < kBChunk; ++jj) {
+                    const int j = c + jj;
+                    if (j >= RB) break;
+                    if (j & 1) acc1 = mfma16<ELEM>(areg[j], bq[cb][jj], acc1);
+                    else acc0 = mfma16<ELEM>(areg[j], bq[cb][jj], acc0);
+                }
+            }
+            // each lane contracts its column with its group's row tile; the
+            // groups then come together in order in columns 0 and 1, one wave
+            // sum per mode
+            const double cv = lane_seg < nseg ? col_contract(acc0 + acc1, lane_p) : 0.0;
+            double part = cv;
+            auto flush_mode = [&](int sg) {
+                const double v = wave_total(col < 2 ? part : 0.0);
+                if (lane == 0) s_ypart[wv][(seg_t >> (4 * sg)) & 15u] += v;
+            };
+#pragma unroll
+            for (int sg = 1; sg < 8; ++sg) {
+                if (sg >= nseg) break;
+                const double cs = from_pair(cv, sg);
+                if ((seg_end >> (sg - 1)) & 1u) {
+                    flush_mode(sg - 1);
+                    part = cs;
+                } else {
+                    part += cs;
+                }
+            }
+            if (nseg > 0) flush_mode(nseg - 1);
+        } else {
             uint4 bq[2][kBChunk];
 #pragma unroll
             for (int jj = 0; jj < kBChunk && jj < RB; ++jj)
                 bq[0][jj] = b_operand(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mreg), jj)));
-            const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
             f32x4 acc0 = zero4, acc1 = zero4;
             double part = 0.0;
 #pragma unroll
@@ -334,7 +436,7 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
         }
         __syncthreads();
         // ---- publish y_i of this workgroup ---------------------------------------
-        unsigned long long* xb = a.xbuf + static_cast<size_t>(tag & 1u) * K * kXStride;
+        const int xb = static_cast<int>((tag & 1u) * static_cast<uint32_t>(K) * kXStride);   // word offset
         if (tid < a.ipw) {
             const int i = g + a.G * tid;
             if (i < K) {
@@ -343,11 +445,17 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
                 for (int w = 1; w < NW; ++w) ys += s_ypart[w][tid];   // waves in order
                 const double y = ys * a.inv_scale[g * a.ipw + tid];
                 const unsigned long long bits = __double_as_longlong(y);
-                const unsigned long long tg = static_cast<unsigned long long>(tag) << 32;
-                __hip_atomic_store(xb + kXStride * i, tg | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(xb + kXStride * i + 1, tg | (bits & 0xFFFFFFFFull), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                const u32x4 rec = {static_cast<uint32_t>(bits >> 32), tag, static_cast<uint32_t>(bits), tag};
+                __builtin_amdgcn_raw_buffer_store_b128(rec, xrec, (xb + kXStride * i) * 8, 0, kSc1);
             }
+        }
+        // ---- energy of w before the step, while wave 0 polls ------------------------
+        if (wv == NW - 1) {
+#pragma clang fp contract(off)
+            double s1 = 0.0;
+            for (int k = lane; k < K; k += 64) s1 += s_w[k] * s_w[k];
+            s1 = wave_total(s1);
+            if (lane == 0) s_red[NW] = s1;
         }
         // ---- gather all y (one wave polls; bounded wait) ----------------------------
         // Lane l reads records l, l + 64, ..., four at a time: a poll pass
@@ -362,22 +470,19 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
             for (int r0 = 0; r0 < nr && !s_flag; r0 += kGroup) {
                 uint32_t pend = (1u << min(kGroup, nr - r0)) - 1u;
                 while (pend) {
-                    unsigned long long v0[kGroup], v1[kGroup];
+                    u32x4 v[kGroup];
 #pragma unroll
                     for (int r = 0; r < kGroup; ++r) {
-                        v0[r] = v1[r] = 0ull;
-                        if ((pend >> r) & 1u) {
-                            const unsigned long long* rec =
-                                xb + static_cast<size_t>(kXStride) * (lane + 64 * (r0 + r));
-                            v0[r] = __hip_atomic_load(rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            v1[r] = __hip_atomic_load(rec + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        }
+                        v[r] = u32x4{0u, 0u, 0u, 0u};
+                        if ((pend >> r) & 1u)
+                            v[r] = __builtin_amdgcn_raw_buffer_load_b128(
+                                xrec, (xb + kXStride * (lane + 64 * (r0 + r))) * 8, 0, kSc1);
                     }
 #pragma unroll
                     for (int r = 0; r < kGroup; ++r) {
-                        if (((pend >> r) & 1u) && (v0[r] >> 32) == tag && (v1[r] >> 32) == tag) {
+                        if (((pend >> r) & 1u) && v[r].y == tag && v[r].w == tag) {
                             s_y[lane + 64 * (r0 + r)] = __longlong_as_double(
-                                static_cast<long long>(((v0[r] & 0xFFFFFFFFull) << 32) | (v1[r] & 0xFFFFFFFFull)));
+                                static_cast<long long>((static_cast<unsigned long long>(v[r].x) << 32) | v[r].z));
                             pend &= ~(1u << r);
                         }
                     }
@@ -395,27 +500,19 @@ __device__ __forceinline__ void persistent_body(const PArgs& a, const int g)
         // ---- FP64 update, identical in every workgroup (psfsolver.cpp:34-59) -----------
         {
 #pragma clang fp contract(off)
-            double s1 = 0.0, s2 = 0.0;
+            double s2 = 0.0;
             for (int k = tid; k < K; k += NT) {
-                const double w = s_w[k];
-                const double wn = w + a.dt * s_y[k];
+                const double wn = s_w[k] + a.dt * s_y[k];
                 s_y[k] = wn;
-                s1 += w * w;
                 s2 += wn * wn;
             }
-            s1 = wave_total(s1);
             s2 = wave_total(s2);
-            if (lane == 0) {
-                s_red[wv] = s1;
-                s_red[NW + wv] = s2;
-            }
+            if (lane == 0) s_red[wv] = s2;
             __syncthreads();
-            double e1 = s_red[0], e2 = s_red[NW];
+            const double e1 = s_red[NW];
+            double e2 = s_red[0];
 #pragma unroll
-            for (int w = 1; w < NW; ++w) {   // waves in order
-                e1 += s_red[w];
-                e2 += s_red[NW + w];
-            }
+            for (int w = 1; w < NW; ++w) e2 += s_red[w];   // waves in order
             if (!(e2 > 0.0) || !__builtin_isfinite(e2) || !__builtin_isfinite(e1)) {
                 if (tid == 0) s_flag = 1;
                 __syncthreads();
