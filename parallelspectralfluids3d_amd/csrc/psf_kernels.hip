@@ -643,20 +643,26 @@ __device__ __forceinline__ float4 trace_advance(const TraceParams& P, const gptr
                                                 const TraceCell& c, const TraceFlux& f)
 {
 #pragma clang fp contract(off)
-    // the x+ neighbour from each y and z pair: dx is 1 (or 0 at a clamp); a
-    // dx of any other value is patched with its own loads
+    // the x+ neighbour from each y and z pair: dx is 1 away from the walls,
+    // 0 at a clamp (the pair's first face again); a dx of any other value is
+    // patched with its own loads.  Waves without a clamped particle skip the
+    // patch.
     const int dx = c.xP - c.xM;
-    auto second = [&](const FluxPair& q) { return dx == 1 ? q.y : q.x; };
-    float r3y = second(f.y[0]), r4y = second(f.y[1]), r7y = second(f.y[2]), r8y = second(f.y[3]);
-    float r3z = second(f.z[0]), r4z = second(f.z[1]), r7z = second(f.z[2]), r8z = second(f.z[3]);
-    if (__builtin_expect(dx != 0 && dx != 1, 0)) {
-        const gptr<const char> ub = reinterpret_cast<gptr<const char>>(u);
-        auto one = [&](uint32_t b) { return *reinterpret_cast<gptr<const float>>(ub + b); };
-        uint32_t iy[4], iz[4];
-        yz_offsets(P, c, iy, iz);
-        const uint32_t db = static_cast<uint32_t>(dx) << 2;
-        r3y = one(iy[0] + db); r4y = one(iy[1] + db); r7y = one(iy[2] + db); r8y = one(iy[3] + db);
-        r3z = one(iz[0] + db); r4z = one(iz[1] + db); r7z = one(iz[2] + db); r8z = one(iz[3] + db);
+    float r3y = f.y[0].y, r4y = f.y[1].y, r7y = f.y[2].y, r8y = f.y[3].y;
+    float r3z = f.z[0].y, r4z = f.z[1].y, r7z = f.z[2].y, r8z = f.z[3].y;
+    if (__builtin_expect(dx != 1, 0)) {
+        if (dx == 0) {
+            r3y = f.y[0].x; r4y = f.y[1].x; r7y = f.y[2].x; r8y = f.y[3].x;
+            r3z = f.z[0].x; r4z = f.z[1].x; r7z = f.z[2].x; r8z = f.z[3].x;
+        } else {
+            const gptr<const char> ub = reinterpret_cast<gptr<const char>>(u);
+            auto one = [&](uint32_t b) { return *reinterpret_cast<gptr<const float>>(ub + b); };
+            uint32_t iy[4], iz[4];
+            yz_offsets(P, c, iy, iz);
+            const uint32_t db = static_cast<uint32_t>(dx) << 2;
+            r3y = one(iy[0] + db); r4y = one(iy[1] + db); r7y = one(iy[2] + db); r8y = one(iy[3] + db);
+            r3z = one(iz[0] + db); r4z = one(iz[1] + db); r7z = one(iz[2] + db); r8z = one(iz[3] + db);
+        }
     }
     const float s0 = face_sign(0, 0, 0, c.zO), s1 = face_sign(1, 0, 0, c.zO);
     const float s2 = face_sign(2, 0, c.yO, 0), s3 = face_sign(3, 0, c.yO, 0);
@@ -707,11 +713,10 @@ __device__ __forceinline__ float4 trace_advance(const TraceParams& P, const gptr
         const float a2 = f.z[2].x * ba + f.z[3].x * bb, a3 = r7z * ba + r8z * bb;
         vz = mixf(mixf(a0, a1, nz0), mixf(a2, a3, nz0), nz2);
     }
-    // glm::clamp(v, lo, hi) = min(max(v, lo), hi)
-    auto clampf = [](float v, float lo, float hi) {
-        const float t = (v < lo) ? lo : v;
-        return (hi < t) ? hi : t;
-    };
+    // glm::clamp(v, lo, hi) = min(max(v, lo), hi), for lo < hi the median
+    // of (v, lo, hi): one v_med3_f32.  (The two differ only for a NaN v,
+    // which the CPU keeps, and for v = -0 on a bound of 0.)
+    auto clampf = [](float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); };
     p.x = clampf(p.x + P.dtf * vx, P.aabb_min[0], P.aabb_max[0]);
     p.y = clampf(p.y + P.dtf * vy, P.aabb_min[1], P.aabb_max[1]);
     p.z = clampf(p.z + P.dtf * vz, P.aabb_min[2], P.aabb_max[2]);
