@@ -1,12 +1,15 @@
 #!/usr/bin/env python3
 """Benchmark of the reduced-space fluid step on one MI355X.
 
-Workload: the 64^3 closed box of the precision configuration -- K = 256
-modes (analytic discrete sin/cos eigenvectors of the box's DEC 1-form
-Laplacian), omega ~ N(0,1) from seed 0 at unit energy, dt = 0.01, no
-viscosity or forcing, 2^20 particles uniform in the box.  A "step" is one
+Workload: the 64^3 closed box of the precision configuration -- the K = 256
+lowest eigenpairs of the box's DEC 1-form Laplacian M as the reference
+builds it (psfsolver.cpp:236-342: wall edges weighted, eigenvalues above
+1e-10), solved on the GPU, and C built from them on the GPU; omega ~ N(0,1)
+from seed 0 at unit energy, dt = 0.01, no viscosity or forcing, 2^20
+particles uniform in the box.  (--basis analytic: the analytic sin/cos box
+modes instead, eigenvectors of the free-slip Laplacian.)  A "step" is one
 AbstractSolver::integrate() frame (psfsolver.cpp:21-234): reduced step
-(BF16 hi+lo MFMA contraction, FP64 update), u = Phi w (FP32 Phi, FP64 sums)
+(BF16 hi+lo MFMA contraction, FP64 update), u = Phi w (FP32 Phi, FP32 sums)
 and one particle pass.  With a frame window W (default 32) the reduced
 steps of W frames run in one persistent launch and their W velocity fields
 in one pass over Phi; every frame still gets its own omega, u and particle
@@ -56,26 +59,44 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_scene(n, K, N, seed=0):
-    from parallelspectralfluids3d_amd import advection, basis
+def build_scene(n, K, N, seed=0, basis_kind="eigen", ctx=None):
+    """The closed box's basis, C, forcing, omega and particles.  basis_kind
+    "eigen": the K lowest eigenpairs of the reference's M (the device
+    eigensolver, penalty-lifted) and C built on the device in an FP64 scene
+    and read back; "analytic": the analytic sin/cos modes and their C on the
+    host."""
+    from parallelspectralfluids3d_amd import _lib as L
+    from parallelspectralfluids3d_amd import advection, basis, hip
     from parallelspectralfluids3d_amd.grid import Grid, box_mask
     g = Grid(n, n, n)
     h = 1.0 / n
+    info = None
     t = time.time()
-    Phi, lam, tab = basis.analytic_box_modes(g, K)
-    V = (basis.curl_operator(g, h) @ Phi.T).T
-    log(f"basis {g.nF} faces x {K} modes in {time.time() - t:.1f}s")
-    t = time.time()
-    C = advection.build_advection_box_analytic(g, h, Phi, lam, tab, V)
-    log(f"structure tensor C ({K}^3) in {time.time() - t:.1f}s")
-    del V
+    if basis_kind == "eigen":
+        Phi, lam, info = basis.laplacian_eigenbasis_device(g, h, box_mask(g), K, ctx)
+        log(f"basis: {K} eigenpairs of M on {g.nF} faces in {time.time() - t:.1f}s ({info['products']} products)")
+        t = time.time()
+        s64 = hip.Scene(ctx, hip.make_desc(g, h, K, L.C_FP64, L.PHI_FP64, 0))
+        advection.build_advection_device(s64, g, h, box_mask(g), Phi, lam)
+        C = s64.get_advection()
+        s64.close()
+        log(f"structure tensor C ({K}^3) on the device in {time.time() - t:.1f}s")
+    else:
+        Phi, lam, tab = basis.analytic_box_modes(g, K)
+        V = (basis.curl_operator(g, h) @ Phi.T).T
+        log(f"basis {g.nF} faces x {K} modes in {time.time() - t:.1f}s")
+        t = time.time()
+        C = advection.build_advection_box_analytic(g, h, Phi, lam, tab, V)
+        log(f"structure tensor C ({K}^3) in {time.time() - t:.1f}s")
+        del V
     gvec = basis.project(Phi, basis.gravity_field(g, box_mask(g)))
     omega = basis.unit_energy_omega(K, seed)
     rng = np.random.default_rng(seed + 1)
     parts = np.zeros((N, 4), np.float32)
     parts[:, :3] = rng.random((N, 3)) - 0.5
     parts[:, 3] = np.inf
-    return dict(grid=g, h=h, Phi=Phi, lam=lam, C=C, g=gvec, omega=omega, parts=parts)
+    return dict(grid=g, h=h, Phi=Phi, lam=lam, C=C, g=gvec, omega=omega, parts=parts, basis=basis_kind,
+                basis_info=info)
 
 
 def make_hip_scene(ctx, sc, c_prec, phi_prec, with_phi, N):
@@ -650,6 +671,9 @@ def main():
     ap.add_argument("--modes", type=int, default=256)
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--reduced-steps", type=int, default=1000)
+    ap.add_argument("--basis", choices=("eigen", "analytic"), default="eigen",
+                    help="eigen: the K lowest eigenpairs of the reference's M, solved on the GPU (default); "
+                         "analytic: the analytic sin/cos box modes")
     ap.add_argument("--window", type=int, default=32,
                     help="frame window: reduced steps and reconstructions of this many frames are computed "
                          "together (1 = one frame at a time)")
@@ -719,9 +743,9 @@ def main():
         return
 
     n, K, N = args.grid, args.modes, args.particles
-    sc = build_scene(n, K, N)
-    F = sc["grid"].nF
     ctx = hip.Context(0)
+    sc = build_scene(n, K, N, basis_kind=args.basis, ctx=ctx)
+    F = sc["grid"].nF
 
     # ---- frame loop (the value) ---------------------------------------------------
     s = make_hip_scene(ctx, sc, L.C_BF16X2, PHI_PREC, True, N)
@@ -954,9 +978,14 @@ def main():
                         "renormalisation, viscosity); u = Phi w with FP32 Phi and w and FP32 sums on the FP32 matrix "
                         "cores (one pass over Phi per frame window); FP32 particles",
         "data": "synthetic",
-        "config": {"workload": f"{n}^3 closed box, K={K} analytic DEC eigenbasis, {N} particles, dt=0.01, "
+        "config": {"workload": f"{n}^3 closed box, K={K} "
+                               f"{'lowest eigenpairs of the DEC 1-form Laplacian M (wall edges weighted)' if sc['basis'] == 'eigen' else 'analytic sin/cos box modes'}, "
+                               f"{N} particles, dt=0.01, "
                                f"one integrate() frame per step", "grid": n, "K": K, "particles": N,
                    "frame_window": W},
+        "basis": {"kind": sc["basis"], "solve": sc["basis_info"],
+                  "note": "the frame loop and the precision sections run on this basis; the CPU oracle gets the "
+                          "same Phi, lambda and C"},
         "ms_per_frame_one_at_a_time": ms_frame1,
         "particle_steps_per_s_one_at_a_time": N / (ms_frame1 * 1e-3),
         "reduced_steps_per_s": res.get("bf16x2"),
