@@ -62,10 +62,7 @@ namespace psf {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-// exchange words in the global address space: the records are read and
-// written with global (not flat) atomic instructions, as in the single-scene
-// step, whose pointers come straight from its kernel arguments
-typedef __attribute__((address_space(1))) unsigned long long xword;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
@@ -79,7 +76,8 @@ constexpr int kGMeta = kNW * kGRB;         // every block of a workgroup is regi
 constexpr int kXW = 2 * kGroupMaxScenes;   // 64-bit words per mode record
 constexpr uint64_t kGWait = 200000000ull;  // s_memrealtime runs at 100 MHz: 2 s
 constexpr int kDiagWords = 8;
-constexpr int kPollR = kGroupMaxK / (2 * kNW);   // record pairs each wave gathers
+constexpr int kPollQ = kGroupMaxK / (4 * kNW);   // four-record groups each wave gathers
+constexpr int kSc1 = 16;                          // buffer cache policy sc1: agent coherence
 
 }  // namespace
 
@@ -154,6 +152,9 @@ __global__ __launch_bounds__(64 * kNW, 1) void k_group_step(const GroupArgs* __r
     const int col = lane & 15, kg = lane >> 4;
     const int K0 = a.K0, S = a.S;
     const int Kp = (K0 + kTile - 1) / kTile * kTile;
+    // this instance's exchange slot (2 kGroupMaxK records of kXW words) as a buffer
+    const __amdgpu_buffer_rsrc_t xrec = __builtin_amdgcn_make_buffer_rsrc(
+        a.xbuf, 0, static_cast<int>(sizeof(unsigned long long) * 2 * kGroupMaxK * kXW), 0x00020000);
 
     for (int b = tid; b < a.bpw; b += NT) s_meta[b] = a.meta[b];
     if (tid < SMAX) {
@@ -276,7 +277,12 @@ __global__ __launch_bounds__(64 * kNW, 1) void k_group_step(const GroupArgs* __r
         }
         __syncthreads();
         // ---- publish y_i of every active scene -------------------------------------
-        xword* xb = (xword*)(a.xbuf + static_cast<size_t>(tag & 1u) * K0 * kXW);   // generic to global
+        // A scene's y_i is the 16-byte (hi, lo) pair of words 2 s, 2 s + 1 of
+        // record i, each word tagged, stored and read with single 16-byte
+        // accesses at agent coherence (sc1, the cache policy of relaxed
+        // agent-scope atomics); a pair seen half old and half new fails the
+        // tag check and is read again.
+        const int xb = static_cast<int>((tag & 1u) * static_cast<uint32_t>(K0) * kXW);   // word offset
         for (int e = tid; e < a.ipw * SMAX; e += NT) {
             const int t = e / SMAX, s = e - t * SMAX;
             double ys = s_ypart[0][t][s];
@@ -289,68 +295,64 @@ __global__ __launch_bounds__(64 * kNW, 1) void k_group_step(const GroupArgs* __r
             const int i = g + a.G * t;
             if (s < S && i < s_K[s]) {
                 const unsigned long long bits = __double_as_longlong(ys);
-                const unsigned long long tg = static_cast<unsigned long long>(tag) << 32;
-                xword* rec = xb + static_cast<size_t>(i) * kXW + 2 * s;
-                __hip_atomic_store(rec, tg | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(rec + 1, tg | (bits & 0xFFFFFFFFull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const u32x4 rec = {static_cast<uint32_t>(bits >> 32), tag, static_cast<uint32_t>(bits), tag};
+                __builtin_amdgcn_raw_buffer_store_b128(rec, xrec, (xb + i * kXW + 2 * s) * 8, 0, kSc1);
             }
         }
         // ---- gather: whole records per wave load ------------------------------------
-        // Lane l of wave wv reads word l & 31 (the hi or lo half of scene
-        // (l & 31) / 2) of record 2 (wv + kNW r) + (l >> 5), r = 0, 1, ...: one
-        // wave-wide load covers two adjacent records, 512 contiguous bytes.  Each
-        // word lands in its half of s_y[s][k].
+        // Lane l of wave wv reads the pair of scene l & 15 in record
+        // 4 (wv + kNW r) + (l >> 4), r = 0, 1, ...: one wave-wide load covers four
+        // adjacent records, 1 KiB contiguous.  Each pair lands in s_y[s][k].
+        // Passes follow each other without a sleep.
         {
-            const int wd = lane & 31, sw = wd >> 1, half = lane >> 5;
+            const int sw = lane & 15, q = lane >> 4;
             const int Ksw = sw < S ? s_K[sw] : 0;
             uint32_t pend = 0;
 #pragma unroll
-            for (int r = 0; r < kPollR; ++r)
-                if (2 * (wv + kNW * r) + half < Ksw) pend |= 1u << r;
-            // record 2 (wv + kNW r) + half, word wd: a wave-uniform base per r plus
+            for (int r = 0; r < kPollQ; ++r)
+                if (4 * (wv + kNW * r) + q < Ksw) pend |= 1u << r;
+            // record 4 (wv + kNW r) + q, pair sw: a wave-uniform offset per r plus
             // the same lane offset for every r
-            const xword* wbase = xb + static_cast<size_t>(2 * wv) * kXW;
-            const int loff = half * kXW + wd;
-            uint32_t* ydst = reinterpret_cast<uint32_t*>(s_y + sw * kSWS + 2 * wv + half) + (1 - (wd & 1));
+            const int loff = xb + (4 * wv + q) * kXW + 2 * sw;
+            double* ydst = s_y + sw * kSWS + 4 * wv + q;
             const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
             bool timed_out = false;
-            unsigned long long vlast = 0ull;
+            u32x4 vlast = {0u, 0u, 0u, 0u};
             while (pend) {
-                unsigned long long v[kPollR];
+                u32x4 v[kPollQ];
 #pragma unroll
-                for (int r = 0; r < kPollR; ++r) {
-                    v[r] = 0ull;
+                for (int r = 0; r < kPollQ; ++r) {
+                    v[r] = u32x4{0u, 0u, 0u, 0u};
                     if ((pend >> r) & 1u)
-                        v[r] = __hip_atomic_load(wbase + r * (2 * kNW * kXW) + loff, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
+                        v[r] = __builtin_amdgcn_raw_buffer_load_b128(xrec, (loff + r * (4 * kNW * kXW)) * 8, 0, kSc1);
                 }
 #pragma unroll
-                for (int r = 0; r < kPollR; ++r) {
-                    if (((pend >> r) & 1u) && (v[r] >> 32) == tag) {
-                        ydst[2 * 2 * kNW * r] = static_cast<uint32_t>(v[r]);
+                for (int r = 0; r < kPollQ; ++r) {
+                    if (((pend >> r) & 1u) && v[r].y == tag && v[r].w == tag) {
+                        ydst[4 * kNW * r] =
+                            __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(v[r].x) << 32) | v[r].z));
                         pend &= ~(1u << r);
                     }
                 }
-                if (pend) {
-                    if (__builtin_amdgcn_s_memrealtime() - t0 > a.wait_ticks) {
-                        timed_out = true;
+                if (pend && __builtin_amdgcn_s_memrealtime() - t0 > a.wait_ticks) {
+                    timed_out = true;
 #pragma unroll
-                        for (int r = 0; r < kPollR; ++r)
-                            if (r == __builtin_ctz(pend)) vlast = v[r];
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
+                    for (int r = 0; r < kPollQ; ++r)
+                        if (r == __builtin_ctz(pend)) vlast = v[r];
+                    break;
                 }
             }
             if (timed_out && atomicCAS(a.diag, 0ull, 1ull) == 0ull) {
                 // the first wait of the launch to give up records what it waited for
                 const int r = __builtin_ctz(pend);
+                const bool hi_ok = vlast.y == tag;
                 a.diag[1] = static_cast<unsigned long long>(a.instance) << 32 | static_cast<uint32_t>(g);
                 a.diag[2] = static_cast<unsigned long long>(step) << 32 | static_cast<uint32_t>(a.nsteps);
-                a.diag[3] = static_cast<unsigned long long>(2 * (wv + kNW * r) + half) << 32 | static_cast<uint32_t>(sw);
+                a.diag[3] = static_cast<unsigned long long>(4 * (wv + kNW * r) + q) << 32 | static_cast<uint32_t>(sw);
                 a.diag[4] = tag;
-                a.diag[5] = vlast;
-                a.diag[6] = static_cast<unsigned long long>(wd & 1);
+                a.diag[5] = hi_ok ? (static_cast<unsigned long long>(vlast.w) << 32 | vlast.z)
+                                  : (static_cast<unsigned long long>(vlast.y) << 32 | vlast.x);
+                a.diag[6] = hi_ok ? 1ull : 0ull;
                 a.diag[7] = static_cast<unsigned long long>(S) << 32 | static_cast<uint32_t>(a.G);
             }
             if (__any(timed_out) && lane == 0) s_flag = 2;
