@@ -44,10 +44,11 @@ FP32_PEAK = 157.3e12
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --counter-pass`
 # (fetched KiB x 2, the factor a 1 GiB hipMemcpy reads back, plus written
 # KiB); counters cannot be read from inside this process.
-RECON_TRAFFIC_64_256 = 821_147_648          # k_reconstruct<float, 1>: 399,391 KiB read, 3,120 KiB written
-WINDOW_TRAFFIC_64_256_W32 = 929_118_528     # k_reconstruct_mfma<float, 2>: 403,749 / 99,845 KiB
-TRACE_TRAFFIC_64_1M = 37_685_248            # k_trace: 10,209 / 16,384 KiB
-TRACE_FRAMES_TRAFFIC_64_1M = 66_196_608     # k_trace_frames (8 frames per launch): 24,131 / 16,384 KiB
+# (the reconstructions: FP32 Phi and w with FP32 sums, the PHI_PREC below)
+RECON_TRAFFIC_64_256 = 825_229_632          # k_reconstruct_mfma<float, 1, float>: 401,384 KiB read, 3,120 KiB written
+WINDOW_TRAFFIC_64_256_W32 = 927_384_704     # k_reconstruct_mfma<float, 2, float>: 402,905 / 99,840 KiB
+TRACE_TRAFFIC_64_1M = 37_672_448            # k_trace: 10,203 / 16,384 KiB
+TRACE_FRAMES_TRAFFIC_64_1M = 70_158_336     # k_trace_frames (8 frames per launch): 26,065 / 16,384 KiB
 
 
 # u = Phi w of the frame loop: FP32 Phi and w with FP32 sums on the FP32 matrix
@@ -851,16 +852,17 @@ def main():
                                       traffic=TRACE_FRAMES_TRAFFIC_64_1M if (n, N) == (64, 1 << 20) else None,
                                       note="algorithmic bytes = particle-steps x 32 B + frames x 4F; the fused "
                                            "launch keeps records in registers across its frames and moves "
-                                           "compulsory_bytes_per_launch. Bound by issue, not HBM: about 220 "
-                                           "vector instructions per particle-step and 12 dependent flux-pair "
-                                           "gathers; address unit busy 75-77% (TA_BUSY_avr), 69% of wave-cycles "
-                                           "waiting on memory (SQ_WAIT_INST_ANY)")
+                                           "compulsory_bytes_per_launch. Bound by the vector-memory path, not HBM: 201 "
+                                           "vector instructions and 12 dependent flux-pair gathers per "
+                                           "particle-step; texture-data unit busy 85% (TD_TD_BUSY_sum per CU), "
+                                           "address unit 73% (TA_BUSY_avr), 66% of wave-cycles waiting on memory "
+                                           "(SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES)")
     rec_bytes = F * K * 4 + F * 4 + K * 8
-    # (the calibrated traffic constants were measured for the FP64-sum kernels)
+    # (the calibrated traffic constants were measured for the FP32-sum kernels)
     fp64_sums = PHI_PREC != 2
     roof["k_reconstruct"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
                                  achieved=rec_bytes / (ms_rec * 1e-3) / 1e9, algorithmic_bytes_per_launch=rec_bytes,
-                                 traffic=RECON_TRAFFIC_64_256 if (n, K) == (64, 256) and fp64_sums else None)
+                                 traffic=RECON_TRAFFIC_64_256 if (n, K) == (64, 256) and not fp64_sums else None)
     if W > 1:
         wflops = 2.0 * F * K * W
         wbytes = F * K * 4 + W * (F * 4 + K * 8)
@@ -868,7 +870,7 @@ def main():
             roof["k_reconstruct_window"] = dict(bound="fp64", unit="TFLOP/s", peak=FP64_PEAK / 1e12,
                                                 achieved=wflops / (ms_wrec * 1e-3) / 1e12,
                                                 algorithmic_flops_per_launch=wflops, algorithmic_bytes_per_launch=wbytes,
-                                                traffic=WINDOW_TRAFFIC_64_256_W32 if (n, K, W) == (64, 256, 32) else None)
+                                                traffic=None)
         else:
             # FP32 matrix cores: 2 F K W flops take less time at their peak than
             # the pass over Phi at the HBM peak, so HBM bounds the launch
@@ -876,7 +878,8 @@ def main():
                                                 achieved=wbytes / (ms_wrec * 1e-3) / 1e9,
                                                 algorithmic_bytes_per_launch=wbytes, algorithmic_flops_per_launch=wflops,
                                                 fp32_mfma_tflops=wflops / (ms_wrec * 1e-3) / 1e12,
-                                                fp32_mfma_peak_tflops=FP32_PEAK / 1e12, traffic=None)
+                                                fp32_mfma_peak_tflops=FP32_PEAK / 1e12,
+                                                traffic=WINDOW_TRAFFIC_64_256_W32 if (n, K, W) == (64, 256, 32) else None)
         cbytes = K * K * (K + 1) // 2 * 4 + 32 * K
         roof["k_persistent"] = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9,
                                     achieved=cbytes / (kern["k_persistent"] * 1e-3) / 1e9,
