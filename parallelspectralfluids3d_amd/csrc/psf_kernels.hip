@@ -283,9 +283,14 @@ __device__ __forceinline__ int scene_of_block(const Tab* __restrict__ tab, int n
 // of one frame are 16 consecutive faces.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-template <typename T, int NT, typename ACC = double>
+// PAIR: a second scene on the same Phi (K2 <= K modes, its own history and
+// fields) rides along: every Phi load feeds both scenes' MFMAs.  Its modes
+// k >= K2 enter as 0, so its sums are the MFMAs it would run alone plus
+// exact zeros: the same bits.
+template <typename T, int NT, typename ACC = double, bool PAIR = false>
 __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint32_t F, int K, const double* hist_,
-                                                 int hist_ld, int W, float* uh_, size_t ldU, uint32_t blk)
+                                                 int hist_ld, int W, float* uh_, size_t ldU, uint32_t blk,
+                                                 int K2 = 0, const double* hist2_ = nullptr, float* uh2_ = nullptr)
 {
     static_assert(std::is_same<ACC, double>::value || std::is_same<T, float>::value,
                   "FP32 accumulation takes FP32 Phi");
@@ -294,17 +299,21 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
     const gptr<const T> Phi = as_global(Phi_);
     const gptr<const double> hist = as_global(hist_);
     const gptr<float> uh = as_global(uh_);
+    const gptr<const double> hist2 = as_global(hist2_);
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const size_t f0 = (static_cast<size_t>(blk) * 4 + wv) * 64;
     if (f0 >= ldF) return;
     const int li = lane & 15, lk = lane >> 4;
     const gptr<const T> arow = Phi + f0 + 4 * li;
-    accv acc[4][NT];
+    constexpr int NS = PAIR ? 2 : 1;   // scenes
+    accv acc[NS][4][NT];
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
+    for (int sc = 0; sc < NS; ++sc)
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = accv{0, 0, 0, 0};
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) acc[sc][rt][ct] = accv{0, 0, 0, 0};
     // frames of the lane's B column: t = 16 ct + li; columns t >= W load a
     // valid element (t clamped) and multiply by 0 instead
     bool tv[NT];
@@ -328,38 +337,49 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
             r.v[1] = __builtin_nontemporal_load(q + 1);
         }
     };
-    auto load_b = [&](int k, double (&b)[NT]) {
+    // B values of both scenes; the second's modes past K2 load mode K2 - 1
+    // (in bounds) and enter as 0
+    auto load_b = [&](int k, double (&b)[NS][NT]) {
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) b[ct] = hist[static_cast<size_t>(k) * hist_ld + tc[ct]];
+        for (int ct = 0; ct < NT; ++ct) b[0][ct] = hist[static_cast<size_t>(k) * hist_ld + tc[ct]];
+        if constexpr (PAIR) {
+            const int k2 = k < K2 ? k : K2 - 1;
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) b[1][ct] = hist2[static_cast<size_t>(k2) * hist_ld + tc[ct]];
+        }
     };
-    auto step = [&](const RawA& r, const double (&braw)[NT]) {
+    auto step = [&](const RawA& r, const double (&braw)[NS][NT], int k) {
         ACC a[4];
         if constexpr (sizeof(T) == 4) {
             a[0] = r.v[0].x; a[1] = r.v[0].y; a[2] = r.v[0].z; a[3] = r.v[0].w;
         } else {
             a[0] = r.v[0].x; a[1] = r.v[0].y; a[2] = r.v[1].x; a[3] = r.v[1].y;
         }
-        ACC b[NT];
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) b[ct] = tv[ct] ? static_cast<ACC>(braw[ct]) : ACC(0);
+        for (int sc = 0; sc < NS; ++sc) {
+            const bool kv = sc == 0 || k < K2;
+            ACC b[NT];
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt)
+            for (int ct = 0; ct < NT; ++ct) b[ct] = (tv[ct] && kv) ? static_cast<ACC>(braw[sc][ct]) : ACC(0);
 #pragma unroll
-            for (int ct = 0; ct < NT; ++ct) {
-                if constexpr (kF32)
-                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
-                else
-                    acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
-            }
+            for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < NT; ++ct) {
+                    if constexpr (kF32)
+                        acc[sc][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b[ct], acc[sc][rt][ct], 0, 0, 0);
+                    else
+                        acc[sc][rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[sc][rt][ct], 0, 0, 0);
+                }
+        }
     };
     // full k-steps (every k < K), branch-free, loads kPipe steps ahead: the
     // queue slot of step k4 is refilled with step k4 + kPipe (clamped to the
     // last full step; the extra loads are in bounds and unused)
-    constexpr int kPipe = 4;   // (6, 8 and 12 measured no faster with FP32 accumulation)
+    constexpr int kPipe = PAIR ? 3 : 4;   // (6, 8 and 12 measured no faster with FP32 accumulation)
     const int K4 = K >> 2;
     if (K4 > 0) {
         RawA qa[kPipe];
-        double qb[kPipe][NT];
+        double qb[kPipe][NS][NT];
 #pragma unroll
         for (int st = 0; st < kPipe; ++st) {
             const int k = 4 * (st < K4 ? st : K4 - 1) + lk;
@@ -370,7 +390,7 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
         for (; k4 + kPipe <= K4; k4 += kPipe) {
 #pragma unroll
             for (int st = 0; st < kPipe; ++st) {
-                step(qa[st], qb[st]);
+                step(qa[st], qb[st], 4 * (k4 + st) + lk);
                 const int nx = k4 + st + kPipe;
                 const int k = 4 * (nx < K4 ? nx : K4 - 1) + lk;
                 load_a(k, qa[st]);
@@ -382,20 +402,22 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
         }
 #pragma unroll
         for (int st = 0; st < kPipe; ++st)
-            if (st < K4 - k4) step(qa[st], qb[st]);
+            if (st < K4 - k4) step(qa[st], qb[st], 4 * (k4 + st) + lk);
     }
     // the last, partial k-step: modes past K enter as 0 * 0
     if (K & 3) {
         const int k = 4 * K4 + lk;
         RawA ra{};
-        double rb[NT];
+        double rb[NS][NT];
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) rb[ct] = 0.0;
+        for (int sc = 0; sc < NS; ++sc)
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) rb[sc][ct] = 0.0;
         if (k < K) {
             load_a(k, ra);
             load_b(k, rb);
         }
-        step(ra, rb);
+        step(ra, rb, k);
     }
     // The stores go through a wave-private LDS tile of 16 frames x 64 faces:
     // a lane's values of one frame are 16-byte pieces 64 bytes apart, and
@@ -406,14 +428,16 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
     __shared__ __attribute__((aligned(16))) float tiles[4][16 * kTileLd];
     float* const tile = tiles[wv];
 #pragma unroll
+    for (int sc = 0; sc < NS; ++sc)
+#pragma unroll
     for (int ct = 0; ct < NT; ++ct) {
         if (16 * ct >= W) break;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             // local face of slot r in tile 0 (tile rt adds rt)
             const int fl = kF32 ? 16 * lk + 4 * r : 4 * (lk + 4 * r);
-            const f32x4 o = {static_cast<float>(acc[0][ct][r]), static_cast<float>(acc[1][ct][r]),
-                             static_cast<float>(acc[2][ct][r]), static_cast<float>(acc[3][ct][r])};
+            const f32x4 o = {static_cast<float>(acc[sc][0][ct][r]), static_cast<float>(acc[sc][1][ct][r]),
+                             static_cast<float>(acc[sc][2][ct][r]), static_cast<float>(acc[sc][3][ct][r])};
             *reinterpret_cast<f32x4*>(tile + li * kTileLd + fl) = o;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -423,7 +447,7 @@ __device__ __forceinline__ void mfma_window_body(const T* Phi_, size_t ldF, uint
             const int row = lk + 4 * s4, t = 16 * ct + row;
             const f32x4 o = *reinterpret_cast<const f32x4*>(tile + row * kTileLd + 4 * li);
             if (t >= W) continue;
-            const gptr<float> ut = uh + static_cast<size_t>(t) * ldU;
+            const gptr<float> ut = (sc == 0 ? uh : as_global(uh2_)) + static_cast<size_t>(t) * ldU;
             const size_t f = f0 + 4 * li;
             if (f + 3 < F) {
                 __builtin_nontemporal_store(o, reinterpret_cast<gptr<f32x4>>(ut + f));
@@ -453,6 +477,20 @@ __global__ __launch_bounds__(256) void k_reconstruct_mfma_multi(const WindowScen
     const WindowScene& w = tab[scene_of_block(tab, n, blockIdx.x)];
     mfma_window_body<float, NT, ACC>(static_cast<const float*>(w.Phi), w.ldF, w.F, w.K, w.hist, W, W, w.uh, w.ldF,
                                      blockIdx.x - w.block0);
+}
+
+// Entries with a second scene (K2 > 0) on the same Phi: one pass over Phi for
+// both (FP32 sums)
+template <int NT>
+__global__ __launch_bounds__(256) void k_reconstruct_mfma_pairs(const WindowScene* __restrict__ tab, int n, int W)
+{
+    const WindowScene& w = tab[scene_of_block(tab, n, blockIdx.x)];
+    if (w.K2 > 0)
+        mfma_window_body<float, NT, float, true>(static_cast<const float*>(w.Phi), w.ldF, w.F, w.K, w.hist, W, W, w.uh,
+                                                 w.ldF, blockIdx.x - w.block0, w.K2, w.hist2, w.uh2);
+    else
+        mfma_window_body<float, NT, float>(static_cast<const float*>(w.Phi), w.ldF, w.F, w.K, w.hist, W, W, w.uh,
+                                           w.ldF, blockIdx.x - w.block0);
 }
 
 // out[k] = hist[k * ld + t]: one column of a window history
@@ -1077,8 +1115,8 @@ hipError_t launch_reconstruct_window_multi(int prec, int window, const WindowSce
     if (prec != PSF_PHI_FP32 && prec != PSF_PHI_FP32_ACC32) return hipErrorInvalidValue;
     if (window != 2 && window != 4 && window != 8 && window != 16 && window != 32) return hipErrorInvalidValue;
     if (prec == PSF_PHI_FP32_ACC32) {
-        if (window == 32) k_reconstruct_mfma_multi<2, float><<<blocks, 256, 0, s>>>(d_tab, n, window);
-        else k_reconstruct_mfma_multi<1, float><<<blocks, 256, 0, s>>>(d_tab, n, window);
+        if (window == 32) k_reconstruct_mfma_pairs<2><<<blocks, 256, 0, s>>>(d_tab, n, window);
+        else k_reconstruct_mfma_pairs<1><<<blocks, 256, 0, s>>>(d_tab, n, window);
     } else {
         if (window == 32) k_reconstruct_mfma_multi<2, double><<<blocks, 256, 0, s>>>(d_tab, n, window);
         else k_reconstruct_mfma_multi<1, double><<<blocks, 256, 0, s>>>(d_tab, n, window);

@@ -58,7 +58,10 @@ struct WindowScene {
     int K;
     const double* hist;         // [K][window]
     float* uh;                  // [window][ldF]
-    uint32_t block0;            // first block of this scene in the launch
+    uint32_t block0;            // first block of this entry in the launch
+    int K2;                     // a second scene on the same Phi with K2 <= K modes (0: none; FP32 sums only)
+    const double* hist2;
+    float* uh2;
 };
 
 // one scene's share of a batched multi-frame particle pass

@@ -615,9 +615,6 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
             nsteps[j] = static_cast<int>(std::min<uint32_t>(static_cast<uint32_t>(W), r->nframes - r->sched));
             hist[j] = q == 0 ? s->dev.hist : r->hist2;
             uh[j] = q == 0 ? s->dev.u_hist : r->u_hist2;
-            const Basis& b = sv->bases[r->basis];
-            sv->h_wtab[q][j] = WindowScene{b.Phi, s->dev.ldF, s->F, static_cast<int>(s->d.K), hist[j], uh[j], wblocks};
-            wblocks += window_blocks(W, s->dev.ldF);
             TraceScene t{};
             t.P = scene_trace_params(s);
             t.u_hist = uh[j];
@@ -629,6 +626,31 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
             t.block0 = tblocks;
             sv->h_ttab[q][j] = t;
             tblocks += (s->n_particles + 255) / 256;
+        }
+        // velocity fields: with FP32 sums the scenes of one basis go in pairs
+        // (by K, largest first), each pair one pass over the basis's Phi
+        int nw = 0;
+        {
+            std::vector<std::vector<int>> by_basis(sv->bases.size());
+            for (int j = 0; j < n; ++j) by_basis[run[j]->basis].push_back(j);
+            const size_t step = sv->cfg.u_fp32_sums ? 2 : 1;
+            for (std::vector<int>& js : by_basis) {
+                std::stable_sort(js.begin(), js.end(), [&](int a, int c) { return sc[a]->d.K > sc[c]->d.K; });
+                for (size_t x = 0; x < js.size(); x += step) {
+                    const int a = js[x];
+                    psf_scene* s = sc[a];
+                    WindowScene w{sv->bases[run[a]->basis].Phi, s->dev.ldF, s->F, static_cast<int>(s->d.K), hist[a],
+                                  uh[a], wblocks, 0, nullptr, nullptr};
+                    if (step == 2 && x + 1 < js.size()) {
+                        const int c = js[x + 1];
+                        w.K2 = static_cast<int>(sc[c]->d.K);
+                        w.hist2 = hist[c];
+                        w.uh2 = uh[c];
+                    }
+                    sv->h_wtab[q][nw++] = w;
+                    wblocks += window_blocks(W, s->dev.ldF);
+                }
+            }
         }
         // the round two back used this parity's history buffers: P waits for
         // its reconstruction (omega history), R for its particles (u history)
@@ -670,9 +692,9 @@ int psf_server_round(psf_server* sv, uint32_t* completed)
         }
         SRV_HIPCHK(ctx, hipEventRecord(sv->ev_step[q], sa));
         SRV_HIPCHK(ctx, hipStreamWaitEvent(sr, sv->ev_step[q], 0));
-        SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_wtab[q], sv->h_wtab[q], sizeof(WindowScene) * n, hipMemcpyHostToDevice, sr));
+        SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_wtab[q], sv->h_wtab[q], sizeof(WindowScene) * nw, hipMemcpyHostToDevice, sr));
         SRV_HIPCHK(ctx, launch_reconstruct_window_multi(sv->cfg.u_fp32_sums ? PSF_PHI_FP32_ACC32 : PSF_PHI_FP32, W,
-                                                        sv->d_wtab[q], n, wblocks, sr));
+                                                        sv->d_wtab[q], nw, wblocks, sr));
         SRV_HIPCHK(ctx, hipEventRecord(sv->ev_rec[q], sr));
         SRV_HIPCHK(ctx, hipStreamWaitEvent(sb, sv->ev_rec[q], 0));
         SRV_HIPCHK(ctx, hipMemcpyAsync(sv->d_ttab[q], sv->h_ttab[q], sizeof(TraceScene) * n, hipMemcpyHostToDevice, sb));
