@@ -72,6 +72,30 @@ def test_alone_equals_mix_bitwise(hip_ctx, u_fp32_sums, shared_basis):
 
 
 @pytest.mark.parametrize("shared_basis", [False, True])
+def test_paired_fields_alone_equals_mix_bitwise(hip_ctx, shared_basis):
+    """With FP32 sums the server computes the velocity fields of two scenes on
+    one basis in one pass over Phi (paired by K).  Mode counts that are not
+    multiples of the MFMA's k-step of 4 and the two-tile window (32) cover the
+    partial step and the second scene's zero modes; each result must keep the
+    bits it has alone."""
+    _say(f"server paired velocity fields, window 32, shared_basis={shared_basis}")
+    sd = make_scene(n=8, K=K0)
+    rng = np.random.default_rng(7)
+    reqs = []
+    for j, (K, frames, N) in enumerate([(23, 40, 600), (17, 70, 800), (9, 33, 300), (24, 65, 500), (13, 50, 257)]):
+        w = rng.standard_normal(K)
+        w /= np.linalg.norm(w)
+        reqs.append((K, frames, w, particles_uniform(sd, N, seed=200 + j, lifetime=25.0)))
+    sv, bid = _server(hip_ctx, sd, window=32, u_fp32_sums=True, shared_basis=shared_basis)
+    mix = _run(sv, bid, reqs)
+    for j, rq in enumerate(reqs):
+        alone = _run(sv, bid, [rq])[0]
+        assert np.array_equal(alone[0].view(np.uint64), mix[j][0].view(np.uint64)), j
+        assert np.array_equal(alone[1].view(np.uint32), mix[j][1].view(np.uint32)), j
+    sv.close()
+
+
+@pytest.mark.parametrize("shared_basis", [False, True])
 @pytest.mark.parametrize("u_fp32_sums", [False, True])
 def test_server_matches_oracle(hip_ctx, u_fp32_sums, shared_basis):
     """omega within the BF16 hi+lo budget of the FP64 oracle; particles within
