@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define PSF_ABI_VERSION 3
+#define PSF_ABI_VERSION 4
 
 #if defined(__GNUC__)
 #define PSF_API __attribute__((visibility("default")))
@@ -218,6 +218,20 @@ typedef struct psf_scene_counters {
 } psf_scene_counters;
 PSF_API int psf_scene_get_counters(psf_scene* scene, psf_scene_counters* out);
 
+/* Kernel time inside the last timed frame region of psf_time_phase (phase
+   11; no counterpart in the reference; for measurement): every persistent
+   reduced-step launch, velocity-field launch and particle-pass launch of the
+   region records its own start and end (the dispatch's timestamps, no extra
+   packets between the kernels), and the durations are summed by kind.  The
+   plain path's reduced steps are not timed.  units: reduced steps, velocity
+   fields, particle frames. */
+typedef struct psf_region_kernels {
+    double step_ms, recon_ms, particle_ms;
+    uint32_t step_launches, recon_launches, particle_launches;
+    uint32_t steps, recon_frames, particle_frames;
+} psf_region_kernels;
+PSF_API int psf_scene_region_kernels(psf_scene* scene, psf_region_kernels* out);
+
 /* Device addresses of a scene's resident buffers, for a caller's own GPU
    code (a renderer drawing the particles or u in place, as the reference's
    OpenCL solver shares its particle buffer with OpenGL,
@@ -363,7 +377,10 @@ PSF_API int psf_density(psf_scene* scene, uint32_t* counts, uint16_t* volume);
  *       10 `count` integrate() frames continuing the current frame window
  *          (as a caller's frame loop runs them; frames computed ahead before
  *          the start event are not in the region), then the deferred
- *          particle passes: count = 1 times one call's frame. */
+ *          particle passes: count = 1 times one call's frame,
+ *       11 phase 3 with every kernel launch of the region between two events
+ *          of its own; psf_scene_region_kernels then reports their sums
+ *          (the events lengthen the region, which phase 3 runs without). */
 PSF_API int psf_time_phase(psf_scene* scene, int phase, int count, double* ms);
 
 /* ---- scene server (continuous batching) ----------------------------------------

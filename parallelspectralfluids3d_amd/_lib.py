@@ -20,6 +20,7 @@ PATH_PLAIN, PATH_PERSISTENT = 0, 1
 PHASE_STEP, PHASE_RECON, PHASE_TRACE, PHASE_FRAME, PHASE_NSTEP, PHASE_WINDOW_RECON = 0, 1, 2, 3, 4, 5
 PHASE_DENSITY_COUNT, PHASE_DENSITY_VOLUME, PHASE_FRAME_DENSITY, PHASE_TRACE_FUSED = 6, 7, 8, 9
 PHASE_FRAME_CONTINUE = 10   # frames continuing the current window (a caller's frame loop)
+PHASE_FRAME_TIMED = 11      # PHASE_FRAME with each launch timed (psf_scene_region_kernels)
 TRACE_FUSE = 8   # frames per fused particle launch of the windowed frame loop
 
 _STATUS = {0: "PSF_OK", -1: "PSF_ERR_ARG", -2: "PSF_ERR_HIP", -3: "PSF_ERR_NOMEM", -4: "PSF_ERR_STATE",
@@ -51,6 +52,12 @@ class SceneCounters(ct.Structure):
                 ("windows_dropped", ct.c_uint64), ("frames_dropped", ct.c_uint64),
                 ("step_launches", ct.c_uint64), ("recon_launches", ct.c_uint64),
                 ("particle_launches", ct.c_uint64)]
+
+
+class RegionKernels(ct.Structure):
+    _fields_ = [("step_ms", ct.c_double), ("recon_ms", ct.c_double), ("particle_ms", ct.c_double),
+                ("step_launches", ct.c_uint32), ("recon_launches", ct.c_uint32), ("particle_launches", ct.c_uint32),
+                ("steps", ct.c_uint32), ("recon_frames", ct.c_uint32), ("particle_frames", ct.c_uint32)]
 
 
 class SceneView(ct.Structure):
@@ -121,6 +128,7 @@ def load():
         "psf_integrate": (i32, [vp, i32]),
         "psf_set_frame_window": (i32, [vp, ct.c_uint32]),
         "psf_scene_get_counters": (i32, [vp, ct.POINTER(SceneCounters)]),
+        "psf_scene_region_kernels": (i32, [vp, ct.POINTER(RegionKernels)]),
         "psf_scene_device_view": (i32, [vp, ct.POINTER(SceneView)]),
         "psf_step_device": (i32, [i32, vp, vp, vp, vp, u32, dbl, i32, i32, vp, vp]),
         "psf_time_phase": (i32, [vp, i32, i32, ct.POINTER(ct.c_double)]),

@@ -221,13 +221,42 @@ int ready_for_step(psf_scene* s)
     return PSF_OK;
 }
 
+// Events of the launches of a timed region (psf_time_phase sets
+// time_kernels): each launch records its own start and end (hipExtLaunchKernel,
+// no marker packets between the kernels); null when not timing.
+hipEvent_t pool_event(psf_scene* s)
+{
+    if (s->ev_used == s->ev_pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        s->ev_pool.push_back(e);
+    }
+    return s->ev_pool[s->ev_used++];
+}
+
+const psf::LaunchEvents* timed_events(psf_scene* s, psf::LaunchEvents& ev)
+{
+    if (!s->time_kernels) return nullptr;
+    ev.start = pool_event(s);
+    ev.stop = pool_event(s);
+    return ev.start && ev.stop ? &ev : nullptr;
+}
+
+void timed_done(psf_scene* s, int kind, uint32_t units, const psf::LaunchEvents* ev)
+{
+    if (ev) s->timed.push_back(psf_scene::TimedLaunch{kind, units, ev->start, ev->stop});
+}
+
 int enqueue_steps(psf_scene* s, int n, double* hist = nullptr, int hist_ld = 0, hipStream_t st = nullptr)
 {
     const int K = static_cast<int>(s->d.K);
     if (!st) st = s->ctx->stream;
     if (s->path == PSF_PATH_PERSISTENT) {
-        hipError_t e = launch_persistent_steps(s, n, st, hist, hist_ld);
+        psf::LaunchEvents ev;
+        const psf::LaunchEvents* pe = n > 0 ? timed_events(s, ev) : nullptr;
+        hipError_t e = launch_persistent_steps(s, n, st, hist, hist_ld, pe);
         if (e != hipSuccess) return set_err(s->ctx, PSF_ERR_HIP, "persistent step launch: %s", hipGetErrorString(e));
+        timed_done(s, 0, static_cast<uint32_t>(n), pe);
         if (n > 0) s->cnt.step_launches++;
         return PSF_OK;
     }
@@ -254,8 +283,11 @@ float* current_u(psf_scene* s)
 
 int enqueue_reconstruct(psf_scene* s)
 {
+    psf::LaunchEvents ev;
+    const psf::LaunchEvents* pe = timed_events(s, ev);
     HIPCHK(s->ctx, launch_reconstruct(s->d.phi_precision, s->dev.Phi, s->dev.ldF, s->F, static_cast<int>(s->d.K),
-                                      s->dev.omega, s->dev.u, s->ctx->stream));
+                                      s->dev.omega, s->dev.u, s->ctx->stream, pe));
+    timed_done(s, 1, 1, pe);
     s->cnt.recon_launches++;
     return PSF_OK;
 }
@@ -268,9 +300,12 @@ int flush_traces(psf_scene* s)
     TraceParams p = trace_params(s);
     DensityGrid D;
     if (int rc = density_for_trace(s, D)) return rc;
+    psf::LaunchEvents ev;
+    const psf::LaunchEvents* pe = timed_events(s, ev);
     HIPCHK(s->ctx, launch_trace_frames(p, win_u(s, s->win_buf) + size_t(s->trace_c0) * s->dev.ldF, s->dev.ldF,
                                        s->trace_pending, s->dev.parts, s->dev.pid, s->n_particles, D,
-                                       s->ctx->stream));
+                                       s->ctx->stream, pe));
+    timed_done(s, 2, s->trace_pending, pe);
     s->cnt.particle_launches++;
     s->frame += s->trace_pending;
     s->trace_pending = 0;
@@ -283,7 +318,11 @@ int enqueue_trace(psf_scene* s, int32_t* dcells)
     TraceParams p = trace_params(s);
     DensityGrid D;
     if (int rc = density_for_trace(s, D)) return rc;
-    HIPCHK(s->ctx, launch_trace(p, current_u(s), s->dev.parts, s->dev.pid, s->n_particles, dcells, D, s->ctx->stream));
+    psf::LaunchEvents ev;
+    const psf::LaunchEvents* pe = timed_events(s, ev);
+    HIPCHK(s->ctx, launch_trace(p, current_u(s), s->dev.parts, s->dev.pid, s->n_particles, dcells, D, s->ctx->stream,
+                                pe));
+    timed_done(s, 2, 1, pe);
     s->cnt.particle_launches++;
     s->frame++;
     return PSF_OK;
@@ -296,8 +335,11 @@ int enqueue_window_fill(psf_scene* s, uint32_t b, hipStream_t st, uint32_t n)
     const int W = static_cast<int>(s->window);
     int rc = enqueue_steps(s, static_cast<int>(n), win_hist(s, b), W, st);
     if (rc) return rc;
+    psf::LaunchEvents ev;
+    const psf::LaunchEvents* pe = timed_events(s, ev);
     HIPCHK(s->ctx, launch_reconstruct_window(s->d.phi_precision, W, static_cast<int>(n), s->dev.Phi, s->dev.ldF, s->F,
-                                             static_cast<int>(s->d.K), win_hist(s, b), win_u(s, b), st));
+                                             static_cast<int>(s->d.K), win_hist(s, b), win_u(s, b), st, pe));
+    timed_done(s, 1, n, pe);
     s->cnt.recon_launches++;
     s->cnt.window_fills++;
     s->cnt.window_frames += n;
@@ -688,6 +730,7 @@ void psf::scene_release(psf_scene* s)
     sfree(s, s->dev.hist_b); sfree(s, s->dev.u_hist_b);
     density_release(s);
     release_side(s);
+    for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     s->ctx->scenes.erase(s);
     delete host_of(s);
     delete s;
@@ -1088,9 +1131,20 @@ int psf_scene_get_counters(psf_scene* s, psf_scene_counters* out)
     return PSF_OK;
 }
 
+int psf_scene_region_kernels(psf_scene* s, psf_region_kernels* out)
+{
+    if (!s || !out) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "null argument");
+    *out = s->region;
+    return PSF_OK;
+}
+
 int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
 {
-    if (!s || !ms || count <= 0 || phase < 0 || phase > 10) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    if (!s || !ms || count <= 0 || phase < 0 || phase > 11) return set_err(s ? s->ctx : nullptr, PSF_ERR_ARG, "bad argument");
+    // phase 11 is phase 3 with every launch of the region timed by its own
+    // events (they lengthen the region by about a tenth, so phase 3 has none)
+    const bool timed = phase == 11;
+    if (timed) phase = 3;
     if (phase == 9 && (s->window < kTraceFuse || !s->has_phi))
         return set_err(s->ctx, PSF_ERR_STATE, "phase 9 needs Phi and a frame window of at least %u", kTraceFuse);
     if (phase == 5 && s->window <= 1) return set_err(s->ctx, PSF_ERR_STATE, "phase 5 needs a frame window > 1");
@@ -1117,6 +1171,12 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
         if (s->side) HIPCHK(ctx, hipStreamSynchronize(s->side));
         s->bounded = true;
         s->fill_budget = static_cast<uint32_t>(count);
+    }
+    if (timed) {
+        // every kernel launched inside the region is timed on its stream
+        s->time_kernels = true;
+        s->timed.clear();
+        s->ev_used = 0;
     }
     hipEvent_t a, b;
     HIPCHK(ctx, hipEventCreate(&a));
@@ -1178,6 +1238,36 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
         if (e != hipSuccess) rc = set_err(ctx, PSF_ERR_HIP, "event timing: %s", hipGetErrorString(e));
         *ms = t;
     }
+    if (rc == PSF_OK && s->time_kernels) {
+        // the region's launches are complete: the end event follows them on
+        // the context stream, which waited for the side stream's window
+        psf_region_kernels r{};
+        for (const psf_scene::TimedLaunch& l : s->timed) {
+            float lt = 0.0f;
+            e = hipEventSynchronize(l.b);
+            if (e == hipSuccess) e = hipEventElapsedTime(&lt, l.a, l.b);
+            if (e != hipSuccess) {
+                rc = set_err(ctx, PSF_ERR_HIP, "kernel timing: %s", hipGetErrorString(e));
+                break;
+            }
+            if (l.kind == 0) {
+                r.step_ms += lt;
+                r.step_launches++;
+                r.steps += l.units;
+            } else if (l.kind == 1) {
+                r.recon_ms += lt;
+                r.recon_launches++;
+                r.recon_frames += l.units;
+            } else {
+                r.particle_ms += lt;
+                r.particle_launches++;
+                r.particle_frames += l.units;
+            }
+        }
+        s->region = r;
+    }
+    s->time_kernels = false;
+    s->timed.clear();
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (rc == PSF_OK && (phase == 0 || phase == 3 || phase == 4 || phase == 8 || phase == 10)) rc = check_status(s);

@@ -5,12 +5,20 @@
 #include <cstdint>
 #include <set>
 #include <string>
+#include <vector>
 
 #include "psf/psf.h"
 
 namespace psf {
 
 constexpr int kWave = 64;
+
+// Optional events a launch records itself (hipExtLaunchKernel: the dispatch's
+// own start and end, no marker packets between kernels), for timing a launch
+// inside a timed region.  Null: an ordinary launch.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
 
 // Device-side state of one scene.  All arrays live in HBM and stay resident
 // between calls; nothing is re-uploaded per frame.
@@ -95,6 +103,18 @@ struct psf_scene {
     bool bounded = false;
     uint32_t fill_budget = 0;
     psf_scene_counters cnt{};    // psf_scene_get_counters
+    // kernels of a timed region (psf_time_phase): an event pair around each
+    // step, velocity-field and particle launch, summed by kind at the end
+    struct TimedLaunch {
+        int kind;                // 0 steps, 1 velocity fields, 2 particle frames
+        uint32_t units;
+        hipEvent_t a, b;
+    };
+    bool time_kernels = false;
+    std::vector<TimedLaunch> timed;
+    std::vector<hipEvent_t> ev_pool;   // events of timed launches, reused across regions
+    size_t ev_used = 0;
+    psf_region_kernels region{};       // psf_scene_region_kernels
     hipStream_t side = nullptr;
     hipEvent_t ev_ready[2] = {nullptr, nullptr};   // side stream: buffer filled
     hipEvent_t ev_fork = nullptr;                  // context stream position the side stream starts from

@@ -8,6 +8,8 @@
 // (paths relative to the reference repository root)
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "psf_kernels.hpp"
 
 namespace psf {
@@ -1068,24 +1070,33 @@ hipError_t launch_update(double* omega, const double* part, const double* visc, 
     return hipGetLastError();
 }
 
+// kernel<<<grid, block, 0, s>>>(args...), or with ev the same launch through
+// hipExtLaunchKernelGGL, which records ev's events at the dispatch's start and end
+#define PSF_LAUNCH(kernel, grid, block, s, ev, ...)                                                        \
+    do {                                                                                                   \
+        if (ev) hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s, (ev)->start, (ev)->stop, 0,   \
+                                      __VA_ARGS__);                                                        \
+        else hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s, __VA_ARGS__);                       \
+    } while (0)
+
 hipError_t launch_reconstruct(int prec, const void* Phi, size_t ldF, uint32_t F, int K, const double* omega,
-                              float* u, hipStream_t s)
+                              float* u, hipStream_t s, const LaunchEvents* ev)
 {
     if (prec == PSF_PHI_FP32_ACC32)   // one column of the FP32-accumulating window kernel (the same bits)
-        return launch_reconstruct_window(prec, 1, 1, Phi, ldF, F, K, omega, u, s);
+        return launch_reconstruct_window(prec, 1, 1, Phi, ldF, F, K, omega, u, s, ev);
     if (prec == PSF_PHI_FP32) {
         constexpr int V = kGemvVecF32;
         const unsigned grid = static_cast<unsigned>((ldF + 256 * V - 1) / (256 * V));
-        k_reconstruct<float, V><<<grid, 256, 0, s>>>(static_cast<const float*>(Phi), ldF, F, K, omega, u);
+        PSF_LAUNCH((k_reconstruct<float, V>), grid, 256, s, ev, static_cast<const float*>(Phi), ldF, F, K, omega, u);
     } else {
         const unsigned grid = static_cast<unsigned>((ldF + 256 * 2 - 1) / (256 * 2));
-        k_reconstruct<double, 2><<<grid, 256, 0, s>>>(static_cast<const double*>(Phi), ldF, F, K, omega, u);
+        PSF_LAUNCH((k_reconstruct<double, 2>), grid, 256, s, ev, static_cast<const double*>(Phi), ldF, F, K, omega, u);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_reconstruct_window(int prec, int hist_ld, int ncols, const void* Phi, size_t ldF, uint32_t F, int K,
-                                     const double* hist, float* uh, hipStream_t s)
+                                     const double* hist, float* uh, hipStream_t s, const LaunchEvents* ev)
 {
     if (ldF % 256 != 0) return hipErrorInvalidValue;
     if (ncols < 1 || ncols > 32 || hist_ld < ncols) return hipErrorInvalidValue;
@@ -1093,17 +1104,21 @@ hipError_t launch_reconstruct_window(int prec, int hist_ld, int ncols, const voi
     if (prec == PSF_PHI_FP32_ACC32) {
         const float* P = static_cast<const float*>(Phi);
         if (ncols > 16)
-            k_reconstruct_mfma<float, 2, float><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
+            PSF_LAUNCH((k_reconstruct_mfma<float, 2, float>), grid, 256, s, ev, P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
         else
-            k_reconstruct_mfma<float, 1, float><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
+            PSF_LAUNCH((k_reconstruct_mfma<float, 1, float>), grid, 256, s, ev, P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
     } else if (prec == PSF_PHI_FP32) {
         const float* P = static_cast<const float*>(Phi);
-        if (ncols > 16) k_reconstruct_mfma<float, 2><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
-        else k_reconstruct_mfma<float, 1><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
+        if (ncols > 16)
+            PSF_LAUNCH((k_reconstruct_mfma<float, 2>), grid, 256, s, ev, P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
+        else
+            PSF_LAUNCH((k_reconstruct_mfma<float, 1>), grid, 256, s, ev, P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
     } else {
         const double* P = static_cast<const double*>(Phi);
-        if (ncols > 16) k_reconstruct_mfma<double, 2><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
-        else k_reconstruct_mfma<double, 1><<<grid, 256, 0, s>>>(P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
+        if (ncols > 16)
+            PSF_LAUNCH((k_reconstruct_mfma<double, 2>), grid, 256, s, ev, P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
+        else
+            PSF_LAUNCH((k_reconstruct_mfma<double, 1>), grid, 256, s, ev, P, ldF, F, K, hist, hist_ld, ncols, uh, ldF);
     }
     return hipGetLastError();
 }
@@ -1145,26 +1160,27 @@ hipError_t launch_gather_column(const double* hist, int ld, int t, int K, double
 }
 
 hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, const uint32_t* pid, uint32_t n,
-                        int32_t* cells, const DensityGrid& D, hipStream_t s)
+                        int32_t* cells, const DensityGrid& D, hipStream_t s, const LaunchEvents* ev)
 {
     if (n == 0) return hipSuccess;
     const uint32_t per = xcd_per((n + 255) / 256);
     if (p.pow2)
-        k_trace<true><<<per * kXcds, 256, 0, s>>>(p, u, parts, pid, n, cells, D, per);
+        PSF_LAUNCH(k_trace<true>, per * kXcds, 256, s, ev, p, u, parts, pid, n, cells, D, per);
     else
-        k_trace<false><<<per * kXcds, 256, 0, s>>>(p, u, parts, pid, n, cells, D, per);
+        PSF_LAUNCH(k_trace<false>, per * kXcds, 256, s, ev, p, u, parts, pid, n, cells, D, per);
     return hipGetLastError();
 }
 
 hipError_t launch_trace_frames(const TraceParams& p, const float* u0, size_t ldU, uint32_t nframes, float4* parts,
-                               const uint32_t* pid, uint32_t n, const DensityGrid& D, hipStream_t s)
+                               const uint32_t* pid, uint32_t n, const DensityGrid& D, hipStream_t s,
+                               const LaunchEvents* ev)
 {
     if (n == 0 || nframes == 0) return hipSuccess;
     const uint32_t per = xcd_per((n + 255) / 256);
     if (p.pow2)
-        k_trace_frames<true><<<per * kXcds, 256, 0, s>>>(p, u0, ldU, nframes, parts, pid, n, D, per);
+        PSF_LAUNCH(k_trace_frames<true>, per * kXcds, 256, s, ev, p, u0, ldU, nframes, parts, pid, n, D, per);
     else
-        k_trace_frames<false><<<per * kXcds, 256, 0, s>>>(p, u0, ldU, nframes, parts, pid, n, D, per);
+        PSF_LAUNCH(k_trace_frames<false>, per * kXcds, 256, s, ev, p, u0, ldU, nframes, parts, pid, n, D, per);
     return hipGetLastError();
 }
 

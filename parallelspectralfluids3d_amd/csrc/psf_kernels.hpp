@@ -87,11 +87,11 @@ hipError_t launch_update(double* omega, const double* part, const double* visc, 
                          hipStream_t s);
 // u = float(Phi w)
 hipError_t launch_reconstruct(int phi_precision, const void* Phi, size_t ldF, uint32_t F, int K,
-                              const double* omega, float* u, hipStream_t s);
+                              const double* omega, float* u, hipStream_t s, const LaunchEvents* ev = nullptr);
 // u_t = float(Phi w_t) for t < ncols (1..32), w_t = hist[k * hist_ld + t],
 // u_t at uh + t * ldF; bitwise equal to launch_reconstruct for each w_t
 hipError_t launch_reconstruct_window(int phi_precision, int hist_ld, int ncols, const void* Phi, size_t ldF, uint32_t F,
-                                     int K, const double* hist, float* uh, hipStream_t s);
+                                     int K, const double* hist, float* uh, hipStream_t s, const LaunchEvents* ev = nullptr);
 // all scenes of d_tab (device memory, ascending block0) in one launch of
 // `blocks` blocks; window_blocks() gives one scene's block count
 hipError_t launch_reconstruct_window_multi(int phi_precision, int window, const WindowScene* d_tab, int n,
@@ -104,11 +104,11 @@ hipError_t launch_gather_column(const double* hist, int ld, int t, int K, double
 // one particle pass; with D.counts set, each particle's new position is also
 // counted into D's grid
 hipError_t launch_trace(const TraceParams& p, const float* u, float4* parts, const uint32_t* pid, uint32_t n,
-                        int32_t* cells, const DensityGrid& D, hipStream_t s);
+                        int32_t* cells, const DensityGrid& D, hipStream_t s, const LaunchEvents* ev = nullptr);
 // nframes particle passes in one launch, frame t with u0 + t * ldU and respawn
 // frame p.frame + t; D counts the positions after the last
 hipError_t launch_trace_frames(const TraceParams& p, const float* u0, size_t ldU, uint32_t nframes, float4* parts,
-                               const uint32_t* pid, uint32_t n, const DensityGrid& D, hipStream_t s);
+                               const uint32_t* pid, uint32_t n, const DensityGrid& D, hipStream_t s, const LaunchEvents* ev = nullptr);
 // counts of the n stored particles into D's grid (added to what it holds)
 hipError_t launch_density_count(const DensityGrid& D, const float4* parts, uint32_t n, hipStream_t s);
 // volume[v] = half(float(counts[v]) / 1e6) and counts[v] = 0, for v < n2048 * 2048

@@ -51,6 +51,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "psf_persistent.hpp"
 #include "psf_wave.hpp"
 
@@ -771,14 +773,17 @@ static PArgs make_pargs(psf_scene* s, int nsteps, double* hist, int hist_ld)
     return a;
 }
 
-hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream, double* hist, int hist_ld)
+hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t stream, double* hist, int hist_ld,
+                                   const LaunchEvents* ev)
 {
     PersistentState* p = st(s);
     if (!p) return hipErrorNotSupported;
     if (nsteps <= 0) return hipSuccess;
     PArgs a = make_pargs(s, nsteps, hist, hist_ld);
     void* args[] = {&a};
-    hipError_t e = hipLaunchKernel(p->fn, dim3(p->G), dim3(64 * kSceneWaves), args, 0, stream);
+    hipError_t e = ev ? hipExtLaunchKernel(p->fn, dim3(p->G), dim3(64 * kSceneWaves), args, 0, stream, ev->start,
+                                           ev->stop, 0)
+                      : hipLaunchKernel(p->fn, dim3(p->G), dim3(64 * kSceneWaves), args, 0, stream);
     if (e == hipSuccess) p->tag += static_cast<uint32_t>(nsteps);
     return e;
 }

@@ -20,7 +20,7 @@ int persistent_upload_c(psf_scene* s, const double* C);
 int persistent_set_wait_ticks(psf_scene* s, uint64_t ticks);
 // hist != nullptr also stores w after step j at hist[k * hist_ld + j]
 hipError_t launch_persistent_steps(psf_scene* s, int nsteps, hipStream_t st, double* hist = nullptr,
-                                   int hist_ld = 0);
+                                   int hist_ld = 0, const LaunchEvents* ev = nullptr);
 
 // Several scenes (all BF16 / BF16 hi+lo, or all FP16) in one launch; scene j
 // runs nsteps[j] steps.  The tables go through pinned staging buffers that

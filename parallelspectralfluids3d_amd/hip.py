@@ -331,6 +331,14 @@ class Scene:
         self._c(self.ctx.lib.psf_scene_get_counters(self.h, ct.byref(c)), "get_counters")
         return {name: int(getattr(c, name)) for name, _ in L.SceneCounters._fields_}
 
+    def region_kernels(self) -> dict:
+        """Kernel time by kind inside the last PHASE_FRAME_TIMED region, each
+        launch bracketed by events on its own stream (psf_scene_region_kernels)."""
+        r = L.RegionKernels()
+        self._c(self.ctx.lib.psf_scene_region_kernels(self.h, ct.byref(r)), "region_kernels")
+        return {name: (float(getattr(r, name)) if name.endswith("_ms") else int(getattr(r, name)))
+                for name, _ in L.RegionKernels._fields_}
+
     def device_view(self) -> dict:
         """Device addresses of the scene's buffers (psf_scene_device_view)."""
         v = L.SceneView()

@@ -15,7 +15,7 @@ def test_exports_every_declared_symbol():
     assert len(names) >= 24
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.psf_abi_version() == 3
+    assert lib.psf_abi_version() == 4
 
 
 def test_footprint_without_gpu():

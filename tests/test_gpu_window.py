@@ -198,7 +198,8 @@ def test_bounded_frame_phase(hip_ctx):
     """The bench's frame phase (psf_time_phase phase 3) starts at a window
     boundary and computes exactly the frames it times: with a window of 32,
     20 timed frames fill 20 frames (none ahead), and the results are the bits
-    of 20 single frames; a caller's frames afterwards continue as usual."""
+    of 20 single frames; a caller's frames afterwards continue as usual.
+    Phase 11 is the same region with each launch timed by events of its own."""
     _say("bounded frame phase")
     from parallelspectralfluids3d_amd import _lib as L
     sd = make_scene(n=8, K=16)
@@ -209,10 +210,27 @@ def test_bounded_frame_phase(hip_ctx):
     win.time_phase(L.PHASE_FRAME, 20)
     one.integrate(20)
     _same(one, win)
+    rk = win.region_kernels()             # phase 3 times no launch of its own
+    assert rk["recon_launches"] == rk["particle_launches"] == rk["step_launches"] == 0
     c1 = win.counters()
     assert c1["windows_dropped"] - c0["windows_dropped"] == 1       # the frames ahead of frame 5
     assert c1["window_frames"] - c0["window_frames"] == 20          # exactly the timed frames
     assert c1["frames"] - c0["frames"] == 20
+    # the same region with its launches timed by their own dispatch events:
+    # one window reconstruction of 20 fields, particle passes of 8 + 8 + 4
+    # frames (the plain path's reduced steps are not timed), same bits
+    region_ms = win.time_phase(L.PHASE_FRAME_TIMED, 20)
+    one.integrate(20)
+    _same(one, win)
+    rk = win.region_kernels()
+    assert (rk["step_launches"], rk["steps"]) == (0, 0)
+    assert (rk["recon_launches"], rk["recon_frames"]) == (1, 20)
+    assert (rk["particle_launches"], rk["particle_frames"]) == (3, 20)
+    assert rk["recon_ms"] > 0 and rk["particle_ms"] > 0
+    assert rk["recon_ms"] + rk["particle_ms"] <= region_ms * 1.001
+    c1 = win.counters()
+    assert c1["window_frames"] - c0["window_frames"] == 40
+    assert c1["frames"] - c0["frames"] == 40
     for s in (one, win):
         s.integrate(45)
     _same(one, win)
@@ -220,6 +238,23 @@ def test_bounded_frame_phase(hip_ctx):
     assert c2["windows_dropped"] == c1["windows_dropped"]
     # ordinary frames: one window of 32 at the first frame and the next one ahead
     assert c2["window_fills"] - c1["window_fills"] == 3
+    # on the persistent path the region's one launch of 20 reduced steps is timed too
+    from parallelspectralfluids3d_amd import hip
+    p = hip.Scene(hip_ctx, hip.make_desc(sd.grid, sd.resolution, 16, C_BF16X2, PHI_FP32, 1024, sd.aabb_min, sd.aabb_max))
+    p.upload_basis(sd.Phi, sd.lam, sd.g)
+    p.upload_advection(sd.C)
+    p.set_omega(sd.omega)
+    p.set_params(0.01, 0.0, False, lifetime=60.0)
+    p.set_particles(particles_uniform(sd, 1024, seed=9, lifetime=60.0))
+    p.set_frame_window(32)
+    region_ms = p.time_phase(L.PHASE_FRAME_TIMED, 20)
+    rk = p.region_kernels()
+    assert (rk["step_launches"], rk["steps"]) == (1, 20)
+    assert (rk["recon_launches"], rk["recon_frames"]) == (1, 20)
+    assert (rk["particle_launches"], rk["particle_frames"]) == (3, 20)
+    assert min(rk["step_ms"], rk["recon_ms"], rk["particle_ms"]) > 0
+    assert rk["step_ms"] + rk["recon_ms"] + rk["particle_ms"] <= region_ms * 1.001
+    p.close()
 
 
 def test_unchanged_params_keep_the_window(hip_ctx):
