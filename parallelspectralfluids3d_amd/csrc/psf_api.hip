@@ -953,33 +953,43 @@ int psf_get_omega(psf_scene* s, double* omega)
     return check_status(s);
 }
 
-// Particles are stored sorted by the cell of the half-voxel-shifted grid
-// that contains them (stable in the caller's order; z-major like the voxels).
-// A particle's interpolation reads faces of the voxels around that cell, so
-// for the x component the lanes of a wave share the y and z voxel rows (and
-// likewise for y and z up to one row), and each gather touches few cache
-// lines.  Every interface below speaks the caller's order.
+// Particles are stored sorted, stable in the caller's order.  A particle's
+// interpolation reads twelve face pairs whose rows and planes are fixed by
+// its half voxels on y and z (the M and O indices of both axes), and the
+// texture path serves four consecutive lanes per cycle, one cache line each
+// when they agree.  With at least 32 particles per half-voxel row on average
+// (the y-z half voxels, 4 ny nz rows), they are sorted by (half voxel on z,
+// half voxel on y, shifted cell on x): four consecutive particles then sit a
+// few voxels apart in one row and read one line per gather.  Sparser sets
+// would spread a row's few particles over the whole row, so they are sorted
+// by the cell of the half-voxel-shifted grid (z-major), which keeps
+// consecutive particles within a cell or two.  Every interface below speaks
+// the caller's order.
 }  // extern "C"
 
-// The cell-sorted storage order of n particles in a scene of descriptor d:
-// sorted[j] = xyzw[pid[j]] (a stable counting sort by the cell of the
-// half-voxel-shifted grid, z-major like the voxels).
+// The storage order of n particles in a scene of descriptor d: sorted[j] =
+// xyzw[pid[j]], a stable counting sort by one of the two keys above.
 void psf::sort_particles(const psf_scene_desc& d, const float* xyzw, uint32_t n, float* sorted, uint32_t* pid)
 {
     std::iota(pid, pid + n, 0u);
     std::vector<uint32_t> key(n);
     const double res = d.resolution;
-    const size_t V = size_t(d.nx) * d.ny * d.nz;
+    const size_t rows = size_t(4) * d.ny * d.nz;
+    const bool halves = size_t(n) >= 32 * rows;
+    const size_t V = (halves ? rows : size_t(d.ny) * d.nz) * d.nx;
+    const uint32_t dims[3] = {d.nx, d.ny, d.nz};
     for (uint32_t i = 0; i < n; ++i) {
         uint32_t c[3];
-        const uint32_t dims[3] = {d.nx, d.ny, d.nz};
         for (int a = 0; a < 3; ++a) {
             const float sa = (d.aabb_min[a] + (d.aabb_max[a] - d.aabb_min[a])) + xyzw[4 * size_t(i) + a];
-            const double q = static_cast<double>(sa) / res - 0.5;
+            // the shifted cell (a / h - 1/2), or on y and z the half voxel (2 a / h)
+            const bool half = halves && a > 0;
+            const double q = half ? 2.0 * static_cast<double>(sa) / res : static_cast<double>(sa) / res - 0.5;
+            const long hi = (half ? 2 : 1) * static_cast<long>(dims[a]) - 1;
             long v = std::isfinite(q) ? static_cast<long>(std::max(-1.0, std::min(q, 1.0e9))) : 0;
-            c[a] = static_cast<uint32_t>(std::max(0L, std::min(v, static_cast<long>(dims[a]) - 1)));
+            c[a] = static_cast<uint32_t>(std::max(0L, std::min(v, hi)));
         }
-        key[i] = (c[2] * d.ny + c[1]) * d.nx + c[0];
+        key[i] = (c[2] * (halves ? 2 * d.ny : d.ny) + c[1]) * d.nx + c[0];
     }
     if (V <= std::max<size_t>(size_t(4) * n, size_t(1) << 16)) {
         std::vector<uint32_t> start(V + 1, 0u);
