@@ -47,8 +47,9 @@ FP32_PEAK = 157.3e12
 # (the reconstructions: FP32 Phi and w with FP32 sums, the PHI_PREC below)
 RECON_TRAFFIC_64_256 = 825_229_632          # k_reconstruct_mfma<float, 1, float>: 401,384 KiB read, 3,120 KiB written
 WINDOW_TRAFFIC_64_256_W32 = 927_384_704     # k_reconstruct_mfma<float, 2, float>: 402,905 / 99,840 KiB
-TRACE_TRAFFIC_64_1M = 37_672_448            # k_trace: 10,203 / 16,384 KiB
-TRACE_FRAMES_TRAFFIC_64_1M = 70_158_336     # k_trace_frames (8 frames per launch): 26,065 / 16,384 KiB
+WINDOW_TRAFFIC_64_256_W20 = 888_993_792     # the same kernel over the 20-frame timed region: 402,879 / 62,400 KiB
+TRACE_TRAFFIC_64_1M = 37_605_376            # k_trace: 10,170 / 16,384 KiB
+TRACE_FRAMES_TRAFFIC_64_1M = 69_654_528     # k_trace_frames (8 frames per launch): 25,819 / 16,384 KiB
 
 
 # u = Phi w of the frame loop: FP32 Phi and w with FP32 sums on the FP32 matrix
@@ -864,11 +865,12 @@ def main():
                                       traffic=TRACE_FRAMES_TRAFFIC_64_1M if (n, N) == (64, 1 << 20) else None,
                                       note="algorithmic bytes = particle-steps x 32 B + frames x 4F; the fused "
                                            "launch keeps records in registers across its frames and moves "
-                                           "compulsory_bytes_per_launch. Bound by the vector-memory path, not HBM: 201 "
+                                           "compulsory_bytes_per_launch. Bound by the vector-memory path, not HBM: 204 "
                                            "vector instructions and 12 dependent flux-pair gathers per "
-                                           "particle-step; texture-data unit busy 85% (TD_TD_BUSY_sum per CU), "
-                                           "address unit 73% (TA_BUSY_avr), 66% of wave-cycles waiting on memory "
-                                           "(SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES)")
+                                           "particle-step, 22.7 L1 accesses per gather (the address unit serves "
+                                           "four lanes per cycle); texture-data unit busy 80% (TD_TD_BUSY_sum per "
+                                           "CU), address unit 69% (TA_BUSY_avr), 63% of wave-cycles waiting on "
+                                           "memory (SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES)")
     rec_bytes = F * K * 4 + F * 4 + K * 8
     # (the calibrated traffic constants were measured for the FP32-sum kernels)
     fp64_sums = PHI_PREC != 2
@@ -909,34 +911,55 @@ def main():
                                        note="plus one counter increment per particle (L2 atomics)")
     for r in roof.values():
         r["frac"] = r["achieved"] / r["peak"]
-    # the headline roofline: the kernel with the largest share of the timed
-    # region, at its average launch duration there (HIP events on its stream)
+    # The headline roofline: the kernel with the largest share of the timed
+    # frame region, at its average launch duration there (HIP events around
+    # each launch on its stream, in the replica of the headline region).
     region_ms = timed_region["kernel_ms"]
-    dominant = max(region_ms, key=region_ms.get)
-    if dominant == "k_trace_frames" and rk["particle_launches"] > 0:
+    region_roof = {}
+    if rk["particle_launches"] > 0:
         per_launch_ms = rk["particle_ms"] / rk["particle_launches"]
         frames_per_launch = rk["particle_frames"] / rk["particle_launches"]
         pb = frames_per_launch * (N * 32 + F * 4)
-        head = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9, achieved=pb / (per_launch_ms * 1e-3) / 1e9,
-                    algorithmic_bytes_per_launch=pb, frames_per_launch=frames_per_launch,
-                    launches=rk["particle_launches"], avg_launch_ms=per_launch_ms,
-                    traffic=TRACE_FRAMES_TRAFFIC_64_1M if (n, N) == (64, 1 << 20) else None,
-                    note="k_trace_frames in the timed frame region: the average duration of its launches "
-                         "(8 + 8 + 4 frames for 20 frames), each bracketed by HIP events on the scene's stream "
-                         "in a replica of the headline region (the events would lengthen the headline one); "
-                         "algorithmic bytes = particle-steps x 32 B + frames x 4F per launch. Bound by the "
-                         "vector-memory path, not HBM (texture-data unit 85% busy, address unit 73%, 201 vector "
-                         "instructions and 12 dependent flux-pair gathers per particle-step). roofline_by_kernel "
-                         "has the same kernel timed separately, 25 launches after the latency loop, when the "
-                         "particles have drifted from their cell-sorted storage order")
-        head["frac"] = head["achieved"] / head["peak"]
-        head["kernel"] = dominant
-        head["ms_per_frame"] = rk["particle_ms"] / max(1, rk["particle_frames"])
+        region_roof["k_trace_frames"] = dict(
+            bound="hbm", unit="GB/s", peak=HBM_PEAK / 1e9, achieved=pb / (per_launch_ms * 1e-3) / 1e9,
+            algorithmic_bytes_per_launch=pb, frames_per_launch=frames_per_launch,
+            launches=rk["particle_launches"], avg_launch_ms=per_launch_ms,
+            traffic=TRACE_FRAMES_TRAFFIC_64_1M if (n, N) == (64, 1 << 20) else None,
+            ms_per_frame=rk["particle_ms"] / max(1, rk["particle_frames"]),
+            note="k_trace_frames in the timed frame region (8 + 8 + 4 frames for 20 frames); algorithmic "
+                 "bytes = particle-steps x 32 B + frames x 4F per launch. Bound by the vector-memory path "
+                 "(the texture address unit serves four lanes per cycle, one cache line each) and 201 vector "
+                 "instructions per particle-step, not HBM")
+    if rk["recon_launches"] > 0:
+        per_launch_ms = rk["recon_ms"] / rk["recon_launches"]
+        wr = rk["recon_frames"] / rk["recon_launches"]
+        rbytes = F * K * 4 + wr * (F * 4 + K * 8)
+        rflops = 2.0 * F * K * wr
+        r = dict(unit="GB/s", peak=HBM_PEAK / 1e9, bound="hbm", achieved=rbytes / (per_launch_ms * 1e-3) / 1e9,
+                 algorithmic_bytes_per_launch=rbytes, algorithmic_flops_per_launch=rflops, frames_per_launch=wr,
+                 launches=rk["recon_launches"], avg_launch_ms=per_launch_ms,
+                 ms_per_frame=rk["recon_ms"] / max(1, rk["recon_frames"]),
+                 traffic=WINDOW_TRAFFIC_64_256_W20 if (n, K, wr) == (64, 256, 20) and not fp64_sums else None,
+                 note="k_reconstruct_mfma in the timed frame region: the 20 frames' velocity fields in one "
+                      "pass over Phi; algorithmic bytes = F K 4 (Phi) + frames x (4F + 8K). FP32 matrix cores "
+                      "in 16-frame column tiles (two tiles for 20 frames), so the FP32 MFMA work is that of 32 "
+                      "frames: HBM and the matrix cores bound it together")
+        if fp64_sums:
+            r.update(bound="fp64", unit="TFLOP/s", peak=FP64_PEAK / 1e12,
+                     achieved=rflops / (per_launch_ms * 1e-3) / 1e12)
+        else:
+            r.update(fp32_mfma_tflops=rflops / (per_launch_ms * 1e-3) / 1e12, fp32_mfma_peak_tflops=FP32_PEAK / 1e12)
+        region_roof["k_reconstruct_mfma"] = r
+    for r in region_roof.values():
+        r["frac"] = r["achieved"] / r["peak"]
+    dominant = max(region_ms, key=region_ms.get)
+    if dominant in region_roof:
+        head = dict(region_roof[dominant])
     else:
         dominant = max(kern, key=kern.get)
         head = dict(roof[dominant])
-        head["kernel"] = dominant
         head["ms_per_frame"] = kern[dominant]
+    head["kernel"] = dominant
 
     # ---- reduced steps/s: one persistent launch vs the plain FP32 path -------------------
     nred = args.reduced_steps
@@ -1039,6 +1062,7 @@ def main():
         "kernel_ms_per_frame": kern,
         "phase_ms_one_at_a_time": {"reduced_step": ms_st, "reconstruct": ms_rec, "trace": ms_tr},
         "roofline": head,
+        "roofline_in_timed_region": region_roof,
         "roofline_by_kernel": roof,
         "density": dens,
         "serving": serving,
