@@ -45,8 +45,8 @@ FP32_PEAK = 157.3e12
 # (fetched KiB x 2, the factor a 1 GiB hipMemcpy reads back, plus written
 # KiB); counters cannot be read from inside this process.
 # (the reconstructions: FP32 Phi and w with FP32 sums, the PHI_PREC below)
-RECON_TRAFFIC_64_256 = 825_229_632          # k_reconstruct_mfma<float, 1, float>: 401,384 KiB read, 3,120 KiB written
-WINDOW_TRAFFIC_64_256_W32 = 927_384_704     # k_reconstruct_mfma<float, 2, float>: 402,905 / 99,840 KiB
+RECON_TRAFFIC_64_256 = 825_237_504          # k_reconstruct_mfma<float, 1, float>: 401,388 KiB read, 3,120 KiB written
+WINDOW_TRAFFIC_64_256_W32 = 927_565_824     # k_reconstruct_mfma<float, 2, float>: 402,993 / 99,840 KiB
 WINDOW_TRAFFIC_64_256_W20 = 888_993_792     # the same kernel over the 20-frame timed region: 402,879 / 62,400 KiB
 TRACE_TRAFFIC_64_1M = 37_605_376            # k_trace: 10,170 / 16,384 KiB
 TRACE_FRAMES_TRAFFIC_64_1M = 69_654_528     # k_trace_frames (8 frames per launch): 25,819 / 16,384 KiB
