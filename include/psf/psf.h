@@ -239,7 +239,7 @@ PSF_API int psf_scene_region_kernels(psf_scene* scene, psf_region_kernels* out);
    destroyed; u and the particle records are those of the current frame once
    psf_ctx_sync has returned (deferred particle passes run first).  Layouts
    as for the uploads (C only for FP64/FP32 scenes, else NULL); particle
-   records are stored sorted by cell, parts[j] holding caller particle pid[j]. */
+   records are stored in a spatial order, parts[j] holding caller particle pid[j]. */
 typedef struct psf_scene_view {
     double* omega;               /* K */
     const double* lambda;        /* K */

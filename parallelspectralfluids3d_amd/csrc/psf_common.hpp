@@ -40,7 +40,7 @@ struct DeviceScene {
     // particles
     float4* parts = nullptr;     // max_particles
     int32_t* cells = nullptr;    // 12 * max_particles (debug output of cell lookups)
-    uint32_t* pid = nullptr;     // caller index of each stored particle (storage is sorted by voxel)
+    uint32_t* pid = nullptr;     // caller index of each stored particle (storage is in sort_particles' spatial order)
     // frame window (psf_set_frame_window): w after each step of the window,
     // [K][window], u of each of its frames, [window][ldF], and a K-vector for
     // reading one column back

@@ -21,7 +21,7 @@ struct SceneOptions {
 };
 
 int scene_create_ex(psf_ctx* ctx, const psf_scene_desc* desc, const SceneOptions& opt, psf_scene** out);
-// the cell-sorted storage order of n particles (psf_set_particles): sorted[j] = xyzw[pid[j]]
+// the spatial storage order of n particles (psf_set_particles): sorted[j] = xyzw[pid[j]]
 void sort_particles(const psf_scene_desc& d, const float* xyzw, uint32_t n, float* sorted, uint32_t* pid);
 // Host-side state of a scene whose device arrays the caller fills with its
 // own copies: particles in storage order with caller indices pid, lambda,
