@@ -578,7 +578,7 @@ def config2_bench(args):
 
 
 def obstacle_bench(args):
-    """A 64^3 box with the voxelised central sphere of config 2, K=256, built
+    """A 64^3 box with the voxelised central sphere of the 32^3 obstacle scene, K=256, built
     entirely on the device: the pruned Laplacian's eigenbasis
     (psf_eigensolve_csr, penalty-lifted), C from that basis
     (psf_scene_build_advection), then the windowed frame loop with 2^20
@@ -689,7 +689,7 @@ def main():
                     help="scene cache file of the configuration: read if present, else built and written")
     ap.add_argument("--config2-build-only", action="store_true", help="build the basis, C and forcing (and write the cache), then stop; with --config2-host-eig no GPU work")
     ap.add_argument("--config2-host-eig", action="store_true",
-                    help="config 2: basis (scipy shift-invert) and C on the host instead of the device")
+                    help="the 32^3 sphere scene: basis (scipy shift-invert) and C on the host instead of the device")
     ap.add_argument("--serving-only", action="store_true", help="only the serving section")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rate", type=float, default=200.0)

@@ -1,19 +1,19 @@
-"""The hot path at the benchmark configurations' own sizes, against the oracle
+"""The hot path at the benchmark workloads' own sizes, against the oracle
 (psfsolver.cpp:21-234 restated in oracle/psf_oracle.c).
 
-  config 2  32^3 box with a central sphere obstacle, K = 128 eigenbasis of
+  sphere    32^3 box with a central sphere obstacle, K = 128 eigenbasis of
             the pruned Laplacian, upward force on modes 0-7, FP32 C on the
             plain path, FP32 Phi, 2000 integrate() frames at dt = 0.005,
             2^18 particles in fluid cells; the oracle runs its own literal C
-  config 3  64^3 closed box, K = 256: BF16, FP16 and BF16 hi+lo contractions
+  precision 64^3 closed box, K = 256: BF16, FP16 and BF16 hi+lo contractions
             (FP32 accumulation, FP64 update) for 1000 steps at dt = 0.01,
-            checked every 100 steps against the FP64 oracle at the north-star
-            budget of 1e-4 relative; the FP64 path to 1e-12; one particle pass
+            checked every 100 steps against the FP64 oracle within the
+            precision target of 1e-4 relative; the FP64 path to 1e-12; one particle pass
             of a 2^20-particle scene, sampled, bit for bit
-  config 4  the serving loop on the stream's real grids {16, 24, 32, 48}^3
+  serving   the serving loop on the stream's real grids {16, 24, 32, 48}^3
             with K up to 192: every request gives the same bits alone and in
             the mix, and sampled requests agree with the oracle
-  config 5  128^3 closed box, K = 512 (Phi 13 GB), 2^24 particles: the
+  large     128^3 closed box, K = 512 (Phi 13 GB), 2^24 particles: the
             size-independent checks (omega against the FP64 path, unit energy,
             sampled u against a host FP64 sum, a sampled particle pass bit for
             bit against the oracle)
@@ -35,8 +35,8 @@ NTHREADS = min(16, os.cpu_count() or 1)
 C_FP64, C_FP32, C_BF16, C_FP16, C_BF16X2 = 0, 1, 2, 3, 4
 PHI_FP64, PHI_FP32 = 0, 1
 # relative omega error of the MFMA precisions against the FP64 oracle after
-# 1000 steps at 64^3/K=256 (the north star's budget)
-NORTH_STAR_BUDGET = 1e-4
+# 1000 steps at 64^3/K=256 (the precision target)
+PRECISION_BUDGET = 1e-4
 
 
 def _say(msg):
@@ -60,7 +60,7 @@ def test_config2_obstacle_fp32_2000_frames(hip_ctx):
     from parallelspectralfluids3d_amd import _lib as L
     from parallelspectralfluids3d_amd import hip
     n, K, N, frames, dt = 32, 128, 1 << 18, 2000, 0.005
-    _say(f"config 2: {n}^3 sphere obstacle, K={K}, FP32 plain path, {frames} frames, {N} particles")
+    _say(f"{n}^3 sphere obstacle, K={K}, FP32 plain path, {frames} frames, {N} particles")
     g = Grid(n, n, n)
     h = 1.0 / n
     amin, amax = (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)
@@ -130,7 +130,7 @@ def test_config3_precisions_1000_steps(hip_ctx, box64):
     from parallelspectralfluids3d_amd import hip
     b = box64
     g, h, K = b["g"], b["h"], b["K"]
-    _say(f"config 3: 64^3 box, K={K}, 1000 steps, omega every 100 vs the oracle")
+    _say(f"64^3 box, K={K}, 1000 steps, omega every 100 vs the oracle")
     zero = np.zeros(K)
     ora = []
     w = b["omega"].copy()
@@ -155,7 +155,7 @@ def test_config3_precisions_1000_steps(hip_ctx, box64):
         print(f"  c_precision {prec}: rel err every 100 steps {['%.2e' % x for x in e]}", flush=True)
     assert max(errs[C_FP64]) <= 1e-12
     for prec in (C_BF16X2, C_BF16, C_FP16):
-        assert max(errs[prec]) <= NORTH_STAR_BUDGET, (prec, errs[prec])
+        assert max(errs[prec]) <= PRECISION_BUDGET, (prec, errs[prec])
 
 
 def test_config3_particle_pass_bit_exact(hip_ctx, box64):
@@ -166,7 +166,7 @@ def test_config3_particle_pass_bit_exact(hip_ctx, box64):
     from parallelspectralfluids3d_amd import hip
     b = box64
     g, h, K, N, dt = b["g"], b["h"], b["K"], 1 << 20, 0.01
-    _say("config 3: 64^3 particle pass, sampled, bit-exact")
+    _say("64^3 particle pass, sampled, bit-exact")
     rng = np.random.default_rng(3)
     parts = np.zeros((N, 4), np.float32)
     parts[:, :3] = rng.random((N, 3), dtype=np.float32) - np.float32(0.5)
@@ -208,7 +208,7 @@ def test_config4_serving_real_grids(hip_ctx, shared_basis):
     modes of the basis (BF16 hi+lo C and FP32 Phi against FP64).  With
     shared_basis the scenes of a grid read one copy of its C."""
     from parallelspectralfluids3d_amd import hip
-    _say(f"config 4: serving on {{16,24,32,48}}^3, K up to 192, shared_basis={shared_basis}")
+    _say(f"serving on {{16,24,32,48}}^3, K up to 192, shared_basis={shared_basis}")
     bases = {}
     for n in SERVE_GRIDS:
         g = Grid(n, n, n)
@@ -279,7 +279,7 @@ def test_config5_large_scene(hip_ctx):
     GPU's u."""
     from parallelspectralfluids3d_amd import hip
     n, K, N, frames, dt = 128, 512, 1 << 24, 500, 0.005
-    _say(f"config 5: {n}^3, K={K}, {N} particles, {frames} frames")
+    _say(f"large box: {n}^3, K={K}, {N} particles, {frames} frames")
     g = Grid(n, n, n)
     h = 1.0 / n
     tab = basis.box_mode_table_normalised(g, K)

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 C_BF16, C_FP16, C_BF16X2 = 2, 3, 4
 # relative omega error vs FP64 after 1000 steps at dt = 0.01; BF16 at the
-# north star's 1e-4 (the 64^3/K=256 case is in test_gpu_configs.py)
+# precision target of 1e-4 (the 64^3/K=256 case is in test_gpu_configs.py)
 BUDGET = {C_BF16: 1e-4, C_FP16: 2e-5, C_BF16X2: 2e-6}
 
 
