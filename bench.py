@@ -952,6 +952,10 @@ def main():
         region_roof["k_reconstruct_mfma"] = r
     for r in region_roof.values():
         r["frac"] = r["achieved"] / r["peak"]
+        if rk["recon_launches"] > 1:
+            r["overlap"] = ("the region spans several frame windows: the next window's reduced steps and "
+                            "velocity fields run on the side stream beside the current window's particle "
+                            "passes, so each launch's duration includes the time it shares the GPU")
     dominant = max(region_ms, key=region_ms.get)
     if dominant in region_roof:
         head = dict(region_roof[dominant])
