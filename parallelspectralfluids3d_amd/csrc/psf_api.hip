@@ -1175,10 +1175,19 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
         if (rc0) return rc0;
     }
     if (int rc0 = flush_traces(s)) return rc0;   // deferred passes belong before the timed region
+    // everything queued so far, on either stream, ends before the start event
     if (phase == 3 || phase == 8) {
-        // everything queued so far, on either stream, ends before the start event
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         if (s->side) HIPCHK(ctx, hipStreamSynchronize(s->side));
+    }
+    hipEvent_t a, b;
+    HIPCHK(ctx, hipEventCreate(&a));
+    if (hipError_t eb = hipEventCreate(&b); eb != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return set_err(ctx, PSF_ERR_HIP, "event create: %s", hipGetErrorString(eb));
+    }
+    // (no early return from here on: the flags below are cleared on every path)
+    if (phase == 3 || phase == 8) {
         s->bounded = true;
         s->fill_budget = static_cast<uint32_t>(count);
     }
@@ -1188,9 +1197,6 @@ int psf_time_phase(psf_scene* s, int phase, int count, double* ms)
         s->timed.clear();
         s->ev_used = 0;
     }
-    hipEvent_t a, b;
-    HIPCHK(ctx, hipEventCreate(&a));
-    HIPCHK(ctx, hipEventCreate(&b));
     int rc = PSF_OK;
     hipError_t e = launch_timed_region_mark(0, ctx->stream);
     if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
